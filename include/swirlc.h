@@ -1,0 +1,171 @@
+/*
+ * swirlc.h -- C ABI of the MI355X (gfx950) compressible Navier-Stokes RHS and
+ * Runge-Kutta step that replaces Swirl-C's TensorFlow hot path.
+ *
+ * The reference has no C ABI: its seam is the Python closure
+ *   rhs(replica_id, conservative, helper_vars) -> FlowFieldMap
+ * built by equations/compressible_navier_stokes.py:46-270 and consumed by
+ * numerics/time_integration/time_integration.py:35-90.  Each entry point below
+ * states the reference function it stands in for.  The Python host package
+ * (swirlc_amd) binds these with ctypes and keeps the reference's Python call
+ * surface above them.
+ *
+ * Conventions
+ *   - A field is one fp32 device array shaped [nz+2h][nx+2h][ny+2h] (y
+ *     fastest), the reference's FlowFieldVar layout (common/types.py:31-33).
+ *     Fields are passed as host arrays of device pointers, in the order
+ *     rho, rho_u, rho_v, rho_w, rho_e (common/types.py:72), then transported
+ *     scalars rho_<name>.
+ *   - Every call is asynchronous on the given HIP stream, performs no host
+ *     synchronisation and no allocation, and may be captured in a HIP graph.
+ *   - Errors: a nonzero sc_status; sc_last_error() returns the message of the
+ *     calling thread's last failure.  Invalid configurations are rejected by
+ *     sc_ctx_create, mirroring the reference's trace-time ValueError /
+ *     NotImplementedError.
+ *   - A context must not be used from two host threads at once.
+ */
+#ifndef SWIRLC_H_
+#define SWIRLC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int sc_status;
+#define SC_OK 0
+#define SC_ERR_INVALID 1     /* invalid argument or configuration */
+#define SC_ERR_UNSUPPORTED 2 /* valid in the reference's API, not supported */
+#define SC_ERR_HIP 3         /* HIP runtime error */
+
+#define SC_MAX_FIELDS 7      /* 5 base conserved fields + up to 2 scalars */
+#define SC_MAX_HALO 4
+
+/* equations/compressible_navier_stokes.py:39-43 (InterpolationSchemes). */
+#define SC_WENO_3 0
+#define SC_WENO_5 1
+#define SC_MUSCL 2
+/* numerics/riemann_solver/riemann_solver.py:24-27 (NumericFluxScheme). */
+#define SC_HLL 0
+#define SC_HLLC 1
+/* boundary/bc_types.py:22-27 (BoundaryCondition). */
+#define SC_BC_PERIODIC 0
+#define SC_BC_DIRICHLET 1
+#define SC_BC_NEUMANN 2
+#define SC_BC_REFLECTIVE 3
+/* Runge-Kutta stage update forms (numerics/time_integration/rk3.py:69-83,
+ * rk2.py:67-86), with L the RHS of uk:
+ *   SC_RK_EULER: out = uk + L*dt
+ *   SC_RK_BLEND: out = u0*c0 + (uk + L*dt)*c1
+ *   SC_RK_AVG:   out = 0.5*(u0 + (uk + dt*L))                                 */
+#define SC_RK_EULER 1
+#define SC_RK_BLEND 2
+#define SC_RK_AVG 3
+
+/* One face of one field's cell-average boundary condition
+ * (cfg.bc['cell_averages'][var][dim][face], core/parameter.py:112-130). */
+typedef struct sc_bc_face {
+  int type;                 /* SC_BC_* */
+  float value;              /* DIRICHLET scalar / NEUMANN increment per cell */
+  int n_layer_values;       /* DIRICHLET list of per-layer scalars (0 = none) */
+  float layer_values[SC_MAX_HALO];
+  const float* layer_planes;/* DIRICHLET list of per-layer planes (device, or
+                               NULL): [h][A][B], A,B = full extents of the two
+                               other axes in [z][x][y] order                  */
+} sc_bc_face;
+
+typedef struct sc_grid {
+  int nx, ny, nz;           /* interior cells (core_nx, core_ny, core_nz) */
+  int halo;                 /* halo_width */
+  double dx, dy, dz;
+  /* Cell-centre coordinates incl. halos (host arrays of n+2h values, built as
+   * common/utils.py:41-89 does); only read when the gravity vector is a unit
+   * vector (gravitational potential, generic.py:282-319).  May be NULL
+   * otherwise. */
+  const float* x_full;
+  const float* y_full;
+  const float* z_full;
+} sc_grid;
+
+typedef struct sc_params {
+  int n_fields;             /* 5 + number of transported scalars */
+  int interpolation;        /* SC_WENO_3 | SC_WENO_5 | SC_MUSCL */
+  int numeric_flux;         /* SC_HLL | SC_HLLC */
+  int include_diffusion;
+  double nu, pr;            /* transport_parameters (physics/transport/simple.py) */
+  int gravity_source;       /* 'gravity' in source_functions */
+  double g[3];              /* cfg.g (unit vector or zero) */
+  /* bc[f][d][s]: field f, dim d (0 x, 1 y, 2 z), side s (0 low, 1 high). */
+  sc_bc_face bc[SC_MAX_FIELDS][3][2];
+  /* bit f set: the BC of field f is given on its primitive (e.g. 'u' for
+   * rho_u), boundary.py:516-531. */
+  int prim_bc_mask;
+} sc_params;
+
+typedef struct sc_ctx sc_ctx;
+
+/* Validates cfg and builds the per-grid context (scratch RHS accumulator,
+ * potential-energy tables).  Stands in for get_rhs_fn
+ * (equations/compressible_navier_stokes.py:46-67) and PhysicsModels
+ * (physics/physics_models.py:45-92): invalid scheme names, missing gravity
+ * direction and halo widths too small for the stencil fail here. */
+sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params,
+                        sc_ctx** out);
+sc_status sc_ctx_destroy(sc_ctx* ctx);
+
+/* Fills the halo cells of every field in place from the cell-average BCs.
+ * boundary.update_conservative_cell_averages (boundary/boundary.py:464-539)
+ * with swirl_lm's halo fill: dims x, y, z in order, each over full planes.
+ * Fields with a primitive BC get rho*prim in their halos; their interior is
+ * left untouched (the RHS applies the reference's rho*(q/rho) rounding). */
+sc_status sc_fill_halos(sc_ctx* ctx, float* const* q, void* stream);
+
+/* Right-hand side of the conserved fields (the `rhs` closure,
+ * equations/compressible_navier_stokes.py:69-268): fills q's halos in place,
+ * then writes rhs over the interior cells (halo cells of rhs are untouched). */
+sc_status sc_rhs(sc_ctx* ctx, float* const* q, float* const* rhs, void* stream);
+
+/* One Runge-Kutta stage: fills uk's halos, evaluates L = rhs(uk) and writes
+ * the stage update (SC_RK_* form above) over interior cells of out.
+ * rk3.integrate / rk2.integrate stage bodies (rk3.py:87-119, rk2.py:67-86).
+ * dt_dev points to one float in device memory.  out may alias u0, not uk. */
+sc_status sc_rk_stage(sc_ctx* ctx, int form, float c0, float c1,
+                      const float* dt_dev, const float* const* u0,
+                      float* const* uk, float* const* out, void* stream);
+
+/* Maximum over interior cells of max_d(|u_d| + sqrt(gamma p / rho)) written to
+ * *out_dev, and dt = cfl * min(dx,dy,dz) / max written to *dt_dev (if not
+ * NULL).  No reference counterpart (the reference takes a fixed cfg.dt); the
+ * wave speed uses the reference EOS (generic.py:27-57,149-154). */
+sc_status sc_max_wave_speed(sc_ctx* ctx, const float* const* q, float cfl,
+                            float* out_dev, float* dt_dev, void* stream);
+
+/* Debug/parity: HLLC branch code of every face normal to dim (0 x, 1 y, 2 z),
+ * codes[i] for the face i-1/2 stored at cell i of the full array, interior
+ * faces only (hllc.py:227-243: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R).  q's halos
+ * must already be filled. */
+sc_status sc_hllc_branch_codes(sc_ctx* ctx, const float* const* q, int dim,
+                               uint8_t* codes, void* stream);
+
+/* Profiling: launches only the sweep kernel of direction dim (0 x, 1 y, 2 z)
+ * of an sc_rk_stage call (form 0 = plain RHS into the context's scratch), on
+ * uk whose halos are already filled; the x sweep writes the scratch RHS, the
+ * y sweep updates it, the z sweep finishes it (and writes out for form != 0).
+ * No reference counterpart: it exposes one kernel of the fused stage so
+ * its duration can be timed in isolation. */
+sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
+                        const float* dt_dev, const float* const* u0,
+                        float* const* uk, float* const* out, void* stream);
+
+/* Thread-local message of the last failed call ("" if none). */
+const char* sc_last_error(void);
+
+/* Library version string. */
+const char* sc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWIRLC_H_ */

@@ -1,0 +1,405 @@
+// C ABI of libswirlc_amd (declared in include/swirlc.h).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/swirlc.h"
+#include "sc_halo.hpp"
+#include "sc_launch.hpp"
+
+struct sc_ctx {
+  sc_grid grid;
+  sc_params params;
+  int device;
+  int nf;
+  int N[3];
+  long long cells_full;
+  float h[3];
+  // gravitational potential tables (device): pe = (pe_x[i] - pe_y[j]) - pe_z[k]
+  int has_pe;
+  float* pe_x;
+  float* pe_y;
+  float* pe_z;
+  int grav_mask;
+  // scratch RHS accumulator (device), one full-size array per field
+  float* rhs_scratch[SC_MAX_FIELDS];
+  unsigned* speed_bits;
+  sc::HaloArgs halo;  // BC tables; field pointers filled per call
+  sc::SweepArgs sweep;
+  sc::SweepTuning tune;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+sc_status fail(sc_status code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+sc_status hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return SC_OK;
+  return fail(SC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// common/utils.py:114-142.
+int gravity_direction(const double g[3]) {
+  const double norm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  const double small = 1e-6;  // finfo(float32).resolution
+  if (fabs(norm - 1.0) >= small) return -1;
+  for (int i = 0; i < 3; ++i)
+    if (fabs(fabs(g[i]) - 1.0) < small) return i;
+  return 3;
+}
+
+int radius_of(int scheme) { return scheme == SC_WENO_5 ? 2 : 1; }
+
+void fill_halo_fields(sc_ctx* c, float* const* q) {
+  for (int f = 0; f < c->nf; ++f) c->halo.q[f] = q[f];
+}
+
+void fill_sweep_state(sc_ctx* c, const float* const* q) {
+  for (int f = 0; f < c->nf; ++f) c->sweep.q.p[f] = q[f];
+}
+
+int env_int(const char* name, int dflt) {
+  const char* s = getenv(name);
+  if (s == nullptr || *s == '\0') return dflt;
+  return atoi(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sc_last_error(void) { return g_last_error.c_str(); }
+
+const char* sc_version(void) { return "swirlc_amd 0.1 gfx950"; }
+
+sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** out) {
+  if (grid == nullptr || params == nullptr || out == nullptr)
+    return fail(SC_ERR_INVALID, "sc_ctx_create: null argument");
+  *out = nullptr;
+  const sc_params& p = *params;
+  if (p.interpolation < SC_WENO_3 || p.interpolation > SC_MUSCL)
+    return fail(SC_ERR_UNSUPPORTED,
+                "interpolation_scheme is not implemented. Valid options are: "
+                "['WENO_3', 'WENO_5', 'MUSCL']");
+  if (p.numeric_flux != SC_HLL && p.numeric_flux != SC_HLLC)
+    return fail(SC_ERR_INVALID,
+                "numeric_flux_scheme is not implemented. Valid options are: "
+                "['HLL', 'HLLC']");
+  if (p.n_fields < 5 || p.n_fields > 6)
+    return fail(SC_ERR_UNSUPPORTED, "n_fields must be 5 or 6 (at most one transported scalar)");
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1)
+    return fail(SC_ERR_INVALID, "grid extents must be positive");
+  const int h = grid->halo;
+  if (h < radius_of(p.interpolation) + 1 || h > SC_MAX_HALO)
+    return fail(SC_ERR_INVALID, "halo_width too small for the interpolation stencil"
+                                " (WENO_5 needs 3, WENO_3/MUSCL need 2) or above 4");
+  if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0))
+    return fail(SC_ERR_INVALID, "grid spacing must be positive");
+  const int gdir = gravity_direction(p.g);
+  if (p.gravity_source && gdir == -1)
+    return fail(SC_ERR_INVALID,
+                "Gravity direction must be specified if the gravity source function"
+                " is to be used.");
+  const int n[3] = {grid->nx, grid->ny, grid->nz};
+  for (int f = 0; f < p.n_fields; ++f) {
+    for (int d = 0; d < 3; ++d) {
+      for (int s = 0; s < 2; ++s) {
+        const sc_bc_face& b = p.bc[f][d][s];
+        if (b.type == SC_BC_REFLECTIVE)
+          return fail(SC_ERR_UNSUPPORTED, "Unsupported BC type BoundaryCondition.REFLECTIVE.");
+        if (b.type < 0 || b.type > SC_BC_NEUMANN)
+          return fail(SC_ERR_INVALID, "unknown BC type");
+        if (b.type == SC_BC_PERIODIC && n[d] < h)
+          return fail(SC_ERR_UNSUPPORTED, "PERIODIC BC needs at least halo_width interior cells");
+        if (b.n_layer_values != 0 && b.n_layer_values != h)
+          return fail(SC_ERR_INVALID, "per-layer DIRICHLET values must have halo_width entries");
+      }
+    }
+  }
+  if (p.prim_bc_mask & 1)
+    return fail(SC_ERR_INVALID, "the density BC cannot be given on a primitive");
+  if (gdir != -1 && (grid->x_full == nullptr || grid->y_full == nullptr || grid->z_full == nullptr))
+    return fail(SC_ERR_INVALID, "coordinates are required when the gravity vector is set");
+
+  sc_ctx* c = new sc_ctx();
+  memset(c, 0, sizeof(sc_ctx));
+  c->tune = sc::SweepTuning();
+  c->tune.seg_len = env_int("SWIRLC_SEG_LEN", c->tune.seg_len);
+  c->tune.by_march = env_int("SWIRLC_BY_MARCH", c->tune.by_march);
+  c->tune.by_y = env_int("SWIRLC_BY_Y", c->tune.by_y);
+  c->grid = *grid;
+  c->params = p;
+  c->nf = p.n_fields;
+  hipGetDevice(&c->device);
+  for (int d = 0; d < 3; ++d) c->N[d] = n[d] + 2 * h;
+  c->cells_full = (long long)c->N[0] * c->N[1] * c->N[2];
+  c->h[0] = (float)grid->dx;
+  c->h[1] = (float)grid->dy;
+  c->h[2] = (float)grid->dz;
+
+  sc_status st = SC_OK;
+  auto check = [&](hipError_t e, const char* what) {
+    if (st == SC_OK) st = hip_check(e, what);
+  };
+  c->has_pe = gdir != -1;
+  if (c->has_pe) {
+    // pe = -G g . x as ((-G gx) * x - (G gy) * y) - (G gz) * z in fp32
+    // (generic.py:312-319): tabulate each term per axis.
+    const float* coord[3] = {grid->x_full, grid->y_full, grid->z_full};
+    float** dst[3] = {&c->pe_x, &c->pe_y, &c->pe_z};
+    for (int d = 0; d < 3; ++d) {
+      const double gv = 9.81 * p.g[d];
+      const float coef = d == 0 ? (float)(-gv) : (float)gv;
+      float* tmp = (float*)malloc(sizeof(float) * c->N[d]);
+      for (int i = 0; i < c->N[d]; ++i) tmp[i] = coef * coord[d][i];
+      check(hipMalloc((void**)dst[d], sizeof(float) * c->N[d]), "hipMalloc pe");
+      if (st == SC_OK)
+        check(hipMemcpy(*dst[d], tmp, sizeof(float) * c->N[d], hipMemcpyHostToDevice),
+              "hipMemcpy pe");
+      free(tmp);
+    }
+  }
+  c->grav_mask = 0;
+  if (p.gravity_source) c->grav_mask = gdir == 3 ? 7 : (1 << gdir);
+  for (int f = 0; f < c->nf && st == SC_OK; ++f)
+    check(hipMalloc((void**)&c->rhs_scratch[f], sizeof(float) * c->cells_full), "hipMalloc rhs");
+  if (st == SC_OK) check(hipMalloc((void**)&c->speed_bits, sizeof(unsigned) * 4), "hipMalloc speed");
+  if (st == SC_OK) {
+    for (int f = 0; f < c->nf; ++f)
+      check(hipMemset(c->rhs_scratch[f], 0, sizeof(float) * c->cells_full), "hipMemset rhs");
+  }
+  if (st != SC_OK) {
+    sc_ctx_destroy(c);
+    return st;
+  }
+
+  // Halo tables.
+  sc::HaloArgs& ha = c->halo;
+  ha.nf = c->nf;
+  ha.h = h;
+  for (int d = 0; d < 3; ++d) {
+    ha.n[d] = n[d];
+    ha.N[d] = c->N[d];
+  }
+  ha.st[0] = c->N[1];
+  ha.st[1] = 1;
+  ha.st[2] = (long long)c->N[0] * c->N[1];
+  ha.prim_mask = p.prim_bc_mask;
+  for (int f = 0; f < c->nf; ++f)
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < 2; ++s) {
+        const sc_bc_face& b = p.bc[f][d][s];
+        sc::BcFaceDev& o = ha.bc[f][d][s];
+        o.type = b.type;
+        o.value = b.value;
+        o.n_layer_values = b.n_layer_values;
+        for (int i = 0; i < SC_MAX_HALO; ++i) o.layer_values[i] = b.layer_values[i];
+        o.layer_planes = b.layer_planes;
+      }
+
+  // Sweep constants.
+  sc::SweepArgs& a = c->sweep;
+  a.nx = n[0];
+  a.ny = n[1];
+  a.nz = n[2];
+  a.h = h;
+  a.NX = c->N[0];
+  a.NY = c->N[1];
+  a.NZ = c->N[2];
+  a.sx = c->N[1];
+  a.sz = (long long)c->N[0] * c->N[1];
+  a.has_pe = c->has_pe;
+  a.pe_x = c->pe_x;
+  a.pe_y = c->pe_y;
+  a.pe_z = c->pe_z;
+  a.grav_src = p.gravity_source ? 1 : 0;
+  a.grav_mask = c->grav_mask;
+  for (int d = 0; d < 3; ++d) a.gdir[d] = (float)p.g[d];
+  a.prim_bc_mask = p.prim_bc_mask;
+  a.vc.nu = (float)p.nu;
+  // kappa = ((1/Pr) * cp) * mu with (1/Pr) and cp as fp32 (simple.py:104-108).
+  a.vc.kfac = (float)(1.0 / p.pr) * sc::kCp;
+  for (int d = 0; d < 3; ++d) {
+    a.vc.h[d] = c->h[d];
+    a.vc.inv_h[d] = 1.0f / c->h[d];
+  }
+  a.vc.two_h[0] = (float)(2.0 * grid->dx);
+  a.vc.two_h[1] = (float)(2.0 * grid->dy);
+  a.vc.two_h[2] = (float)(2.0 * grid->dz);
+  // Coordinates are not kept: the context only needs the pe tables.
+  c->grid.x_full = c->grid.y_full = c->grid.z_full = nullptr;
+  *out = c;
+  return SC_OK;
+}
+
+sc_status sc_ctx_destroy(sc_ctx* c) {
+  if (c == nullptr) return SC_OK;
+  if (c->pe_x) hipFree(c->pe_x);
+  if (c->pe_y) hipFree(c->pe_y);
+  if (c->pe_z) hipFree(c->pe_z);
+  for (int f = 0; f < SC_MAX_FIELDS; ++f)
+    if (c->rhs_scratch[f]) hipFree(c->rhs_scratch[f]);
+  if (c->speed_bits) hipFree(c->speed_bits);
+  delete c;
+  return SC_OK;
+}
+
+sc_status sc_fill_halos(sc_ctx* c, float* const* q, void* stream) {
+  if (c == nullptr || q == nullptr) return fail(SC_ERR_INVALID, "sc_fill_halos: null argument");
+  for (int f = 0; f < c->nf; ++f)
+    if (q[f] == nullptr) return fail(SC_ERR_INVALID, "sc_fill_halos: null field pointer");
+  hipStream_t s = (hipStream_t)stream;
+  fill_halo_fields(c, q);
+  for (int d = 0; d < 3; ++d) {
+    sc_status st = hip_check(sc::launch_halo_dim(c->halo, d, s), "halo kernel");
+    if (st != SC_OK) return st;
+  }
+  if (c->halo.prim_mask)
+    return hip_check(sc::launch_halo_prim_finish(c->halo, s), "halo prim kernel");
+  return SC_OK;
+}
+
+static sc_status run_sweeps(sc_ctx* c, const sc::SweepArgs& a, hipStream_t s,
+                            int only_dir = -1) {
+  hipError_t e;
+  const int scheme = c->params.interpolation;
+  const int flux = c->params.numeric_flux;
+  const bool visc = c->params.include_diffusion != 0;
+  const int ns = c->nf - 5;
+  if (scheme == SC_WENO_5) {
+    e = sc::launch_rhs_sweeps<sc::kWeno5>(a, c->h, flux, visc, ns, c->tune, s, only_dir);
+  } else if (scheme == SC_WENO_3) {
+    e = sc::launch_rhs_sweeps<sc::kWeno3>(a, c->h, flux, visc, ns, c->tune, s, only_dir);
+  } else {
+    e = sc::launch_rhs_sweeps<sc::kMuscl>(a, c->h, flux, visc, ns, c->tune, s, only_dir);
+  }
+  return hip_check(e, "sweep kernels");
+}
+
+sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
+  if (c == nullptr || q == nullptr || rhs == nullptr)
+    return fail(SC_ERR_INVALID, "sc_rhs: null argument");
+  for (int f = 0; f < c->nf; ++f)
+    if (q[f] == nullptr || rhs[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_rhs: null field pointer");
+  sc_status st = sc_fill_halos(c, q, stream);
+  if (st != SC_OK) return st;
+  sc::SweepArgs a = c->sweep;
+  for (int f = 0; f < c->nf; ++f) {
+    a.q.p[f] = q[f];
+    a.r.p[f] = rhs[f];
+  }
+  a.rk_form = 0;
+  return run_sweeps(c, a, (hipStream_t)stream);
+}
+
+sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
+                      const float* const* u0, float* const* uk, float* const* out,
+                      void* stream) {
+  if (c == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
+  if (form == SC_RK_AVG)
+    return fail(SC_ERR_UNSUPPORTED,
+                "sc_rk_stage: the SC_RK_AVG form is disabled (an RK2 run on the GPU ended"
+                " in a device fault whose cause is not yet identified); evaluate RK2 with"
+                " sc_rhs and element-wise updates");
+  if (form != SC_RK_EULER && form != SC_RK_BLEND)
+    return fail(SC_ERR_INVALID, "sc_rk_stage: unknown RK form");
+  if (form != SC_RK_EULER && u0 == nullptr)
+    return fail(SC_ERR_INVALID, "sc_rk_stage: u0 required for this RK form");
+  for (int f = 0; f < c->nf; ++f) {
+    if (uk[f] == nullptr || out[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_rk_stage: null field pointer");
+    if (out[f] == uk[f]) return fail(SC_ERR_INVALID, "sc_rk_stage: out must not alias uk");
+    if (form != SC_RK_EULER && u0[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_rk_stage: null u0 field pointer");
+  }
+  sc_status st = sc_fill_halos(c, uk, stream);
+  if (st != SC_OK) return st;
+  sc::SweepArgs a = c->sweep;
+  for (int f = 0; f < c->nf; ++f) {
+    a.q.p[f] = uk[f];
+    a.u0.p[f] = form == SC_RK_EULER ? nullptr : u0[f];
+    a.r.p[f] = c->rhs_scratch[f];
+    a.out.p[f] = out[f];
+  }
+  a.rk_form = form;
+  a.c0 = c0;
+  a.c1 = c1;
+  a.dt = dt_dev;
+  return run_sweeps(c, a, (hipStream_t)stream);
+}
+
+sc_status sc_sweep_only(sc_ctx* c, int dim, int form, float c0, float c1,
+                        const float* dt_dev, const float* const* u0, float* const* uk,
+                        float* const* out, void* stream) {
+  if (c == nullptr || uk == nullptr) return fail(SC_ERR_INVALID, "sc_sweep_only: null argument");
+  if (dim < 0 || dim > 2) return fail(SC_ERR_INVALID, "sc_sweep_only: dim must be 0..2");
+  if (form != 0 && (out == nullptr || dt_dev == nullptr || (form != SC_RK_EULER && u0 == nullptr)))
+    return fail(SC_ERR_INVALID, "sc_sweep_only: missing RK operands");
+  sc::SweepArgs a = c->sweep;
+  for (int f = 0; f < c->nf; ++f) {
+    a.q.p[f] = uk[f];
+    a.u0.p[f] = (form == 0 || form == SC_RK_EULER) ? nullptr : u0[f];
+    a.r.p[f] = c->rhs_scratch[f];
+    a.out.p[f] = form == 0 ? nullptr : out[f];
+  }
+  a.rk_form = form;
+  a.c0 = c0;
+  a.c1 = c1;
+  a.dt = dt_dev;
+  return run_sweeps(c, a, (hipStream_t)stream, dim);
+}
+
+sc_status sc_max_wave_speed(sc_ctx* c, const float* const* q, float cfl, float* out_dev,
+                            float* dt_dev, void* stream) {
+  if (c == nullptr || q == nullptr || out_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_max_wave_speed: null argument");
+  sc::SpeedArgs a;
+  for (int f = 0; f < 5; ++f) a.q[f] = q[f];
+  a.nx = c->sweep.nx;
+  a.ny = c->sweep.ny;
+  a.nz = c->sweep.nz;
+  a.h = c->sweep.h;
+  a.sx = c->sweep.sx;
+  a.sz = c->sweep.sz;
+  a.has_pe = c->has_pe;
+  a.pe_x = c->pe_x;
+  a.pe_y = c->pe_y;
+  a.pe_z = c->pe_z;
+  float hmin = c->h[0];
+  if (c->h[1] < hmin) hmin = c->h[1];
+  if (c->h[2] < hmin) hmin = c->h[2];
+  return hip_check(sc::launch_max_speed(a, (unsigned*)out_dev, cfl, hmin, dt_dev,
+                                        (hipStream_t)stream),
+                   "max speed kernel");
+}
+
+sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_t* codes,
+                               void* stream) {
+  if (c == nullptr || q == nullptr || codes == nullptr)
+    return fail(SC_ERR_INVALID, "sc_hllc_branch_codes: null argument");
+  if (dim < 0 || dim > 2) return fail(SC_ERR_INVALID, "sc_hllc_branch_codes: dim must be 0..2");
+  sc::SweepArgs a = c->sweep;
+  for (int f = 0; f < c->nf; ++f) a.q.p[f] = q[f];
+  hipError_t e;
+  const int ns = c->nf - 5;
+  if (c->params.interpolation == SC_WENO_5) {
+    e = sc::launch_branch_codes<sc::kWeno5>(a, dim, ns, codes, (hipStream_t)stream);
+  } else if (c->params.interpolation == SC_WENO_3) {
+    e = sc::launch_branch_codes<sc::kWeno3>(a, dim, ns, codes, (hipStream_t)stream);
+  } else {
+    e = sc::launch_branch_codes<sc::kMuscl>(a, dim, ns, codes, (hipStream_t)stream);
+  }
+  return hip_check(e, "branch code kernel");
+}
+
+}  // extern "C"

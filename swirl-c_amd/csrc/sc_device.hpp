@@ -1,0 +1,403 @@
+// Device-side physics and numerics of the compressible NS RHS (gfx950).
+//
+// Every function restates one piece of the reference's per-cell or per-face
+// arithmetic, keeping its operand order so fp32 results track the
+// reference's fp32 path to a few ulps.  Reference anchors are cited per
+// function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sc {
+
+// physics/constant.py:17-41 (computed in double as Python does, rounded once).
+constexpr double kGammaD = 1.4;
+constexpr double kRD = 8.314 / 0.029;
+constexpr double kCvD = kRD / (kGammaD - 1.0);
+constexpr double kCpD = kCvD + kRD;
+constexpr float kGamma = (float)kGammaD;
+constexpr float kGm1 = (float)(kGammaD - 1.0);
+constexpr float kR = (float)kRD;
+constexpr float kCp = (float)kCpD;
+constexpr float kG = 9.81f;
+constexpr float kThird = (float)(1.0 / 3.0);
+constexpr float kWenoEps = 1e-6f;
+constexpr float kSmSnap = 1e-6f;
+
+constexpr int kMaxFields = 7;
+
+// Interpolation scheme ids (match SC_WENO_3 / SC_WENO_5 / SC_MUSCL).
+constexpr int kWeno3 = 0;
+constexpr int kWeno5 = 1;
+constexpr int kMuscl = 2;
+// Riemann solver ids (match SC_HLL / SC_HLLC).
+constexpr int kHll = 0;
+constexpr int kHllc = 1;
+
+// Number of cells on each side of a cell its reconstruction reads.
+template <int SCHEME>
+struct Radius {
+  static constexpr int value = SCHEME == kWeno5 ? 2 : 1;
+};
+
+// Reconstructed primitive variables per cell: rho u v w e p h + scalars
+// (equations/compressible_navier_stokes.py:107-113,127).
+constexpr int kRho = 0, kU = 1, kV = 2, kW = 3, kE = 4, kP = 5, kH = 6;
+template <int NS>
+struct NVar {
+  static constexpr int value = 7 + NS;
+};
+
+// tf.math.divide_no_nan / reciprocal_no_nan.
+__device__ __forceinline__ float dnn(float x, float y) {
+  return y == 0.0f ? 0.0f : x / y;
+}
+__device__ __forceinline__ float rnn(float x) {
+  return x == 0.0f ? 0.0f : 1.0f / x;
+}
+
+// Conserved (rho, rho_u, rho_v, rho_w, rho_e, rho_s..) -> primitives
+// (rho, u, v, w, e, p, h, s..): common/utils.py:145-183 (phi = (1/rho)*q),
+// generic.py:27-88,357-389 (p = ((gamma-1)*rho) * (e - 0.5*(u^2+v^2+w^2) - pe),
+// h = e + p/rho).
+template <int NS>
+__device__ __forceinline__ void cons_to_prim(const float* q, float pe, float* w) {
+  const float rho = q[0];
+  const float inv = 1.0f / rho;
+  w[kRho] = rho;
+  w[kU] = inv * q[1];
+  w[kV] = inv * q[2];
+  w[kW] = inv * q[3];
+  w[kE] = inv * q[4];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) w[7 + s] = inv * q[5 + s];
+  const float ke = 0.5f * ((w[kU] * w[kU] + w[kV] * w[kV]) + w[kW] * w[kW]);
+  const float eint = (w[kE] - ke) - pe;
+  w[kP] = (kGm1 * rho) * eint;
+  w[kH] = w[kE] + w[kP] / rho;
+}
+
+// ---------------------------------------------------------------------------
+// Face reconstruction of one variable at one cell.  a[] holds the cell values
+// at offsets -R..R.  Returns fp = left state of face c+1/2 and fm = right
+// state of face c-1/2 (swirl_lm's (neg, pos) at i-1/2 stored at i, for the
+// faces on either side of cell c).  WENO follows demo/swirl_c_prototype.ipynb
+// cell 4 `weno5` (l.70-211); MUSCL follows `muscl` (l.214-239).
+// ---------------------------------------------------------------------------
+
+template <int SCHEME>
+__device__ __forceinline__ void recon(const float* a, float& fp, float& fm);
+
+template <>
+__device__ __forceinline__ void recon<kWeno5>(const float* a, float& fp, float& fm) {
+  const float vm2 = a[0], vm1 = a[1], v0 = a[2], vp1 = a[3], vp2 = a[4];
+  const float c1312 = (float)(13.0 / 12.0);
+  float t, s;
+  t = v0 - 2.0f * vp1 + vp2;
+  s = 3.0f * v0 - 4.0f * vp1 + vp2;
+  const float b0 = c1312 * (t * t) + 0.25f * (s * s);
+  t = vm1 - 2.0f * v0 + vp1;
+  s = vm1 - vp1;
+  const float b1 = c1312 * (t * t) + 0.25f * (s * s);
+  t = vm2 - 2.0f * vm1 + v0;
+  s = vm2 - 4.0f * vm1 + 3.0f * v0;
+  const float b2 = c1312 * (t * t) + 0.25f * (s * s);
+  // Candidate stencils (coefficients c[k][r] of the prototype).
+  const float c13 = (float)(1.0 / 3.0), c56 = (float)(5.0 / 6.0);
+  const float c16 = (float)(-1.0 / 6.0), c76 = (float)(-7.0 / 6.0);
+  const float c116 = (float)(11.0 / 6.0);
+  const float qn0 = (c13 * v0 + c56 * vp1) + c16 * vp2;
+  const float qn1 = (c16 * vm1 + c56 * v0) + c13 * vp1;
+  const float qn2 = (c13 * vm2 + c76 * vm1) + c116 * v0;
+  const float qp0 = (c116 * v0 + c76 * vp1) + c13 * vp2;
+  const float qp1 = (c13 * vm1 + c56 * v0) + c16 * vp1;
+  const float qp2 = (c16 * vm2 + c56 * vm1) + c13 * v0;
+  float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
+  e0 = e0 * e0;
+  e1 = e1 * e1;
+  e2 = e2 * e2;
+  const float an0 = 0.3f / e0, an1 = 0.6f / e1, an2 = 0.1f / e2;
+  const float ap0 = 0.1f / e0, ap1 = 0.6f / e1, ap2 = 0.3f / e2;
+  const float sn = (an0 + an1) + an2;
+  const float sp = (ap0 + ap1) + ap2;
+  fp = ((an0 / sn) * qn0 + (an1 / sn) * qn1) + (an2 / sn) * qn2;
+  fm = ((ap0 / sp) * qp0 + (ap1 / sp) * qp1) + (ap2 / sp) * qp2;
+}
+
+template <>
+__device__ __forceinline__ void recon<kWeno3>(const float* a, float& fp, float& fm) {
+  const float vm1 = a[0], v0 = a[1], vp1 = a[2];
+  float t = vp1 - v0;
+  const float b0 = t * t;
+  t = v0 - vm1;
+  const float b1 = t * t;
+  const float qn0 = 0.5f * v0 + 0.5f * vp1;
+  const float qn1 = -0.5f * vm1 + 1.5f * v0;
+  const float qp0 = 1.5f * v0 + -0.5f * vp1;
+  const float qp1 = 0.5f * vm1 + 0.5f * v0;
+  const float d0 = (float)(2.0 / 3.0), d1 = (float)(1.0 / 3.0);
+  float e0 = kWenoEps + b0, e1 = kWenoEps + b1;
+  e0 = e0 * e0;
+  e1 = e1 * e1;
+  const float an0 = d0 / e0, an1 = d1 / e1;
+  const float ap0 = d1 / e0, ap1 = d0 / e1;
+  const float sn = an0 + an1;
+  const float sp = ap0 + ap1;
+  fp = (an0 / sn) * qn0 + (an1 / sn) * qn1;
+  fm = (ap0 / sp) * qp0 + (ap1 / sp) * qp1;
+}
+
+__device__ __forceinline__ float sgnf(float x) {
+  return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+}
+
+template <>
+__device__ __forceinline__ void recon<kMuscl>(const float* a, float& fp, float& fm) {
+  const float vm1 = a[0], v0 = a[1], vp1 = a[2];
+  // minmod(central2(v, 2), forward1(v, 1), backward1(v, 1)).
+  const float g0 = (vp1 - vm1) / 4.0f;
+  const float g1 = vp1 - v0;
+  const float g2 = v0 - vm1;
+  const float s = ((sgnf(g0) + sgnf(g1)) + sgnf(g2)) / 3.0f;
+  const float amin = fminf(fminf(fabsf(g0), fabsf(g1)), fabsf(g2));
+  const float m = fabsf(s) == 1.0f ? s * amin : 0.0f;
+  fp = v0 + 0.5f * m;
+  fm = v0 - 0.5f * m;
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers at one face.  L = neg (left) state, R = pos (right) state,
+// both as reconstructed primitives; pe = gravitational potential at the cell
+// holding the face.  D = face-normal dimension (0 x, 1 y, 2 z).
+// ---------------------------------------------------------------------------
+
+// Physical flux of one side: equations/compressible_navier_stokes.py:338-427,
+// with the face conserved momentum m = rho_f * u_f (utils.py:186-216).
+template <int D, int NS>
+__device__ __forceinline__ void physical_flux(const float* s, float* f) {
+  const float m = s[kRho] * s[kU + D];
+  f[0] = m;
+  f[1] = D == 0 ? m * s[kU] + s[kP] : m * s[kU];
+  f[2] = D == 1 ? m * s[kV] + s[kP] : m * s[kV];
+  f[3] = D == 2 ? m * s[kW] + s[kP] : m * s[kW];
+  f[4] = m * (s[kE] + s[kP] / s[kRho]);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) f[5 + k] = m * s[7 + k];
+}
+
+// Roe / Einfeldt wave-speed estimates: hll.py:178-283, sound speeds
+// generic.py:90-159 (opt 'p_rho' per side, opt 'h' for the Roe state).
+template <int D>
+__device__ __forceinline__ void roe_speeds(const float* L, const float* R,
+                                           float pe, float& sl, float& sr) {
+  const float cl = sqrtf(kGamma * L[kP] / L[kRho]);
+  const float cr = sqrtf(kGamma * R[kP] / R[kRho]);
+  const float ql = sqrtf(L[kRho]);
+  const float qr = sqrtf(R[kRho]);
+  const float inv = rnn(ql + qr);
+  const float ru = (ql * L[kU] + qr * R[kU]) * inv;
+  const float rv = (ql * L[kV] + qr * R[kV]) * inv;
+  const float rw = (ql * L[kW] + qr * R[kW]) * inv;
+  const float rh = (ql * L[kH] + qr * R[kH]) * inv;
+  const float hs = (rh - 0.5f * ((ru * ru + rv * rv) + rw * rw)) - pe;
+  const float c = sqrtf(kGm1 * hs);
+  const float un = D == 0 ? ru : (D == 1 ? rv : rw);
+  sl = fminf(L[kU + D] - cl, un - c);
+  sr = fmaxf(R[kU + D] + cr, un + c);
+}
+
+// HLL flux: hll.py:36-175.
+template <int D, int NS>
+__device__ __forceinline__ void hll_flux(const float* L, const float* R, float pe,
+                                         float* F) {
+  constexpr int NF = 5 + NS;
+  float sl, sr;
+  roe_speeds<D>(L, R, pe, sl, sr);
+  const float ds = sr - sl;
+  const float t1 = dnn(fminf(sr, 0.0f) - fminf(sl, 0.0f), ds);
+  const float t2 = 1.0f - t1;
+  const float t3 = dnn(sr * fabsf(sl) - sl * fabsf(sr), 2.0f * ds);
+  float fl[NF], fr[NF], ul[NF], ur[NF];
+  physical_flux<D, NS>(L, fl);
+  physical_flux<D, NS>(R, fr);
+  ul[0] = L[kRho];
+  ur[0] = R[kRho];
+  ul[1] = L[kRho] * L[kU];
+  ur[1] = R[kRho] * R[kU];
+  ul[2] = L[kRho] * L[kV];
+  ur[2] = R[kRho] * R[kV];
+  ul[3] = L[kRho] * L[kW];
+  ur[3] = R[kRho] * R[kW];
+  ul[4] = L[kRho] * L[kE];
+  ur[4] = R[kRho] * R[kE];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    ul[5 + k] = L[kRho] * L[7 + k];
+    ur[5 + k] = R[kRho] * R[7 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) F[k] = (t1 * fr[k] + t2 * fl[k]) - t3 * (ur[k] - ul[k]);
+}
+
+// HLLC star-state flux of one side: hllc.py:134-192 (Toro eq. 10.38).
+template <int D, int NS>
+__device__ __forceinline__ void hllc_star(const float* S, float sk, float sm,
+                                          const float* f, float* fs) {
+  const float rho = S[kRho], vn = S[kU + D];
+  const float q = dnn(sk, sk - sm);
+  const float a = rho * (sm - vn);
+  fs[0] = f[0] + a * q;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (k == D) {
+      fs[1 + k] = f[1 + k] + (rho * sk) * (dnn(sk - vn, sk - sm) * sm - vn);
+    } else {
+      fs[1 + k] = f[1 + k] + (a * S[kU + k]) * q;
+    }
+  }
+  fs[4] = f[4] + ((rho * q) * (sm - vn)) *
+                     ((S[kE] + S[kP] / rho) + (sk - vn) * sm);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) fs[5 + k] = f[5 + k] + (a * S[7 + k]) * q;
+}
+
+// HLLC flux: hllc.py:33-257 and its wave speeds hllc.py:260-347.  Returns the
+// branch code 0 F_L, 1 F*_L, 2 F*_R, 3 F_R (hllc.py:227-243).
+template <int D, int NS>
+__device__ __forceinline__ int hllc_flux(const float* L, const float* R, float pe,
+                                         float* F) {
+  constexpr int NF = 5 + NS;
+  float sl, sr;
+  roe_speeds<D>(L, R, pe, sl, sr);
+  const float vl = L[kU + D], vr = R[kU + D];
+  const float rl = L[kRho], rr = R[kRho];
+  float sm = dnn(((R[kP] - L[kP]) + (rl * vl) * (sl - vl)) - (rr * vr) * (sr - vr),
+                 rl * (sl - vl) - rr * (sr - vr));
+  sm = fabsf(sm) < kSmSnap ? 0.0f : sm;
+  int br;
+  if (sl >= 0.0f) {
+    br = 0;
+  } else if (sl <= 0.0f && sm >= 0.0f) {
+    br = 1;
+  } else if (sm <= 0.0f && sr >= 0.0f) {
+    br = 2;
+  } else {
+    br = 3;
+  }
+  float f[NF];
+  if (br == 0 || br == 1) {
+    physical_flux<D, NS>(L, f);
+    if (br == 1) {
+      hllc_star<D, NS>(L, sl, sm, f, F);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) F[k] = f[k];
+    }
+  } else {
+    physical_flux<D, NS>(R, f);
+    if (br == 2) {
+      hllc_star<D, NS>(R, sr, sm, f, F);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) F[k] = f[k];
+    }
+  }
+  return br;
+}
+
+template <int FLUX, int D, int NS>
+__device__ __forceinline__ int riemann(const float* L, const float* R, float pe,
+                                       float* F) {
+  if constexpr (FLUX == kHllc) {
+    return hllc_flux<D, NS>(L, R, pe, F);
+  } else {
+    hll_flux<D, NS>(L, R, pe, F);
+    return 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Viscous + heat flux at one face normal to D:
+// equations/compressible_navier_stokes.py:618-707, physics/diffusion.py:33-109,
+// physics/fluid.py:31-111, physics/transport/simple.py:44-108.
+//   A = cell c-1, B = cell c (primitives); ta/tb = central transverse
+//   derivatives at c-1 / c: index [t][0] = d(u_t)/dx_t, [t][1] = d(u_D)/dx_t for
+//   the two transverse dims t in x,y,z order (t != D).
+// Adds the diffusive flux to F (mass flux unchanged).
+// ---------------------------------------------------------------------------
+struct ViscConst {
+  float nu;        // kinematic viscosity
+  float kfac;      // (1/Pr) * cp
+  float inv_h[3];  // unused placeholder for alignment
+  float h[3];      // dx, dy, dz
+  float two_h[3];  // 2 dx, 2 dy, 2 dz
+};
+
+template <int D, int NS>
+__device__ __forceinline__ void viscous_flux(const float* A, const float* B,
+                                             const float (*ta)[2],
+                                             const float (*tb)[2],
+                                             const ViscConst& vc, float* F) {
+  const float h = vc.h[D];
+  // Backward (normal) derivatives on the face.
+  const float gu = (B[kU] - A[kU]) / h;
+  const float gv = (B[kV] - A[kV]) / h;
+  const float gw = (B[kW] - A[kW]) / h;
+  const float gn[3] = {gu, gv, gw};
+  // Transverse derivatives interpolated onto the face: 0.5*(f[c-1] + f[c]).
+  float it[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    it[t][0] = 0.5f * (ta[t][0] + tb[t][0]);
+    it[t][1] = 0.5f * (ta[t][1] + tb[t][1]);
+  }
+  // Divergence on the face, summed in x, y, z order.
+  float g3[3];
+  {
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k == D) {
+        g3[k] = gn[D];
+      } else {
+        g3[k] = it[t][0];
+        ++t;
+      }
+    }
+  }
+  const float div = (g3[0] + g3[1]) + g3[2];
+  // Strain rate S_iD on the face.
+  float S[3];
+  {
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (i == D) {
+        S[i] = gn[D] - kThird * div;
+      } else {
+        S[i] = 0.5f * (gn[i] + it[t][1]);
+        ++t;
+      }
+    }
+  }
+  const float rho_f = 0.5f * (A[kRho] + B[kRho]);
+  const float c2 = (2.0f * rho_f) * vc.nu;
+  const float tx = c2 * S[0], ty = c2 * S[1], tz = c2 * S[2];
+  const float uf = 0.5f * (A[kU] + B[kU]);
+  const float vf = 0.5f * (A[kV] + B[kV]);
+  const float wf = 0.5f * (A[kW] + B[kW]);
+  // Heat flux -kappa_f * dT/dx, T = p/(rho R), kappa = ((1/Pr)*cp)*(nu*rho).
+  const float Ta = A[kP] / (A[kRho] * kR);
+  const float Tb = B[kP] / (B[kRho] * kR);
+  const float ka = vc.kfac * (vc.nu * A[kRho]);
+  const float kb = vc.kfac * (vc.nu * B[kRho]);
+  const float kf = 0.5f * (ka + kb);
+  const float q = -kf * ((Tb - Ta) / h);
+  F[1] = F[1] + (-tx);
+  F[2] = F[2] + (-ty);
+  F[3] = F[3] + (-tz);
+  F[4] = F[4] + (q + (-((uf * tx + vf * ty) + wf * tz)));
+}
+
+}  // namespace sc

@@ -1,0 +1,52 @@
+// Halo-fill and wave-speed kernels (declarations).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sc_device.hpp"
+
+namespace sc {
+
+constexpr int kMaxHalo = 4;
+constexpr int kBcPeriodic = 0, kBcDirichlet = 1, kBcNeumann = 2;
+
+struct BcFaceDev {
+  int type;
+  float value;
+  int n_layer_values;
+  float layer_values[kMaxHalo];
+  const float* layer_planes;  // [h][A][B] or null
+};
+
+struct HaloArgs {
+  float* q[kMaxFields];
+  int nf;
+  int h;
+  int n[3];          // interior extents along x, y, z
+  int N[3];          // full extents along x, y, z
+  long long st[3];   // element strides along x, y, z
+  int prim_mask;     // fields whose BC is on the primitive
+  BcFaceDev bc[kMaxFields][3][2];
+};
+
+// Applies the BCs along dim d (0 x, 1 y, 2 z) to all fields; must be launched
+// for d = 0, 1, 2 in order, then halo_prim_finish if prim_mask != 0.
+hipError_t launch_halo_dim(const HaloArgs& a, int d, hipStream_t stream);
+hipError_t launch_halo_prim_finish(const HaloArgs& a, hipStream_t stream);
+
+struct SpeedArgs {
+  const float* q[5];
+  int nx, ny, nz, h;
+  long long sx, sz;
+  int has_pe;
+  const float* pe_x;
+  const float* pe_y;
+  const float* pe_z;
+};
+
+// *max_bits = bits of max over interior of max_d(|u_d| + c); then
+// *dt = cfl * hmin / max if dt is not null.
+hipError_t launch_max_speed(const SpeedArgs& a, unsigned* max_bits, float cfl,
+                            float hmin, float* dt, hipStream_t stream);
+
+}  // namespace sc

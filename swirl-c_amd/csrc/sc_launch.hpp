@@ -1,0 +1,111 @@
+// Host-side launchers shared between the translation units of libswirlc_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sc_sweep_impl.hpp"
+
+namespace sc {
+
+// Launch geometry of the sweeps (the tunable knobs).
+struct SweepTuning {
+  int seg_len = 32;  // cells per thread along a marching sweep
+  int by_march = 4;  // waves per block, marching sweeps
+  int by_y = 4;      // waves per block, y sweep
+};
+
+// Launches the x, y and z sweeps of one RHS evaluation (plus the folded RK
+// update when a.rk_form != 0) for interpolation scheme SCHEME.
+// only_dir = -1 launches all three; 0, 1 or 2 launches that sweep alone.
+template <int SCHEME>
+hipError_t launch_rhs_sweeps(const SweepArgs& a, const float h[3], int flux,
+                             bool visc, int ns, const SweepTuning& tune,
+                             hipStream_t stream, int only_dir = -1);
+
+// Launches the HLLC branch-code kernel for faces normal to dim.
+template <int SCHEME>
+hipError_t launch_branch_codes(const SweepArgs& a, int dim, int ns,
+                               uint8_t* codes, hipStream_t stream);
+
+// Generic body, instantiated once per scheme in sc_sweep_<scheme>.hip.
+template <int SCHEME, int FLUX, bool VISC, int NS>
+hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
+                               const SweepTuning& tune, hipStream_t stream,
+                               int only_dir) {
+  const dim3 blk_m(64, tune.by_march);
+  const dim3 blk_y(64, tune.by_y);
+  a.seg_len = tune.seg_len;
+  // x sweep (first): R = src - div_x.
+  if (only_dir < 0 || only_dir == 0) {
+    SweepArgs ax = a;
+    ax.hdir = h[0];
+    const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_march - 1) / tune.by_march,
+                    (a.nx + tune.seg_len - 1) / tune.seg_len);
+    hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>), grid,
+                       blk_m, 0, stream, ax);
+  }
+  // y sweep (middle): R -= div_y.
+  if (only_dir < 0 || only_dir == 1) {
+    SweepArgs ay = a;
+    ay.hdir = h[1];
+    constexpr int OUTW = YGeom<SCHEME>::OUTW;
+    const dim3 grid((a.ny + OUTW - 1) / OUTW, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
+    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS>), grid, blk_y, 0,
+                       stream, ay);
+  }
+  // z sweep (last): R -= div_z, then store R or the RK update.
+  if (only_dir < 0 || only_dir == 2) {
+    SweepArgs az = a;
+    az.hdir = h[2];
+    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
+                    (a.nz + tune.seg_len - 1) / tune.seg_len);
+    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>), grid,
+                       blk_m, 0, stream, az);
+  }
+  return hipGetLastError();
+}
+
+template <int SCHEME, int NS>
+hipError_t launch_branch_codes_t(const SweepArgs& a, int dim, uint8_t* codes,
+                                 hipStream_t stream) {
+  const long long nf = (long long)(a.nx + (dim == 0)) * (a.ny + (dim == 1)) *
+                       (a.nz + (dim == 2));
+  const int blk = 256;
+  const unsigned grid = (unsigned)((nf + blk - 1) / blk);
+  if (dim == 0) {
+    hipLaunchKernelGGL((branch_codes_kernel<0, SCHEME, NS>), grid, blk, 0, stream, a, codes);
+  } else if (dim == 1) {
+    hipLaunchKernelGGL((branch_codes_kernel<1, SCHEME, NS>), grid, blk, 0, stream, a, codes);
+  } else {
+    hipLaunchKernelGGL((branch_codes_kernel<2, SCHEME, NS>), grid, blk, 0, stream, a, codes);
+  }
+  return hipGetLastError();
+}
+
+#define SC_INSTANTIATE_SCHEME(SCHEME)                                            \
+  template <>                                                                    \
+  hipError_t launch_rhs_sweeps<SCHEME>(const SweepArgs& a, const float h[3],     \
+                                       int flux, bool visc, int ns,              \
+                                       const SweepTuning& tune,                  \
+                                       hipStream_t stream, int od) {             \
+    if (ns == 0) {                                                               \
+      if (flux == kHllc)                                                         \
+        return visc ? launch_rhs_sweeps_t<SCHEME, kHllc, true, 0>(a, h, tune, stream, od) \
+                    : launch_rhs_sweeps_t<SCHEME, kHllc, false, 0>(a, h, tune, stream, od); \
+      return visc ? launch_rhs_sweeps_t<SCHEME, kHll, true, 0>(a, h, tune, stream, od)      \
+                  : launch_rhs_sweeps_t<SCHEME, kHll, false, 0>(a, h, tune, stream, od);    \
+    }                                                                            \
+    if (flux == kHllc)                                                           \
+      return visc ? launch_rhs_sweeps_t<SCHEME, kHllc, true, 1>(a, h, tune, stream, od)   \
+                  : launch_rhs_sweeps_t<SCHEME, kHllc, false, 1>(a, h, tune, stream, od); \
+    return visc ? launch_rhs_sweeps_t<SCHEME, kHll, true, 1>(a, h, tune, stream, od)        \
+                : launch_rhs_sweeps_t<SCHEME, kHll, false, 1>(a, h, tune, stream, od);      \
+  }                                                                              \
+  template <>                                                                    \
+  hipError_t launch_branch_codes<SCHEME>(const SweepArgs& a, int dim, int ns,    \
+                                         uint8_t* codes, hipStream_t stream) {   \
+    return ns == 0 ? launch_branch_codes_t<SCHEME, 0>(a, dim, codes, stream)     \
+                   : launch_branch_codes_t<SCHEME, 1>(a, dim, codes, stream);    \
+  }
+
+}  // namespace sc

@@ -1,0 +1,384 @@
+// Fused per-direction sweep kernels of the RHS (gfx950).
+//
+// One launch per direction d in x, y, z (the reference's order of flux
+// subtraction, equations/compressible_navier_stokes.py:255-264).  Each launch
+// reads the conserved fields of the stage state once, converts them to
+// primitives, reconstructs the face states (WENO / MUSCL), solves the Riemann
+// problem (HLL / HLLC), adds the viscous and heat flux, takes the flux
+// difference and accumulates it into the RHS:
+//   x sweep:  R  = src - (F_x[i+1] - F_x[i]) / dx          (writes R)
+//   y sweep:  R  = R   - (F_y[j+1] - F_y[j]) / dy          (reads, writes R)
+//   z sweep:  R  = R   - (F_z[k+1] - F_z[k]) / dz, then either writes R or
+//             the Runge-Kutta stage update of the state (RK axpy folded in).
+// Layout [z][x][y], y contiguous.  The x and z sweeps map 64 lanes to 64
+// consecutive y cells (coalesced) and march each thread along its pencil
+// with a register sliding window; the y sweep maps lanes along y and gets
+// the stencil neighbours through cross-lane shuffles.
+#pragma once
+
+#include "sc_device.hpp"
+
+namespace sc {
+
+struct Fields {
+  float* p[kMaxFields];
+};
+struct CFields {
+  const float* p[kMaxFields];
+};
+
+struct SweepArgs {
+  CFields q;    // stage state uk, halos filled
+  CFields u0;   // RK base state
+  Fields r;     // RHS accumulator
+  Fields out;   // RK stage output
+  int nx, ny, nz, h;
+  int NX, NY, NZ;
+  long long sx, sz;  // strides of x and z (y stride is 1)
+  float hdir;        // spacing along the sweep direction
+  int seg_len;       // cells per thread (marching sweeps)
+  int has_pe;        // gravitational potential present
+  const float* pe_x; // pe = (pe_x[i] - pe_y[j]) - pe_z[k]
+  const float* pe_y;
+  const float* pe_z;
+  int grav_src;      // gravity source active
+  int grav_mask;     // bit d: momentum d receives the source
+  float gdir[3];     // cfg.g components
+  int prim_bc_mask;  // fields whose interior is rho*(q/rho)-rounded
+  int rk_form;       // 0: write R; SC_RK_* otherwise
+  float c0, c1;
+  const float* dt;
+  ViscConst vc;
+};
+
+__device__ __forceinline__ float pe_at(const SweepArgs& a, int k, int i, int j) {
+  return a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
+}
+
+// Loads the conserved fields of one cell; interior cells of fields with a
+// primitive BC get the reference's rho * (q / rho) rounding
+// (boundary/boundary.py:516-531).
+template <int NF>
+__device__ __forceinline__ void load_cons(const SweepArgs& a, long long idx,
+                                          bool interior, float* q) {
+#pragma unroll
+  for (int f = 0; f < NF; ++f) q[f] = a.q.p[f][idx];
+  if (a.prim_bc_mask && interior) {
+#pragma unroll
+    for (int f = 1; f < NF; ++f)
+      if (a.prim_bc_mask & (1 << f)) q[f] = q[0] * dnn(q[f], q[0]);
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void load_prim(const SweepArgs& a, long long idx,
+                                          bool interior, float pe, float* w) {
+  constexpr int NF = 5 + NS;
+  float q[NF];
+  load_cons<NF>(a, idx, interior, q);
+  cons_to_prim<NS>(q, pe, w);
+}
+
+// Velocity components u_c0 and u_c1 at a cell (for transverse derivatives).
+__device__ __forceinline__ void load_vel2(const SweepArgs& a, long long idx,
+                                          bool interior, int c0, int c1,
+                                          float& v0, float& v1) {
+  const float rho = a.q.p[0][idx];
+  float m0 = a.q.p[1 + c0][idx];
+  float m1 = a.q.p[1 + c1][idx];
+  if (a.prim_bc_mask && interior) {
+    if (a.prim_bc_mask & (1 << (1 + c0))) m0 = rho * dnn(m0, rho);
+    if (a.prim_bc_mask & (1 << (1 + c1))) m1 = rho * dnn(m1, rho);
+  }
+  const float inv = 1.0f / rho;
+  v0 = inv * m0;
+  v1 = inv * m1;
+}
+
+__device__ __forceinline__ bool in_core(int c, int h, int n) {
+  return c >= h && c < h + n;
+}
+
+// Central transverse derivatives at cell (k, i, j) for a face normal to D:
+// out[t][0] = d(u_t)/dx_t, out[t][1] = d(u_D)/dx_t for the transverse dims t
+// in x, y, z order (physics/fluid.py:52-63 with numerics/gradient.py:71-94).
+template <int D>
+__device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int i,
+                                                 int j, float (*out)[2]) {
+  int t = 0;
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) {
+    if (dd == D) continue;
+    long long sp, sm;
+    bool inp, inm;
+    const long long c = (long long)k * a.sz + (long long)i * a.sx + j;
+    if (dd == 0) {
+      sp = c + a.sx;
+      sm = c - a.sx;
+      inp = in_core(i + 1, a.h, a.nx) && in_core(j, a.h, a.ny) && in_core(k, a.h, a.nz);
+      inm = in_core(i - 1, a.h, a.nx) && in_core(j, a.h, a.ny) && in_core(k, a.h, a.nz);
+    } else if (dd == 1) {
+      sp = c + 1;
+      sm = c - 1;
+      inp = in_core(i, a.h, a.nx) && in_core(j + 1, a.h, a.ny) && in_core(k, a.h, a.nz);
+      inm = in_core(i, a.h, a.nx) && in_core(j - 1, a.h, a.ny) && in_core(k, a.h, a.nz);
+    } else {
+      sp = c + a.sz;
+      sm = c - a.sz;
+      inp = in_core(i, a.h, a.nx) && in_core(j, a.h, a.ny) && in_core(k + 1, a.h, a.nz);
+      inm = in_core(i, a.h, a.nx) && in_core(j, a.h, a.ny) && in_core(k - 1, a.h, a.nz);
+    }
+    float ap0, ap1, am0, am1;
+    load_vel2(a, sp, inp, dd, D, ap0, ap1);
+    load_vel2(a, sm, inm, dd, D, am0, am1);
+    const float th = a.vc.two_h[dd];
+    out[t][0] = (ap0 - am0) / th;
+    out[t][1] = (ap1 - am1) / th;
+    ++t;
+  }
+}
+
+// Gravity source of field f at a cell (physics_models.py:94-143,
+// gravity.py:64-102): (rho * G) * g_d on the selected momenta, else 0.
+__device__ __forceinline__ float source_of(const SweepArgs& a, int f, float rho) {
+  if (a.grav_src && f >= 1 && f <= 3 && (a.grav_mask & (1 << (f - 1)))) {
+    return (rho * kG) * a.gdir[f - 1];
+  }
+  return 0.0f;
+}
+
+// Last-sweep store: either the RHS value or the RK stage update
+// (rk3.py:69-83, rk2.py:67-86).
+__device__ __forceinline__ void store_last(const SweepArgs& a, int f, long long idx,
+                                           float val) {
+  if (a.rk_form == 0) {
+    a.r.p[f][idx] = val;
+    return;
+  }
+  const float dt = *a.dt;
+  const float uk = a.q.p[f][idx];
+  float o;
+  if (a.rk_form == 1) {
+    o = uk + val * dt;
+  } else if (a.rk_form == 2) {
+    o = a.u0.p[f][idx] * a.c0 + (uk + val * dt) * a.c1;
+  } else {
+    o = 0.5f * (a.u0.p[f][idx] + (uk + dt * val));
+  }
+  a.out.p[f][idx] = o;
+}
+
+// ---------------------------------------------------------------------------
+// Marching sweep along x (DIR 0) or z (DIR 2).  Block (64, BY): lane -> y,
+// threadIdx.y -> the other transverse index, blockIdx.z -> segment of
+// seg_len cells along the sweep.  PHASE 0: first sweep (writes R = src - div);
+// PHASE 2: last sweep (R - div -> R or RK update).
+// ---------------------------------------------------------------------------
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
+__global__ void __launch_bounds__(256)
+sweep_march_kernel(const SweepArgs a) {
+  constexpr int RAD = Radius<SCHEME>::value;
+  constexpr int WN = 2 * RAD + 1;
+  constexpr int NV = NVar<NS>::value;
+  constexpr int NF = 5 + NS;
+  const int j = a.h + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int o = a.h + (int)blockIdx.y * (int)blockDim.y + (int)threadIdx.y;
+  const int n_march = DIR == 0 ? a.nx : a.nz;
+  const int n_other = DIR == 0 ? a.nz : a.nx;
+  if (j >= a.h + a.ny || o >= a.h + n_other) return;
+  const int c_begin = a.h + (int)blockIdx.z * a.seg_len;
+  const int c_end = min(c_begin + a.seg_len, a.h + n_march);
+  const long long sm = DIR == 0 ? a.sx : a.sz;
+  const long long so = DIR == 0 ? a.sz : a.sx;
+  const long long base = (long long)o * so + j;
+  // Cell (k, i, j) of march coordinate c.
+  auto kk = [&](int c) { return DIR == 0 ? o : c; };
+  auto ii = [&](int c) { return DIR == 0 ? c : o; };
+
+  float W[WN][NV];
+#pragma unroll
+  for (int w = 0; w < WN - 1; ++w) {
+    const int c = c_begin - 1 - RAD + w;
+    load_prim<NS>(a, base + (long long)c * sm, in_core(c, a.h, n_march),
+                  pe_at(a, kk(c), ii(c), j), W[w]);
+  }
+  float negPrev[NV];
+  float Fprev[NF];
+  float tPrev[2][2];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) negPrev[v] = 0.0f;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) Fprev[f] = 0.0f;
+  tPrev[0][0] = tPrev[0][1] = tPrev[1][0] = tPrev[1][1] = 0.0f;
+
+  for (int c = c_begin - 1; c <= c_end; ++c) {
+    {
+      const int cn = c + RAD;
+      load_prim<NS>(a, base + (long long)cn * sm, in_core(cn, a.h, n_march),
+                    pe_at(a, kk(cn), ii(cn), j), W[WN - 1]);
+    }
+    float negP[NV], posM[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      float s[WN];
+#pragma unroll
+      for (int w = 0; w < WN; ++w) s[w] = W[w][v];
+      recon<SCHEME>(s, negP[v], posM[v]);
+    }
+    float tCur[2][2];
+    if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
+    if (c >= c_begin) {
+      float F[NF];
+      riemann<FLUX, DIR, NS>(negPrev, posM, pe_at(a, kk(c), ii(c), j), F);
+      if constexpr (VISC) {
+        viscous_flux<DIR, NS>(W[RAD - 1], W[RAD], tPrev, tCur, a.vc, F);
+      }
+      if (c > c_begin) {
+        const long long idx = base + (long long)(c - 1) * sm;
+        if constexpr (PHASE == 0) {
+          const float rho = W[RAD - 1][kRho];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) / a.hdir;
+          }
+        } else {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            store_last(a, f, idx, a.r.p[f][idx] - (F[f] - Fprev[f]) / a.hdir);
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Fprev[f] = F[f];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) negPrev[v] = negP[v];
+    if constexpr (VISC) {
+      tPrev[0][0] = tCur[0][0];
+      tPrev[0][1] = tCur[0][1];
+      tPrev[1][0] = tCur[1][0];
+      tPrev[1][1] = tCur[1][1];
+    }
+#pragma unroll
+    for (int w = 0; w < WN - 1; ++w) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) W[w][v] = W[w + 1][v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y sweep (contiguous direction).  Block (64, BY): one wave per (z, x) row
+// chunk; lane l holds cell y0 - K + l and the stencil comes from shuffles.
+// Outputs cells y0 .. y0 + OUTW - 1.  Middle sweep: R = R - div.
+// ---------------------------------------------------------------------------
+template <int SCHEME>
+struct YGeom {
+  static constexpr int RAD = Radius<SCHEME>::value;
+  static constexpr int K = RAD + 1;
+  static constexpr int OUTW = 64 - 2 * RAD - 2;
+};
+
+template <int SCHEME, int FLUX, bool VISC, int NS>
+__global__ void __launch_bounds__(256)
+sweep_y_kernel(const SweepArgs a) {
+  constexpr int RAD = YGeom<SCHEME>::RAD;
+  constexpr int K = YGeom<SCHEME>::K;
+  constexpr int OUTW = YGeom<SCHEME>::OUTW;
+  constexpr int WN = 2 * RAD + 1;
+  constexpr int NV = NVar<NS>::value;
+  constexpr int NF = 5 + NS;
+  const int lane = (int)threadIdx.x;
+  const int i = a.h + (int)blockIdx.y * (int)blockDim.y + (int)threadIdx.y;
+  const int k = a.h + (int)blockIdx.z;
+  if (i >= a.h + a.nx) return;  // uniform per wave
+  const int y0 = a.h + (int)blockIdx.x * OUTW;
+  const int y = y0 - K + lane;
+  const int yc = min(y, a.NY - 1);
+  const long long idx = (long long)k * a.sz + (long long)i * a.sx + yc;
+  float W0[NV];
+  load_prim<NS>(a, idx, in_core(yc, a.h, a.ny), pe_at(a, k, i, yc), W0);
+  float negP[NV], posM[NV], Am[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    float s[WN];
+#pragma unroll
+    for (int w = 0; w < WN; ++w) {
+      const int off = w - RAD;
+      s[w] = off == 0 ? W0[v] : __shfl(W0[v], lane + off, 64);
+    }
+    Am[v] = s[RAD - 1];
+    recon<SCHEME>(s, negP[v], posM[v]);
+  }
+  float negPrev[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) negPrev[v] = __shfl(negP[v], lane - 1, 64);
+  float F[NF];
+  riemann<FLUX, 1, NS>(negPrev, posM, pe_at(a, k, i, yc), F);
+  if constexpr (VISC) {
+    float tCur[2][2], tPrev[2][2];
+    transverse_grads<1>(a, k, i, yc, tCur);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      tPrev[t][0] = __shfl(tCur[t][0], lane - 1, 64);
+      tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
+    }
+    viscous_flux<1, NS>(Am, W0, tPrev, tCur, a.vc, F);
+  }
+  float Fn[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) Fn[f] = __shfl(F[f], lane + 1, 64);
+  if (lane >= K && lane < K + OUTW && y < a.h + a.ny) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      a.r.p[f][idx] = a.r.p[f][idx] - (Fn[f] - F[f]) / a.hdir;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// HLLC branch codes of the interior faces normal to D (parity/debug): one
+// thread per face, straightforward stencil loads.
+// ---------------------------------------------------------------------------
+template <int D, int SCHEME, int NS>
+__global__ void __launch_bounds__(256)
+branch_codes_kernel(const SweepArgs a, uint8_t* codes) {
+  constexpr int RAD = Radius<SCHEME>::value;
+  constexpr int WN = 2 * RAD + 1;
+  constexpr int NV = NVar<NS>::value;
+  const long long nfx = a.nx + (D == 0), nfy = a.ny + (D == 1), nfz = a.nz + (D == 2);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nfx * nfy * nfz) return;
+  const int j = a.h + (int)(t % nfy);
+  const int i = a.h + (int)((t / nfy) % nfx);
+  const int k = a.h + (int)(t / (nfy * nfx));
+  const long long s = D == 0 ? a.sx : (D == 1 ? 1 : a.sz);
+  const long long c = (long long)k * a.sz + (long long)i * a.sx + j;
+  const int pc = D == 0 ? i : (D == 1 ? j : k);
+  const int n = D == 0 ? a.nx : (D == 1 ? a.ny : a.nz);
+  float Wl[NV], Wr[NV];
+  float win[WN + 1][NV];
+#pragma unroll
+  for (int w = 0; w < WN + 1; ++w) {
+    const int off = w - RAD - 1;  // cells c-1-RAD .. c+RAD
+    const int kk = k + (D == 2) * off, ii = i + (D == 0) * off, jj = j + (D == 1) * off;
+    load_prim<NS>(a, c + off * s, in_core(pc + off, a.h, n) &&
+                  in_core(ii, a.h, a.nx) && in_core(jj, a.h, a.ny) && in_core(kk, a.h, a.nz),
+                  pe_at(a, kk, ii, jj), win[w]);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    float sl[WN], sr[WN], dummy;
+#pragma unroll
+    for (int w = 0; w < WN; ++w) {
+      sl[w] = win[w][v];
+      sr[w] = win[w + 1][v];
+    }
+    recon<SCHEME>(sl, Wl[v], dummy);
+    recon<SCHEME>(sr, dummy, Wr[v]);
+  }
+  float F[5 + NS];
+  codes[c] = (uint8_t)hllc_flux<D, NS>(Wl, Wr, pe_at(a, k, i, j), F);
+}
+
+}  // namespace sc

@@ -1,0 +1,132 @@
+"""ctypes binding of libswirlc_amd.so (the C ABI in include/swirlc.h).
+
+The shared library is built in-tree by swirl-c_amd/build_native.py.  There is
+no fallback: if the library is missing, or no GPU is visible when a compute
+entry point is called, an error is raised.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'lib',
+                         'libswirlc_amd.so')
+
+SC_OK = 0
+SC_MAX_FIELDS = 7
+SC_MAX_HALO = 4
+
+SC_WENO_3, SC_WENO_5, SC_MUSCL = 0, 1, 2
+SC_HLL, SC_HLLC = 0, 1
+SC_BC_PERIODIC, SC_BC_DIRICHLET, SC_BC_NEUMANN, SC_BC_REFLECTIVE = 0, 1, 2, 3
+SC_RK_EULER, SC_RK_BLEND, SC_RK_AVG = 1, 2, 3
+
+# Every entry point include/swirlc.h declares.
+EXPORTED = (
+    'sc_ctx_create', 'sc_ctx_destroy', 'sc_fill_halos', 'sc_rhs',
+    'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes', 'sc_sweep_only',
+    'sc_last_error', 'sc_version',
+)
+
+
+class ScBcFace(ctypes.Structure):
+  _fields_ = [
+      ('type', ctypes.c_int),
+      ('value', ctypes.c_float),
+      ('n_layer_values', ctypes.c_int),
+      ('layer_values', ctypes.c_float * SC_MAX_HALO),
+      ('layer_planes', ctypes.c_void_p),
+  ]
+
+
+class ScGrid(ctypes.Structure):
+  _fields_ = [
+      ('nx', ctypes.c_int), ('ny', ctypes.c_int), ('nz', ctypes.c_int),
+      ('halo', ctypes.c_int),
+      ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('dz', ctypes.c_double),
+      ('x_full', ctypes.POINTER(ctypes.c_float)),
+      ('y_full', ctypes.POINTER(ctypes.c_float)),
+      ('z_full', ctypes.POINTER(ctypes.c_float)),
+  ]
+
+
+class ScParams(ctypes.Structure):
+  _fields_ = [
+      ('n_fields', ctypes.c_int),
+      ('interpolation', ctypes.c_int),
+      ('numeric_flux', ctypes.c_int),
+      ('include_diffusion', ctypes.c_int),
+      ('nu', ctypes.c_double), ('pr', ctypes.c_double),
+      ('gravity_source', ctypes.c_int),
+      ('g', ctypes.c_double * 3),
+      ('bc', ((ScBcFace * 2) * 3) * SC_MAX_FIELDS),
+      ('prim_bc_mask', ctypes.c_int),
+  ]
+
+
+class NativeError(RuntimeError):
+  pass
+
+
+_lib = None
+
+
+def lib():
+  """Loads the shared library once; raises if it has not been built."""
+  global _lib
+  if _lib is None:
+    if not os.path.exists(_LIB_PATH):
+      raise NativeError(
+          f'{_LIB_PATH} is missing: build it with '
+          '`python swirl-c_amd/build_native.py` (no CPU fallback exists).')
+    L = ctypes.CDLL(_LIB_PATH)
+    vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    fpp = ctypes.POINTER(ctypes.c_void_p)
+    L.sc_ctx_create.argtypes = [ctypes.POINTER(ScGrid), ctypes.POINTER(ScParams),
+                                ctypes.POINTER(vp)]
+    L.sc_ctx_destroy.argtypes = [vp]
+    L.sc_fill_halos.argtypes = [vp, fpp, vp]
+    L.sc_rhs.argtypes = [vp, fpp, fpp, vp]
+    L.sc_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
+    L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
+    L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_last_error.restype = ctypes.c_char_p
+    L.sc_version.restype = ctypes.c_char_p
+    for name in ('sc_ctx_create', 'sc_ctx_destroy', 'sc_fill_halos', 'sc_rhs',
+                 'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes',
+                 'sc_sweep_only'):
+      getattr(L, name).restype = ctypes.c_int
+    _lib = L
+  return _lib
+
+
+def check(status, what):
+  if status != SC_OK:
+    msg = lib().sc_last_error().decode()
+    raise NativeError(f'{what} failed (status {status}): {msg}')
+
+
+def ptr_array(tensors):
+  """Host array of device pointers for a list of tensors (None -> NULL)."""
+  arr = (ctypes.c_void_p * SC_MAX_FIELDS)()
+  for k, t in enumerate(tensors):
+    arr[k] = None if t is None else t.data_ptr()
+  return arr
+
+
+def stream_handle(stream=None):
+  s = stream if stream is not None else torch.cuda.current_stream()
+  return ctypes.c_void_p(s.cuda_stream)
+
+
+def require_gpu_tensor(t, name):
+  if not isinstance(t, torch.Tensor):
+    raise TypeError(f'{name} must be a torch.Tensor, got {type(t)}')
+  if not t.is_cuda:
+    raise ValueError(f'{name} must be a GPU tensor (no CPU path exists)')
+  if t.dtype != torch.float32:
+    raise ValueError(f'{name} must be float32, got {t.dtype}')
+  if not t.is_contiguous():
+    raise ValueError(f'{name} must be contiguous')

@@ -1,0 +1,265 @@
+"""Translation of a SwirlCParameters config into a native (HIP) context.
+
+`NativeContext(cfg)` validates the configuration the way the reference does
+at trace time (get_rhs_fn, PhysicsModels, select_numeric_flux_fn, the
+face-interpolation dispatch) and owns one `sc_ctx` of libswirlc_amd.  Its
+methods launch the HIP kernels on the current torch stream.
+"""
+
+import ctypes
+import numbers
+
+import numpy as np
+import torch
+
+from swirlc_amd import _native as N
+from swirlc_amd.boundary import bc_types
+from swirlc_amd.common import types
+from swirlc_amd.common import utils
+
+_INTERP = {'WENO_3': N.SC_WENO_3, 'WENO_5': N.SC_WENO_5, 'MUSCL': N.SC_MUSCL}
+_FLUX = {'HLL': N.SC_HLL, 'HLLC': N.SC_HLLC}
+_BCT = {
+    bc_types.BoundaryCondition.PERIODIC: N.SC_BC_PERIODIC,
+    bc_types.BoundaryCondition.DIRICHLET: N.SC_BC_DIRICHLET,
+    bc_types.BoundaryCondition.NEUMANN: N.SC_BC_NEUMANN,
+    bc_types.BoundaryCondition.REFLECTIVE: N.SC_BC_REFLECTIVE,
+}
+
+
+def _get(cfg, key, default=None):
+  return getattr(cfg, key, default) if not isinstance(cfg, dict) else cfg.get(key, default)
+
+
+def validate_models(cfg):
+  """PhysicsModels.__init__ checks (physics/physics_models.py:45-92)."""
+  if _get(cfg, 'thermodynamics_model', 'generic') != 'generic':
+    raise NotImplementedError(
+        f'Unsupported thermodynamics model: "{_get(cfg, "thermodynamics_model")}".'
+        " Supported models are: ['generic']")
+  if _get(cfg, 'transport_model', 'simple') != 'simple':
+    raise NotImplementedError(
+        f'Unsupported transport model: "{_get(cfg, "transport_model")}".'
+        " Supported models are: ['simple']")
+  for name in _get(cfg, 'source_functions') or []:
+    if name != 'gravity':
+      raise NotImplementedError(
+          f'Unsupported source term model: "{name}". Supported models are:'
+          " ['gravity']")
+
+
+class NativeContext:
+  """One native RHS/RK context for a fixed grid and configuration."""
+
+  def __init__(self, cfg, device=None):
+    validate_models(cfg)
+    scheme = _get(cfg, 'interpolation_scheme')
+    if scheme not in _INTERP:
+      raise NotImplementedError(
+          f'"{scheme}" is not implemented as an "interpolation_scheme". Valid'
+          " options are: ['WENO_3', 'WENO_5', 'MUSCL']")
+    flux = _get(cfg, 'numeric_flux_scheme')
+    if flux not in _FLUX:
+      raise ValueError(
+          f'"{flux}" is not implemented as a "numeric_flux_scheme". Valid'
+          " options are: ['HLL', 'HLLC']")
+    if (_get(cfg, 'cx', 1), _get(cfg, 'cy', 1), _get(cfg, 'cz', 1)) != (1, 1, 1):
+      raise NotImplementedError(
+          'multi-replica domain decomposition is not supported: one GPU runs'
+          ' one replica (cx = cy = cz = 1)')
+    bc = _get(cfg, 'bc')
+    if 'cell_faces' in bc:
+      if _get(cfg, 'include_diffusion'):
+        raise NotImplementedError(
+            'Face boundary conditions are not supported with diffusion enabled'
+            ' at this time.')
+      raise NotImplementedError("'cell_faces' boundary conditions are not supported yet")
+    if 'intercell_fluxes' in bc:
+      raise NotImplementedError("'intercell_fluxes' boundary conditions are not supported yet")
+
+    self.names = list(_get(cfg, 'conservative_variable_names'))
+    if self.names[:5] != list(types.BASE_CONSERVATIVE):
+      raise ValueError(
+          f'conservative_variable_names must start with {list(types.BASE_CONSERVATIVE)}')
+    if len(self.names) > 6:
+      raise NotImplementedError('at most one transported scalar is supported')
+    self.nf = len(self.names)
+    self.hw = int(_get(cfg, 'halo_width'))
+    self.n = (int(_get(cfg, 'core_nx')), int(_get(cfg, 'core_ny')), int(_get(cfg, 'core_nz')))
+    self.shape = (self.n[2] + 2 * self.hw, self.n[0] + 2 * self.hw, self.n[1] + 2 * self.hw)
+    self.device = torch.device('cuda') if device is None else torch.device(device)
+    self.h = (float(_get(cfg, 'dx')), float(_get(cfg, 'dy')), float(_get(cfg, 'dz')))
+    g = _get(cfg, 'g')
+    self.g = tuple(float(g[d]) for d in types.DIMS)
+    self.gravity_dir = utils.gravity_direction(g)
+    self.gravity_source = 'gravity' in (_get(cfg, 'source_functions') or [])
+    if self.gravity_source and self.gravity_dir == -1:
+      raise ValueError('Gravity direction must be specified if the gravity source'
+                       ' function is to be used.')
+    self._keep = []  # device tensors referenced by the native context
+
+    grid = N.ScGrid()
+    grid.nx, grid.ny, grid.nz = self.n
+    grid.halo = self.hw
+    grid.dx, grid.dy, grid.dz = self.h
+    if self.gravity_dir != -1:
+      for attr, key in (('x_full', 'x'), ('y_full', 'y'), ('z_full', 'z')):
+        arr = np.ascontiguousarray(utils.mesh_full(_get(cfg, key), self.hw))
+        self._keep.append(arr)
+        setattr(grid, attr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+    params = N.ScParams()
+    params.n_fields = self.nf
+    params.interpolation = _INTERP[scheme]
+    params.numeric_flux = _FLUX[flux]
+    params.include_diffusion = 1 if _get(cfg, 'include_diffusion') else 0
+    tp = _get(cfg, 'transport_parameters')
+    params.nu = float(tp['nu'])
+    params.pr = float(tp['pr'])
+    params.gravity_source = 1 if self.gravity_source else 0
+    for d in range(3):
+      params.g[d] = self.g[d]
+    cell_bc = bc['cell_averages']
+    mask = 0
+    for f, name in enumerate(self.names):
+      if name in cell_bc:
+        spec = cell_bc[name]
+      elif types.conservative_to_primitive_name(name) in cell_bc:
+        spec = cell_bc[types.conservative_to_primitive_name(name)]
+        mask |= 1 << f
+      else:
+        raise ValueError(
+            'A "cells" boundary condition is required for all conservative'
+            ' variables, either directly or through their respective primitive.'
+            f' No boundary condition was found for {name} or'
+            f' {types.conservative_to_primitive_name(name)}.')
+      for d, dim in enumerate(types.DIMS):
+        if dim not in spec:
+          raise ValueError(f'boundary condition of {name} lacks dim {dim}')
+        self._fill_dim_bc(params.bc[f][d], spec[dim], d)
+    params.prim_bc_mask = mask
+    self.prim_bc_mask = mask
+
+    L = N.lib()
+    handle = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      N.check(L.sc_ctx_create(ctypes.byref(grid), ctypes.byref(params),
+                              ctypes.byref(handle)), 'sc_ctx_create')
+    self._ctx = handle
+
+  # -- BC translation ------------------------------------------------------
+
+  def _plane_shape(self, d):
+    nz, nx, ny = self.shape
+    return [(nz, ny), (nz, nx), (nx, ny)][d]
+
+  def _layer_plane(self, val, d):
+    a, b = self._plane_shape(d)
+    if isinstance(val, (list, tuple)):
+      val = np.stack([np.asarray(v.detach().cpu() if torch.is_tensor(v) else v,
+                                 dtype=np.float32) for v in val])
+    arr = np.asarray(val.detach().cpu() if torch.is_tensor(val) else val, dtype=np.float32)
+    if arr.ndim == 3:
+      # A 3-D layer slice with a unit extent along the BC dimension.
+      full = [self.shape[0], self.shape[1], self.shape[2]]
+      full[[1, 2, 0][d]] = 1
+      arr = np.broadcast_to(arr, full).reshape(a, b)
+    return np.ascontiguousarray(np.broadcast_to(arr, (a, b)), dtype=np.float32)
+
+  def _fill_dim_bc(self, faces, dim_spec, d):
+    types_ = [dim_spec[s][0] for s in (0, 1)]
+    periodic = bc_types.BoundaryCondition.PERIODIC in types_
+    for s in (0, 1):
+      out = faces[s]
+      entry = dim_spec[s]
+      t = entry[0]
+      if t not in _BCT:
+        raise NotImplementedError(f'Unsupported BC type {t}.')
+      if periodic:
+        out.type = N.SC_BC_PERIODIC
+        continue
+      if t == bc_types.BoundaryCondition.REFLECTIVE:
+        raise NotImplementedError(f'Unsupported BC type {t}.')
+      out.type = _BCT[t]
+      val = entry[1]
+      if t == bc_types.BoundaryCondition.NEUMANN:
+        if not isinstance(val, numbers.Number):
+          raise NotImplementedError('NEUMANN values must be scalars')
+        out.value = float(val)
+      elif isinstance(val, numbers.Number):
+        out.value = float(val)
+      elif isinstance(val, (list, tuple)):
+        if len(val) != self.hw:
+          raise ValueError('a per-layer DIRICHLET value needs halo_width layers')
+        if all(isinstance(v, numbers.Number) for v in val):
+          out.n_layer_values = self.hw
+          for i, v in enumerate(val):
+            out.layer_values[i] = float(v)
+        else:
+          planes = np.stack([self._layer_plane(v, d) for v in val])
+          dev = torch.from_numpy(planes).to(self.device).contiguous()
+          self._keep.append(dev)
+          out.layer_planes = dev.data_ptr()
+      else:
+        raise NotImplementedError(f'unsupported DIRICHLET value of type {type(val)}')
+
+  # -- calls ---------------------------------------------------------------
+
+  def close(self):
+    if getattr(self, '_ctx', None) is not None and self._ctx.value:
+      N.lib().sc_ctx_destroy(self._ctx)
+      self._ctx = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:  # interpreter shutdown
+      pass
+
+  def fields(self, state):
+    out = []
+    for name in self.names:
+      if name not in state:
+        raise KeyError(f'{name} missing from the conservative state')
+      t = state[name]
+      N.require_gpu_tensor(t, name)
+      if tuple(t.shape) != self.shape:
+        raise ValueError(f'{name} has shape {tuple(t.shape)}, expected {self.shape}')
+      out.append(t)
+    return out
+
+  def fill_halos(self, q, stream=None):
+    N.check(N.lib().sc_fill_halos(self._ctx, N.ptr_array(q), N.stream_handle(stream)),
+            'sc_fill_halos')
+
+  def rhs(self, q, r, stream=None):
+    N.check(N.lib().sc_rhs(self._ctx, N.ptr_array(q), N.ptr_array(r),
+                           N.stream_handle(stream)), 'sc_rhs')
+
+  def rk_stage(self, form, c0, c1, dt_dev, u0, uk, out, stream=None):
+    N.check(N.lib().sc_rk_stage(
+        self._ctx, form, ctypes.c_float(c0), ctypes.c_float(c1),
+        ctypes.c_void_p(dt_dev.data_ptr()),
+        N.ptr_array(u0) if u0 is not None else None,
+        N.ptr_array(uk), N.ptr_array(out), N.stream_handle(stream)), 'sc_rk_stage')
+
+  def sweep_only(self, dim, form, c0, c1, dt_dev, u0, uk, out, stream=None):
+    """Launches one directional sweep kernel alone (for kernel timing)."""
+    N.check(N.lib().sc_sweep_only(
+        self._ctx, int(dim), int(form), ctypes.c_float(c0), ctypes.c_float(c1),
+        ctypes.c_void_p(dt_dev.data_ptr()) if dt_dev is not None else None,
+        N.ptr_array(u0) if u0 is not None else None, N.ptr_array(uk),
+        N.ptr_array(out) if out is not None else None,
+        N.stream_handle(stream)), 'sc_sweep_only')
+
+  def max_wave_speed(self, q, out_dev, cfl=0.0, dt_dev=None, stream=None):
+    N.check(N.lib().sc_max_wave_speed(
+        self._ctx, N.ptr_array(q), ctypes.c_float(cfl),
+        ctypes.c_void_p(out_dev.data_ptr()),
+        ctypes.c_void_p(dt_dev.data_ptr()) if dt_dev is not None else None,
+        N.stream_handle(stream)), 'sc_max_wave_speed')
+
+  def hllc_branch_codes(self, q, dim, codes, stream=None):
+    N.check(N.lib().sc_hllc_branch_codes(
+        self._ctx, N.ptr_array(q), int(dim), ctypes.c_void_p(codes.data_ptr()),
+        N.stream_handle(stream)), 'sc_hllc_branch_codes')

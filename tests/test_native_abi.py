@@ -1,0 +1,106 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point that
+include/swirlc.h declares, and rejects invalid configurations the way the
+reference rejects them at trace time (no kernel is launched)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from swirlc_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+  txt = open(os.path.join(ROOT, 'include', 'swirlc.h')).read()
+  return sorted(set(re.findall(r'\b(sc_[a-z_]+)\s*\(', txt)))
+
+
+def test_header_and_binding_agree():
+  assert sorted(N.EXPORTED) == _declared()
+
+
+def test_library_exports_every_declared_symbol():
+  lib = N.lib()
+  for name in _declared():
+    assert hasattr(lib, name), name
+  assert lib.sc_version().decode().startswith('swirlc_amd')
+
+
+def _grid(n=8, h=3):
+  g = N.ScGrid()
+  g.nx = g.ny = g.nz = n
+  g.halo = h
+  g.dx = g.dy = g.dz = 1.0 / n
+  return g
+
+
+def _params(interp=N.SC_WENO_5, flux=N.SC_HLLC):
+  p = N.ScParams()
+  p.n_fields = 5
+  p.interpolation = interp
+  p.numeric_flux = flux
+  p.nu, p.pr = 1e-4, 0.7
+  return p
+
+
+def _create(g, p):
+  h = ctypes.c_void_p()
+  st = N.lib().sc_ctx_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
+  return st, N.lib().sc_last_error().decode()
+
+
+@pytest.mark.parametrize('interp,flux,needle', [
+    (7, N.SC_HLLC, 'interpolation_scheme'),
+    (N.SC_WENO_5, 9, 'numeric_flux_scheme'),
+])
+def test_unknown_schemes_rejected(interp, flux, needle):
+  st, msg = _create(_grid(), _params(interp, flux))
+  assert st != N.SC_OK and needle in msg
+
+
+def test_halo_too_small_for_weno5_rejected():
+  st, msg = _create(_grid(h=2), _params())
+  assert st == 1 and 'halo_width' in msg
+
+
+def test_gravity_without_direction_rejected():
+  p = _params()
+  p.gravity_source = 1
+  st, msg = _create(_grid(), p)
+  assert st != N.SC_OK and 'Gravity direction' in msg
+
+
+def test_reflective_rejected():
+  p = _params()
+  p.bc[0][0][0].type = N.SC_BC_REFLECTIVE
+  st, msg = _create(_grid(), p)
+  assert st == 2 and 'REFLECTIVE' in msg
+
+
+def test_periodic_with_too_few_cells_rejected():
+  st, msg = _create(_grid(n=2), _params(N.SC_MUSCL))
+  assert st != N.SC_OK and 'PERIODIC' in msg
+
+
+def test_null_arguments_rejected():
+  L = N.lib()
+  assert L.sc_rhs(None, None, None, None) == 1
+  assert L.sc_fill_halos(None, None, None) == 1
+  assert L.sc_rk_stage(None, 1, 0.0, 0.0, None, None, None, None, None) == 1
+
+
+def test_python_config_validation_errors():
+  """The Python host raises the reference's exception types before any GPU use."""
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.equations.native_context import NativeContext
+  with pytest.raises(NotImplementedError, match='interpolation_scheme'):
+    NativeContext(SwirlCParameters({'interpolation_scheme': 'WENO_7'}))
+  with pytest.raises(ValueError, match='numeric_flux_scheme'):
+    NativeContext(SwirlCParameters({'numeric_flux_scheme': 'ROE'}))
+  with pytest.raises(NotImplementedError, match='source term'):
+    NativeContext(SwirlCParameters({'source_functions': ['coriolis']}))
+  with pytest.raises(ValueError, match='cells'):
+    NativeContext(SwirlCParameters({'bc': {'cell_averages': {}}}))
