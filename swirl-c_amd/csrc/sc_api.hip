@@ -231,9 +231,9 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
     a.vc.h[d] = c->h[d];
     a.vc.inv_h[d] = 1.0f / c->h[d];
   }
-  a.vc.two_h[0] = (float)(2.0 * grid->dx);
-  a.vc.two_h[1] = (float)(2.0 * grid->dy);
-  a.vc.two_h[2] = (float)(2.0 * grid->dz);
+  a.vc.inv_two_h[0] = 1.0f / (float)(2.0 * grid->dx);
+  a.vc.inv_two_h[1] = 1.0f / (float)(2.0 * grid->dy);
+  a.vc.inv_two_h[2] = 1.0f / (float)(2.0 * grid->dz);
   // Coordinates are not kept: the context only needs the pe tables.
   c->grid.x_full = c->grid.y_full = c->grid.z_full = nullptr;
   *out = c;

@@ -49,12 +49,24 @@ struct NVar {
   static constexpr int value = 7 + NS;
 };
 
+// Hardware reciprocal and square root (v_rcp_f32 / v_sqrt_f32, 1 ulp).  The
+// reference's IEEE divisions become reciprocal-multiplies: every result stays
+// within a few ulps of the IEEE value, far inside the 1e-5 parity bound, and
+// a division costs 2 instructions instead of about 10.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // tf.math.divide_no_nan / reciprocal_no_nan.
 __device__ __forceinline__ float dnn(float x, float y) {
-  return y == 0.0f ? 0.0f : x / y;
+  return y == 0.0f ? 0.0f : x * frcp(y);
 }
 __device__ __forceinline__ float rnn(float x) {
-  return x == 0.0f ? 0.0f : 1.0f / x;
+  return x == 0.0f ? 0.0f : frcp(x);
+}
+// divide_no_nan with an IEEE division, for the boundary-condition paths whose
+// values must match the reference bit for bit (cheap: halo slabs only).
+__device__ __forceinline__ float dnn_exact(float x, float y) {
+  return y == 0.0f ? 0.0f : x / y;
 }
 
 // Conserved (rho, rho_u, rho_v, rho_w, rho_e, rho_s..) -> primitives
@@ -64,7 +76,7 @@ __device__ __forceinline__ float rnn(float x) {
 template <int NS>
 __device__ __forceinline__ void cons_to_prim(const float* q, float pe, float* w) {
   const float rho = q[0];
-  const float inv = 1.0f / rho;
+  const float inv = frcp(rho);
   w[kRho] = rho;
   w[kU] = inv * q[1];
   w[kV] = inv * q[2];
@@ -75,7 +87,7 @@ __device__ __forceinline__ void cons_to_prim(const float* q, float pe, float* w)
   const float ke = 0.5f * ((w[kU] * w[kU] + w[kV] * w[kV]) + w[kW] * w[kW]);
   const float eint = (w[kE] - ke) - pe;
   w[kP] = (kGm1 * rho) * eint;
-  w[kH] = w[kE] + w[kP] / rho;
+  w[kH] = w[kE] + w[kP] * inv;
 }
 
 // ---------------------------------------------------------------------------
@@ -113,16 +125,16 @@ __device__ __forceinline__ void recon<kWeno5>(const float* a, float& fp, float& 
   const float qp0 = (c116 * v0 + c76 * vp1) + c13 * vp2;
   const float qp1 = (c13 * vm1 + c56 * v0) + c16 * vp1;
   const float qp2 = (c16 * vm2 + c56 * vm1) + c13 * v0;
+  // alpha_r = d_r / (eps + beta_r)^2: one reciprocal per stencil serves the
+  // weights of both faces; omega_r = alpha_r / sum(alpha).
   float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
-  e0 = e0 * e0;
-  e1 = e1 * e1;
-  e2 = e2 * e2;
-  const float an0 = 0.3f / e0, an1 = 0.6f / e1, an2 = 0.1f / e2;
-  const float ap0 = 0.1f / e0, ap1 = 0.6f / e1, ap2 = 0.3f / e2;
-  const float sn = (an0 + an1) + an2;
-  const float sp = (ap0 + ap1) + ap2;
-  fp = ((an0 / sn) * qn0 + (an1 / sn) * qn1) + (an2 / sn) * qn2;
-  fm = ((ap0 / sp) * qp0 + (ap1 / sp) * qp1) + (ap2 / sp) * qp2;
+  const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1), i2 = frcp(e2 * e2);
+  const float an0 = 0.3f * i0, an1 = 0.6f * i1, an2 = 0.1f * i2;
+  const float ap0 = 0.1f * i0, ap1 = 0.6f * i1, ap2 = 0.3f * i2;
+  const float rn = frcp((an0 + an1) + an2);
+  const float rp = frcp((ap0 + ap1) + ap2);
+  fp = ((an0 * qn0 + an1 * qn1) + an2 * qn2) * rn;
+  fm = ((ap0 * qp0 + ap1 * qp1) + ap2 * qp2) * rp;
 }
 
 template <>
@@ -138,14 +150,11 @@ __device__ __forceinline__ void recon<kWeno3>(const float* a, float& fp, float& 
   const float qp1 = 0.5f * vm1 + 0.5f * v0;
   const float d0 = (float)(2.0 / 3.0), d1 = (float)(1.0 / 3.0);
   float e0 = kWenoEps + b0, e1 = kWenoEps + b1;
-  e0 = e0 * e0;
-  e1 = e1 * e1;
-  const float an0 = d0 / e0, an1 = d1 / e1;
-  const float ap0 = d1 / e0, ap1 = d0 / e1;
-  const float sn = an0 + an1;
-  const float sp = ap0 + ap1;
-  fp = (an0 / sn) * qn0 + (an1 / sn) * qn1;
-  fm = (ap0 / sp) * qp0 + (ap1 / sp) * qp1;
+  const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1);
+  const float an0 = d0 * i0, an1 = d1 * i1;
+  const float ap0 = d1 * i0, ap1 = d0 * i1;
+  fp = (an0 * qn0 + an1 * qn1) * frcp(an0 + an1);
+  fm = (ap0 * qp0 + ap1 * qp1) * frcp(ap0 + ap1);
 }
 
 __device__ __forceinline__ float sgnf(float x) {
@@ -156,12 +165,13 @@ template <>
 __device__ __forceinline__ void recon<kMuscl>(const float* a, float& fp, float& fm) {
   const float vm1 = a[0], v0 = a[1], vp1 = a[2];
   // minmod(central2(v, 2), forward1(v, 1), backward1(v, 1)).
-  const float g0 = (vp1 - vm1) / 4.0f;
+  const float g0 = (vp1 - vm1) * 0.25f;
   const float g1 = vp1 - v0;
   const float g2 = v0 - vm1;
-  const float s = ((sgnf(g0) + sgnf(g1)) + sgnf(g2)) / 3.0f;
+  // sign(sum)/3 is +-1 exactly when all three signs agree (|sum| == 3).
+  const float s = (sgnf(g0) + sgnf(g1)) + sgnf(g2);
   const float amin = fminf(fminf(fabsf(g0), fabsf(g1)), fabsf(g2));
-  const float m = fabsf(s) == 1.0f ? s * amin : 0.0f;
+  const float m = fabsf(s) == 3.0f ? copysignf(amin, s) : 0.0f;
   fp = v0 + 0.5f * m;
   fm = v0 - 0.5f * m;
 }
@@ -181,7 +191,7 @@ __device__ __forceinline__ void physical_flux(const float* s, float* f) {
   f[1] = D == 0 ? m * s[kU] + s[kP] : m * s[kU];
   f[2] = D == 1 ? m * s[kV] + s[kP] : m * s[kV];
   f[3] = D == 2 ? m * s[kW] + s[kP] : m * s[kW];
-  f[4] = m * (s[kE] + s[kP] / s[kRho]);
+  f[4] = m * (s[kE] + s[kP] * frcp(s[kRho]));
 #pragma unroll
   for (int k = 0; k < NS; ++k) f[5 + k] = m * s[7 + k];
 }
@@ -191,17 +201,17 @@ __device__ __forceinline__ void physical_flux(const float* s, float* f) {
 template <int D>
 __device__ __forceinline__ void roe_speeds(const float* L, const float* R,
                                            float pe, float& sl, float& sr) {
-  const float cl = sqrtf(kGamma * L[kP] / L[kRho]);
-  const float cr = sqrtf(kGamma * R[kP] / R[kRho]);
-  const float ql = sqrtf(L[kRho]);
-  const float qr = sqrtf(R[kRho]);
+  const float cl = fsqrt(kGamma * L[kP] * frcp(L[kRho]));
+  const float cr = fsqrt(kGamma * R[kP] * frcp(R[kRho]));
+  const float ql = fsqrt(L[kRho]);
+  const float qr = fsqrt(R[kRho]);
   const float inv = rnn(ql + qr);
   const float ru = (ql * L[kU] + qr * R[kU]) * inv;
   const float rv = (ql * L[kV] + qr * R[kV]) * inv;
   const float rw = (ql * L[kW] + qr * R[kW]) * inv;
   const float rh = (ql * L[kH] + qr * R[kH]) * inv;
   const float hs = (rh - 0.5f * ((ru * ru + rv * rv) + rw * rw)) - pe;
-  const float c = sqrtf(kGm1 * hs);
+  const float c = fsqrt(kGm1 * hs);
   const float un = D == 0 ? ru : (D == 1 ? rv : rw);
   sl = fminf(L[kU + D] - cl, un - c);
   sr = fmaxf(R[kU + D] + cr, un + c);
@@ -215,9 +225,10 @@ __device__ __forceinline__ void hll_flux(const float* L, const float* R, float p
   float sl, sr;
   roe_speeds<D>(L, R, pe, sl, sr);
   const float ds = sr - sl;
-  const float t1 = dnn(fminf(sr, 0.0f) - fminf(sl, 0.0f), ds);
+  const float rds = ds == 0.0f ? 0.0f : frcp(ds);
+  const float t1 = (fminf(sr, 0.0f) - fminf(sl, 0.0f)) * rds;
   const float t2 = 1.0f - t1;
-  const float t3 = dnn(sr * fabsf(sl) - sl * fabsf(sr), 2.0f * ds);
+  const float t3 = (sr * fabsf(sl) - sl * fabsf(sr)) * (0.5f * rds);
   float fl[NF], fr[NF], ul[NF], ur[NF];
   physical_flux<D, NS>(L, fl);
   physical_flux<D, NS>(R, fr);
@@ -245,19 +256,21 @@ template <int D, int NS>
 __device__ __forceinline__ void hllc_star(const float* S, float sk, float sm,
                                           const float* f, float* fs) {
   const float rho = S[kRho], vn = S[kU + D];
-  const float q = dnn(sk, sk - sm);
+  const float den = sk - sm;
+  const float r = den == 0.0f ? 0.0f : frcp(den);
+  const float q = sk * r;
   const float a = rho * (sm - vn);
   fs[0] = f[0] + a * q;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (k == D) {
-      fs[1 + k] = f[1 + k] + (rho * sk) * (dnn(sk - vn, sk - sm) * sm - vn);
+      fs[1 + k] = f[1 + k] + (rho * sk) * (((sk - vn) * r) * sm - vn);
     } else {
       fs[1 + k] = f[1 + k] + (a * S[kU + k]) * q;
     }
   }
   fs[4] = f[4] + ((rho * q) * (sm - vn)) *
-                     ((S[kE] + S[kP] / rho) + (sk - vn) * sm);
+                     ((S[kE] + S[kP] * frcp(rho)) + (sk - vn) * sm);
 #pragma unroll
   for (int k = 0; k < NS; ++k) fs[5 + k] = f[5 + k] + (a * S[7 + k]) * q;
 }
@@ -329,9 +342,9 @@ __device__ __forceinline__ int riemann(const float* L, const float* R, float pe,
 struct ViscConst {
   float nu;        // kinematic viscosity
   float kfac;      // (1/Pr) * cp
-  float inv_h[3];  // unused placeholder for alignment
-  float h[3];      // dx, dy, dz
-  float two_h[3];  // 2 dx, 2 dy, 2 dz
+  float inv_h[3];      // 1/dx, 1/dy, 1/dz
+  float h[3];          // dx, dy, dz
+  float inv_two_h[3];  // 1/(2 dx), 1/(2 dy), 1/(2 dz)
 };
 
 template <int D, int NS>
@@ -339,11 +352,11 @@ __device__ __forceinline__ void viscous_flux(const float* A, const float* B,
                                              const float (*ta)[2],
                                              const float (*tb)[2],
                                              const ViscConst& vc, float* F) {
-  const float h = vc.h[D];
+  const float ih = vc.inv_h[D];
   // Backward (normal) derivatives on the face.
-  const float gu = (B[kU] - A[kU]) / h;
-  const float gv = (B[kV] - A[kV]) / h;
-  const float gw = (B[kW] - A[kW]) / h;
+  const float gu = (B[kU] - A[kU]) * ih;
+  const float gv = (B[kV] - A[kV]) * ih;
+  const float gw = (B[kW] - A[kW]) * ih;
   const float gn[3] = {gu, gv, gw};
   // Transverse derivatives interpolated onto the face: 0.5*(f[c-1] + f[c]).
   float it[2][2];
@@ -388,12 +401,12 @@ __device__ __forceinline__ void viscous_flux(const float* A, const float* B,
   const float vf = 0.5f * (A[kV] + B[kV]);
   const float wf = 0.5f * (A[kW] + B[kW]);
   // Heat flux -kappa_f * dT/dx, T = p/(rho R), kappa = ((1/Pr)*cp)*(nu*rho).
-  const float Ta = A[kP] / (A[kRho] * kR);
-  const float Tb = B[kP] / (B[kRho] * kR);
+  const float Ta = A[kP] * frcp(A[kRho] * kR);
+  const float Tb = B[kP] * frcp(B[kRho] * kR);
   const float ka = vc.kfac * (vc.nu * A[kRho]);
   const float kb = vc.kfac * (vc.nu * B[kRho]);
   const float kf = 0.5f * (ka + kb);
-  const float q = -kf * ((Tb - Ta) / h);
+  const float q = -kf * ((Tb - Ta) * ih);
   F[1] = F[1] + (-tx);
   F[2] = F[2] + (-ty);
   F[3] = F[3] + (-tz);

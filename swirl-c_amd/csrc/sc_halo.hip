@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256) halo_dim_kernel(const HaloArgs a, int d) 
     const long long idx = base + (long long)c * s;
     const float x = v[idx];
     if (!prim || earlier_halo) return x;
-    return dnn(x, rho[idx]);
+    return dnn_exact(x, rho[idx]);
   };
   auto put = [&](int c, float x) { v[base + (long long)c * s] = x; };
 

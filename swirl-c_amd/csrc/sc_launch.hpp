@@ -38,7 +38,7 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
   // x sweep (first): R = src - div_x.
   if (only_dir < 0 || only_dir == 0) {
     SweepArgs ax = a;
-    ax.hdir = h[0];
+    ax.inv_hdir = 1.0f / h[0];
     const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_march - 1) / tune.by_march,
                     (a.nx + tune.seg_len - 1) / tune.seg_len);
     hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>), grid,
@@ -47,7 +47,7 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
   // y sweep (middle): R -= div_y.
   if (only_dir < 0 || only_dir == 1) {
     SweepArgs ay = a;
-    ay.hdir = h[1];
+    ay.inv_hdir = 1.0f / h[1];
     constexpr int OUTW = YGeom<SCHEME>::OUTW;
     const dim3 grid((a.ny + OUTW - 1) / OUTW, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
     hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS>), grid, blk_y, 0,
@@ -56,7 +56,7 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
   // z sweep (last): R -= div_z, then store R or the RK update.
   if (only_dir < 0 || only_dir == 2) {
     SweepArgs az = a;
-    az.hdir = h[2];
+    az.inv_hdir = 1.0f / h[2];
     const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
                     (a.nz + tune.seg_len - 1) / tune.seg_len);
     hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>), grid,

@@ -35,7 +35,7 @@ struct SweepArgs {
   int nx, ny, nz, h;
   int NX, NY, NZ;
   long long sx, sz;  // strides of x and z (y stride is 1)
-  float hdir;        // spacing along the sweep direction
+  float inv_hdir;    // 1 / spacing along the sweep direction
   int seg_len;       // cells per thread (marching sweeps)
   int has_pe;        // gravitational potential present
   const float* pe_x; // pe = (pe_x[i] - pe_y[j]) - pe_z[k]
@@ -59,15 +59,25 @@ __device__ __forceinline__ float pe_at(const SweepArgs& a, int k, int i, int j) 
 // primitive BC get the reference's rho * (q / rho) rounding
 // (boundary/boundary.py:516-531).
 template <int NF>
-__device__ __forceinline__ void load_cons(const SweepArgs& a, long long idx,
-                                          bool interior, float* q) {
+__device__ __forceinline__ void load_raw(const SweepArgs& a, long long idx, float* q) {
 #pragma unroll
   for (int f = 0; f < NF; ++f) q[f] = a.q.p[f][idx];
+}
+
+template <int NF>
+__device__ __forceinline__ void prim_bc_round(const SweepArgs& a, bool interior, float* q) {
   if (a.prim_bc_mask && interior) {
 #pragma unroll
     for (int f = 1; f < NF; ++f)
-      if (a.prim_bc_mask & (1 << f)) q[f] = q[0] * dnn(q[f], q[0]);
+      if (a.prim_bc_mask & (1 << f)) q[f] = q[0] * dnn_exact(q[f], q[0]);
   }
+}
+
+template <int NF>
+__device__ __forceinline__ void load_cons(const SweepArgs& a, long long idx,
+                                          bool interior, float* q) {
+  load_raw<NF>(a, idx, q);
+  prim_bc_round<NF>(a, interior, q);
 }
 
 template <int NS>
@@ -87,10 +97,10 @@ __device__ __forceinline__ void load_vel2(const SweepArgs& a, long long idx,
   float m0 = a.q.p[1 + c0][idx];
   float m1 = a.q.p[1 + c1][idx];
   if (a.prim_bc_mask && interior) {
-    if (a.prim_bc_mask & (1 << (1 + c0))) m0 = rho * dnn(m0, rho);
-    if (a.prim_bc_mask & (1 << (1 + c1))) m1 = rho * dnn(m1, rho);
+    if (a.prim_bc_mask & (1 << (1 + c0))) m0 = rho * dnn_exact(m0, rho);
+    if (a.prim_bc_mask & (1 << (1 + c1))) m1 = rho * dnn_exact(m1, rho);
   }
-  const float inv = 1.0f / rho;
+  const float inv = frcp(rho);
   v0 = inv * m0;
   v1 = inv * m1;
 }
@@ -131,9 +141,9 @@ __device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int 
     float ap0, ap1, am0, am1;
     load_vel2(a, sp, inp, dd, D, ap0, ap1);
     load_vel2(a, sm, inm, dd, D, am0, am1);
-    const float th = a.vc.two_h[dd];
-    out[t][0] = (ap0 - am0) / th;
-    out[t][1] = (ap1 - am1) / th;
+    const float ith = a.vc.inv_two_h[dd];
+    out[t][0] = (ap0 - am0) * ith;
+    out[t][1] = (ap1 - am1) * ith;
     ++t;
   }
 }
@@ -211,11 +221,19 @@ sweep_march_kernel(const SweepArgs a) {
   for (int f = 0; f < NF; ++f) Fprev[f] = 0.0f;
   tPrev[0][0] = tPrev[0][1] = tPrev[1][0] = tPrev[1][1] = 0.0f;
 
+  // Raw conserved values of the cell entering the window, loaded one
+  // iteration ahead so the loads overlap the arithmetic of the current cell.
+  float qn[NF];
+  load_raw<NF>(a, base + (long long)(c_begin - 1 + RAD) * sm, qn);
   for (int c = c_begin - 1; c <= c_end; ++c) {
     {
       const int cn = c + RAD;
-      load_prim<NS>(a, base + (long long)cn * sm, in_core(cn, a.h, n_march),
-                    pe_at(a, kk(cn), ii(cn), j), W[WN - 1]);
+      float qc[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) qc[f] = qn[f];
+      if (c < c_end) load_raw<NF>(a, base + (long long)(cn + 1) * sm, qn);
+      prim_bc_round<NF>(a, in_core(cn, a.h, n_march), qc);
+      cons_to_prim<NS>(qc, pe_at(a, kk(cn), ii(cn), j), W[WN - 1]);
     }
     float negP[NV], posM[NV];
 #pragma unroll
@@ -239,12 +257,12 @@ sweep_march_kernel(const SweepArgs a) {
           const float rho = W[RAD - 1][kRho];
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) / a.hdir;
+            a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
           }
         } else {
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            store_last(a, f, idx, a.r.p[f][idx] - (F[f] - Fprev[f]) / a.hdir);
+            store_last(a, f, idx, a.r.p[f][idx] - (F[f] - Fprev[f]) * a.inv_hdir);
           }
         }
       }
@@ -331,7 +349,7 @@ sweep_y_kernel(const SweepArgs a) {
   if (lane >= K && lane < K + OUTW && y < a.h + a.ny) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      a.r.p[f][idx] = a.r.p[f][idx] - (Fn[f] - F[f]) / a.hdir;
+      a.r.p[f][idx] = a.r.p[f][idx] - (Fn[f] - F[f]) * a.inv_hdir;
     }
   }
 }

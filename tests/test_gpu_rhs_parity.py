@@ -54,7 +54,7 @@ def _bcs(name, n, h):
         'rho_w': {'x': {0: (BC.NEUMANN, 0.0), 1: (BC.NEUMANN, 0.0)},
                   'y': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
                   'z': {0: (BC.DIRICHLET, 0.0), 1: (BC.DIRICHLET, 0.0)}},
-        'e': {'x': {0: (BC.NEUMANN, 0.0), 1: (BC.DIRICHLET, 3.0)},
+        'e': {'x': {0: (BC.NEUMANN, 0.0), 1: (BC.DIRICHLET, 6.0)},
               'y': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
               'z': {0: (BC.NEUMANN, 0.05), 1: (BC.NEUMANN, 0.0)}},
     }}
