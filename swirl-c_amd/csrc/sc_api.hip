@@ -135,6 +135,11 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   c->tune.seg_len = env_int("SWIRLC_SEG_LEN", c->tune.seg_len);
   c->tune.by_march = env_int("SWIRLC_BY_MARCH", c->tune.by_march);
   c->tune.by_y = env_int("SWIRLC_BY_Y", c->tune.by_y);
+  // Blocks are 64 x by threads; the kernels are built for at most 256.
+  auto clamp = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  c->tune.by_march = clamp(c->tune.by_march, 1, 4);
+  c->tune.by_y = clamp(c->tune.by_y, 1, 4);
+  c->tune.seg_len = clamp(c->tune.seg_len, 4, 1 << 20);
   c->grid = *grid;
   c->params = p;
   c->nf = p.n_fields;

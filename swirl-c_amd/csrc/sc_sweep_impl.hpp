@@ -314,6 +314,11 @@ sweep_y_kernel(const SweepArgs a) {
   const int y = y0 - K + lane;
   const int yc = min(y, a.NY - 1);
   const long long idx = (long long)k * a.sz + (long long)i * a.sx + yc;
+  const bool out_lane = lane >= K && lane < K + OUTW && y < a.h + a.ny;
+  // The RHS accumulator is read first so its latency overlaps the arithmetic.
+  float rin[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) rin[f] = out_lane ? a.r.p[f][idx] : 0.0f;
   float W0[NV];
   load_prim<NS>(a, idx, in_core(yc, a.h, a.ny), pe_at(a, k, i, yc), W0);
   float negP[NV], posM[NV], Am[NV];
@@ -346,10 +351,10 @@ sweep_y_kernel(const SweepArgs a) {
   float Fn[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) Fn[f] = __shfl(F[f], lane + 1, 64);
-  if (lane >= K && lane < K + OUTW && y < a.h + a.ny) {
+  if (out_lane) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      a.r.p[f][idx] = a.r.p[f][idx] - (Fn[f] - F[f]) * a.inv_hdir;
+      a.r.p[f][idx] = rin[f] - (Fn[f] - F[f]) * a.inv_hdir;
     }
   }
 }
