@@ -43,6 +43,10 @@ def parse():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-n', type=int, default=64)
   ap.add_argument('--cpu-steps', type=int, default=2)
+  ap.add_argument('--batch-instances', type=int, default=256)
+  ap.add_argument('--batch-steps', type=int, default=10)
+  ap.add_argument('--batch-loop-steps', type=int, default=3)
+  ap.add_argument('--no-batch', action='store_true')
   return ap.parse_args()
 
 
@@ -67,6 +71,66 @@ def time_sweeps(solver, reps):
     ev1.synchronize()
     out[name] = ev0.elapsed_time(ev1) / 1e3 / reps
   return out
+
+
+def bench_batch(n_inst, steps, loop_steps):
+  """Batched ensemble (256 3-D Riemann problems, WENO5 + HLLC, inviscid, RK3,
+  per-instance CFL dt): one launch per phase over all instances, graph
+  replayed, against a per-instance loop of the single-grid kernels."""
+  import torch
+  from swirlc_amd import configs
+  from swirlc_amd.core.batch import BatchSolver
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.core.solver import Solver
+  cfg, members = configs.riemann_ensemble(n_inst, seed=0)
+  cells = sum(g[0] * g[1] * g[2] for g, _ in members)
+  batch = BatchSolver(SwirlCParameters(dict(cfg)), members, cfl=0.5, use_graph=True)
+  batch.step(2)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  batch.step(steps)
+  torch.cuda.synchronize()
+  t_batch = (time.perf_counter() - t0) / steps
+  finite = all(bool(torch.isfinite(t).all()) for t in batch.u)
+  del batch
+  torch.cuda.empty_cache()
+  solvers = [Solver(SwirlCParameters(configs.member_cfg(cfg, g)),
+                    {k: torch.from_numpy(v).cuda() for k, v in st.items()}, cfl=0.5,
+                    use_graph=False) for g, st in members]
+  del members
+  for s in solvers:
+    s.step(1)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(loop_steps):
+    for s in solvers:
+      s.step(1)
+  torch.cuda.synchronize()
+  t_loop = (time.perf_counter() - t0) / loop_steps
+  # The same loop with every launch recorded into one HIP graph (GPU-side
+  # cost of per-instance launches, without Python overhead).
+  side = torch.cuda.Stream()
+  side.wait_stream(torch.cuda.current_stream())
+  g = torch.cuda.CUDAGraph()
+  with torch.cuda.stream(side):
+    with torch.cuda.graph(g, stream=side):
+      for s in solvers:
+        s._step_body()
+  torch.cuda.current_stream().wait_stream(side)
+  g.replay()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(loop_steps):
+    g.replay()
+  torch.cuda.synchronize()
+  t_loop_graph = (time.perf_counter() - t0) / loop_steps
+  return {'instances': n_inst, 'cells': cells, 'ms_per_step': round(t_batch * 1e3, 3),
+          'Mcell_updates_per_s': round(cells / t_batch / 1e6, 1),
+          'loop_ms_per_step': round(t_loop * 1e3, 3),
+          'loop_graph_ms_per_step': round(t_loop_graph * 1e3, 3),
+          'speedup_vs_loop': round(t_loop / t_batch, 2),
+          'speedup_vs_graphed_loop': round(t_loop_graph / t_batch, 2),
+          'finite': finite}
 
 
 def cpu_baseline(n, steps):
@@ -133,6 +197,15 @@ def main():
               'kernel_ms': {k: round(v * 1e3, 4) for k, v in ktimes.items()}}
   step_frac = cells * STEP_BYTES_VISCOUS / (el / args.steps) / HBM_PEAK
 
+  batched = None
+  if not args.no_batch:
+    print(f'[bench] batched ensemble: {args.batch_instances} instances', file=sys.stderr,
+          flush=True)
+    del solver
+    torch.cuda.empty_cache()
+    batched = bench_batch(args.batch_instances, args.batch_steps, args.batch_loop_steps)
+    print(f'[bench] batched: {batched}', file=sys.stderr, flush=True)
+
   cpu = None
   if not args.no_cpu:
     print(f'[bench] CPU oracle sample {args.cpu_n}^3 x {args.cpu_steps} steps',
@@ -149,6 +222,7 @@ def main():
       'roofline': roofline,
       'step_hbm_frac_700B': round(step_frac, 4),
       'cpu_baseline': cpu,
+      'batched': batched,
       'finite': finite,
   }
   print(json.dumps(line), flush=True)

@@ -158,6 +158,37 @@ sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0,
                         float* const* uk, float* const* out, void* stream);
 
+/* ---- Batched ensembles (no reference counterpart) ------------------------
+ * Many independent grids of different sizes sharing one configuration,
+ * advanced together: each phase (halo fill per dim, x / y / z sweep, wave
+ * speed) is one launch over a packed list of (instance, block) tiles, with
+ * per-instance time steps read from a device array, so a whole RK step of
+ * the ensemble is a fixed launch sequence that can be captured in a HIP graph.
+ * Instance i's field f occupies elements [offset, offset + (nx+2h)(ny+2h)(nz+2h))
+ * of arena f, laid out like a single-grid field.  Restrictions: 5 fields, no
+ * gravitational potential, no primitive-variable BCs, no per-layer planes. */
+typedef struct sc_instance {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  long long offset;
+} sc_instance;
+
+typedef struct sc_batch sc_batch;
+
+sc_status sc_batch_create(const sc_params* params, int halo, int n_instances,
+                          const sc_instance* instances, sc_batch** out);
+sc_status sc_batch_destroy(sc_batch* batch);
+/* sc_fill_halos for every instance (arena base pointers). */
+sc_status sc_batch_fill_halos(sc_batch* batch, float* const* q, void* stream);
+/* sc_rk_stage for every instance; dt_dev holds one float per instance. */
+sc_status sc_batch_rk_stage(sc_batch* batch, int form, float c0, float c1,
+                            const float* dt_dev, const float* const* u0,
+                            float* const* uk, float* const* out, void* stream);
+/* Per-instance maximum wave speed (speed_dev[i]) and, if dt_dev is not NULL,
+ * dt_dev[i] = cfl * min(dx_i, dy_i, dz_i) / speed_dev[i]. */
+sc_status sc_batch_max_wave_speed(sc_batch* batch, const float* const* q, float cfl,
+                                  float* speed_dev, float* dt_dev, void* stream);
+
 /* Thread-local message of the last failed call ("" if none). */
 const char* sc_last_error(void);
 

@@ -408,3 +408,318 @@ sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched ensembles.
+// ---------------------------------------------------------------------------
+
+#include <vector>
+
+#include "sc_batch_impl.hpp"
+
+namespace sc {
+template <int SCHEME>
+hipError_t launch_batch_sweeps(const BatchArgs& b, const BatchTiles& tiles, int flux,
+                               bool visc, const SweepTuning& tune, hipStream_t stream);
+}
+
+struct sc_batch {
+  sc_params params;
+  int h = 0;
+  int n_inst = 0;
+  int nf = 5;
+  long long total = 0;  // arena length (elements)
+  sc::BatchInst* inst_dev = nullptr;
+  sc::BatchHaloInst* hinst_dev = nullptr;
+  sc::BatchTile* tiles_dev[3] = {nullptr, nullptr, nullptr};
+  sc::BatchTiles tiles;
+  sc::BatchHaloTile* htiles_dev[3] = {nullptr, nullptr, nullptr};
+  int nhtiles[3] = {0, 0, 0};
+  sc::BatchHaloTile* stiles_dev = nullptr;
+  int nstiles = 0;
+  float* hmin_dev = nullptr;
+  unsigned* speed_bits = nullptr;
+  float* r_arena[SC_MAX_FIELDS] = {};
+  sc::HaloArgs halo_cfg;
+  sc::SweepArgs proto;
+  sc::SweepTuning tune;
+};
+
+namespace {
+
+template <typename T>
+sc_status upload(const std::vector<T>& v, T** dst) {
+  if (v.empty()) {
+    *dst = nullptr;
+    return SC_OK;
+  }
+  sc_status st = hip_check(hipMalloc((void**)dst, sizeof(T) * v.size()), "hipMalloc batch table");
+  if (st != SC_OK) return st;
+  return hip_check(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice),
+                   "hipMemcpy batch table");
+}
+
+int grid_for(int ntiles) { return ntiles < 2048 ? (ntiles > 0 ? ntiles : 1) : 2048; }
+
+}  // namespace
+
+extern "C" {
+
+sc_status sc_batch_destroy(sc_batch* b) {
+  if (b == nullptr) return SC_OK;
+  hipFree(b->inst_dev);
+  hipFree(b->hinst_dev);
+  for (int d = 0; d < 3; ++d) {
+    hipFree(b->tiles_dev[d]);
+    hipFree(b->htiles_dev[d]);
+  }
+  hipFree(b->stiles_dev);
+  hipFree(b->hmin_dev);
+  hipFree(b->speed_bits);
+  for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->r_arena[f]);
+  delete b;
+  return SC_OK;
+}
+
+sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_instance* inst,
+                          sc_batch** out) {
+  if (params == nullptr || inst == nullptr || out == nullptr || n < 1)
+    return fail(SC_ERR_INVALID, "sc_batch_create: null argument or no instances");
+  *out = nullptr;
+  const sc_params& p = *params;
+  if (p.n_fields != 5)
+    return fail(SC_ERR_UNSUPPORTED, "batched ensembles support the 5 base fields only");
+  if (gravity_direction(p.g) != -1)
+    return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support a gravity vector");
+  if (p.prim_bc_mask != 0)
+    return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support primitive-variable BCs");
+  // Validate the shared configuration with a throwaway single-grid check of
+  // every instance shape (schemes, halo width, BC types).
+  long long end = 0;
+  for (int i = 0; i < n; ++i) {
+    const sc_instance& g = inst[i];
+    sc_grid grid{};
+    grid.nx = g.nx;
+    grid.ny = g.ny;
+    grid.nz = g.nz;
+    grid.halo = halo;
+    grid.dx = g.dx;
+    grid.dy = g.dy;
+    grid.dz = g.dz;
+    if (p.interpolation < SC_WENO_3 || p.interpolation > SC_MUSCL)
+      return fail(SC_ERR_UNSUPPORTED, "interpolation_scheme is not implemented");
+    if (p.numeric_flux != SC_HLL && p.numeric_flux != SC_HLLC)
+      return fail(SC_ERR_INVALID, "numeric_flux_scheme is not implemented");
+    if (g.nx < 1 || g.ny < 1 || g.nz < 1 || !(g.dx > 0 && g.dy > 0 && g.dz > 0))
+      return fail(SC_ERR_INVALID, "instance extents and spacings must be positive");
+    if (halo < radius_of(p.interpolation) + 1 || halo > SC_MAX_HALO)
+      return fail(SC_ERR_INVALID, "halo_width too small for the interpolation stencil");
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    for (int f = 0; f < 5; ++f)
+      for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s) {
+          const sc_bc_face& bf = p.bc[f][d][s];
+          if (bf.type < SC_BC_PERIODIC || bf.type > SC_BC_NEUMANN)
+            return fail(SC_ERR_UNSUPPORTED, "unsupported BC type");
+          if (bf.type == SC_BC_PERIODIC && nn[d] < halo)
+            return fail(SC_ERR_UNSUPPORTED, "PERIODIC BC needs at least halo_width interior cells");
+          if (bf.layer_planes != nullptr)
+            return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support per-layer planes");
+          if (bf.n_layer_values != 0 && bf.n_layer_values != halo)
+            return fail(SC_ERR_INVALID, "per-layer DIRICHLET values must have halo_width entries");
+        }
+    const long long cells = (long long)(g.nx + 2 * halo) * (g.ny + 2 * halo) * (g.nz + 2 * halo);
+    if (g.offset < end)
+      return fail(SC_ERR_INVALID, "instance offsets must be increasing and non-overlapping");
+    end = g.offset + cells;
+  }
+
+  sc_batch* b = new sc_batch();
+  b->params = p;
+  b->h = halo;
+  b->n_inst = n;
+  b->total = end;
+  b->tune.seg_len = env_int("SWIRLC_SEG_LEN", b->tune.seg_len);
+  b->tune.by_march = env_int("SWIRLC_BY_MARCH", b->tune.by_march);
+  b->tune.by_y = env_int("SWIRLC_BY_Y", b->tune.by_y);
+  auto clamp = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  b->tune.by_march = clamp(b->tune.by_march, 1, 4);
+  b->tune.by_y = clamp(b->tune.by_y, 1, 4);
+  b->tune.seg_len = clamp(b->tune.seg_len, 4, 1 << 20);
+
+  const int rad = radius_of(p.interpolation);
+  const int outw = 64 - 2 * rad - 2;
+  std::vector<sc::BatchInst> bi(n);
+  std::vector<sc::BatchHaloInst> hi(n);
+  std::vector<float> hmin(n);
+  std::vector<sc::BatchTile> tl[3];
+  std::vector<sc::BatchHaloTile> ht[3], stl;
+  for (int i = 0; i < n; ++i) {
+    const sc_instance& g = inst[i];
+    const double hh[3] = {g.dx, g.dy, g.dz};
+    sc::BatchInst& e = bi[i];
+    e.nx = g.nx;
+    e.ny = g.ny;
+    e.nz = g.nz;
+    e.off = g.offset;
+    for (int d = 0; d < 3; ++d) {
+      e.h[d] = (float)hh[d];
+      e.inv_h[d] = 1.0f / (float)hh[d];
+      e.inv_two_h[d] = 1.0f / (float)(2.0 * hh[d]);
+    }
+    hi[i].n[0] = g.nx;
+    hi[i].n[1] = g.ny;
+    hi[i].n[2] = g.nz;
+    hi[i].off = g.offset;
+    float hm = e.h[0];
+    if (e.h[1] < hm) hm = e.h[1];
+    if (e.h[2] < hm) hm = e.h[2];
+    hmin[i] = hm;
+    const int seg = b->tune.seg_len, bym = b->tune.by_march, byy = b->tune.by_y;
+    // x sweep: (y chunks of 64, z groups, x segments); y sweep: (y chunks,
+    // x groups, z); z sweep: (y chunks, x groups, z segments).
+    for (int bz = 0; bz < (g.nx + seg - 1) / seg; ++bz)
+      for (int by = 0; by < (g.nz + bym - 1) / bym; ++by)
+        for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[0].push_back({i, bx, by, bz});
+    for (int bz = 0; bz < g.nz; ++bz)
+      for (int by = 0; by < (g.nx + byy - 1) / byy; ++by)
+        for (int bx = 0; bx < (g.ny + outw - 1) / outw; ++bx) tl[1].push_back({i, bx, by, bz});
+    for (int bz = 0; bz < (g.nz + seg - 1) / seg; ++bz)
+      for (int by = 0; by < (g.nx + bym - 1) / bym; ++by)
+        for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[2].push_back({i, bx, by, bz});
+    const long long N[3] = {g.nx + 2LL * halo, g.ny + 2LL * halo, g.nz + 2LL * halo};
+    const long long plane[3] = {N[2] * N[1], N[2] * N[0], N[0] * N[1]};
+    for (int d = 0; d < 3; ++d) {
+      const long long threads = 5 * plane[d];
+      for (long long k = 0; k < (threads + 255) / 256; ++k) ht[d].push_back({i, (int)k});
+    }
+    const long long cells = (long long)g.nx * g.ny * g.nz;
+    for (long long k = 0; k < (cells + sc::kSpeedChunk - 1) / sc::kSpeedChunk; ++k)
+      stl.push_back({i, (int)k});
+  }
+  sc_status st = upload(bi, &b->inst_dev);
+  if (st == SC_OK) st = upload(hi, &b->hinst_dev);
+  if (st == SC_OK) st = upload(hmin, &b->hmin_dev);
+  for (int d = 0; d < 3 && st == SC_OK; ++d) {
+    st = upload(tl[d], &b->tiles_dev[d]);
+    b->tiles.t[d] = b->tiles_dev[d];
+    b->tiles.n[d] = (int)tl[d].size();
+    b->tiles.grid[d] = grid_for(b->tiles.n[d]);
+    if (st == SC_OK) st = upload(ht[d], &b->htiles_dev[d]);
+    b->nhtiles[d] = (int)ht[d].size();
+  }
+  if (st == SC_OK) st = upload(stl, &b->stiles_dev);
+  b->nstiles = (int)stl.size();
+  if (st == SC_OK)
+    st = hip_check(hipMalloc((void**)&b->speed_bits, sizeof(unsigned) * n), "hipMalloc speed");
+  for (int f = 0; f < 5 && st == SC_OK; ++f) {
+    st = hip_check(hipMalloc((void**)&b->r_arena[f], sizeof(float) * b->total), "hipMalloc R");
+    if (st == SC_OK)
+      st = hip_check(hipMemset(b->r_arena[f], 0, sizeof(float) * b->total), "hipMemset R");
+  }
+  if (st != SC_OK) {
+    sc_batch_destroy(b);
+    return st;
+  }
+  // Shared halo configuration (BC tables).
+  sc::HaloArgs& ha = b->halo_cfg;
+  memset(&ha, 0, sizeof(ha));
+  ha.nf = 5;
+  ha.h = halo;
+  for (int f = 0; f < 5; ++f)
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < 2; ++s) {
+        const sc_bc_face& bf = p.bc[f][d][s];
+        sc::BcFaceDev& o = ha.bc[f][d][s];
+        o.type = bf.type;
+        o.value = bf.value;
+        o.n_layer_values = bf.n_layer_values;
+        for (int k = 0; k < SC_MAX_HALO; ++k) o.layer_values[k] = bf.layer_values[k];
+        o.layer_planes = nullptr;
+      }
+  // Shared sweep configuration.
+  sc::SweepArgs& a = b->proto;
+  memset(&a, 0, sizeof(a));
+  a.h = halo;
+  a.has_pe = 0;
+  a.grav_src = 0;
+  a.prim_bc_mask = 0;
+  a.vc.nu = (float)p.nu;
+  a.vc.kfac = (float)(1.0 / p.pr) * sc::kCp;
+  *out = b;
+  return SC_OK;
+}
+
+sc_status sc_batch_fill_halos(sc_batch* b, float* const* q, void* stream) {
+  if (b == nullptr || q == nullptr) return fail(SC_ERR_INVALID, "sc_batch_fill_halos: null argument");
+  for (int f = 0; f < 5; ++f)
+    if (q[f] == nullptr) return fail(SC_ERR_INVALID, "sc_batch_fill_halos: null arena");
+  for (int d = 0; d < 3; ++d) {
+    sc_status st = hip_check(
+        sc::launch_batch_halo_dim(b->halo_cfg, q, b->hinst_dev, b->htiles_dev[d], b->nhtiles[d],
+                                  grid_for(b->nhtiles[d]), d, (hipStream_t)stream),
+        "batch halo kernel");
+    if (st != SC_OK) return st;
+  }
+  return SC_OK;
+}
+
+sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const float* dt_dev,
+                            const float* const* u0, float* const* uk, float* const* out,
+                            void* stream) {
+  if (b == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_batch_rk_stage: null argument");
+  if (form != SC_RK_EULER && form != SC_RK_BLEND)
+    return fail(SC_ERR_UNSUPPORTED, "sc_batch_rk_stage: form must be SC_RK_EULER or SC_RK_BLEND");
+  if (form == SC_RK_BLEND && u0 == nullptr)
+    return fail(SC_ERR_INVALID, "sc_batch_rk_stage: u0 required");
+  for (int f = 0; f < 5; ++f) {
+    if (uk[f] == nullptr || out[f] == nullptr || (form == SC_RK_BLEND && u0[f] == nullptr))
+      return fail(SC_ERR_INVALID, "sc_batch_rk_stage: null arena");
+    if (out[f] == uk[f]) return fail(SC_ERR_INVALID, "sc_batch_rk_stage: out must not alias uk");
+  }
+  sc_status st = sc_batch_fill_halos(b, uk, stream);
+  if (st != SC_OK) return st;
+  sc::BatchArgs ba;
+  memset(&ba, 0, sizeof(ba));
+  ba.proto = b->proto;
+  ba.proto.rk_form = form;
+  ba.proto.c0 = c0;
+  ba.proto.c1 = c1;
+  ba.nf = 5;
+  for (int f = 0; f < 5; ++f) {
+    ba.q.p[f] = uk[f];
+    ba.u0.p[f] = form == SC_RK_BLEND ? u0[f] : nullptr;
+    ba.r.p[f] = b->r_arena[f];
+    ba.out.p[f] = out[f];
+  }
+  ba.inst = b->inst_dev;
+  ba.dt = dt_dev;
+  hipError_t e;
+  const int flux = b->params.numeric_flux;
+  const bool visc = b->params.include_diffusion != 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->params.interpolation == SC_WENO_5) {
+    e = sc::launch_batch_sweeps<sc::kWeno5>(ba, b->tiles, flux, visc, b->tune, s);
+  } else if (b->params.interpolation == SC_WENO_3) {
+    e = sc::launch_batch_sweeps<sc::kWeno3>(ba, b->tiles, flux, visc, b->tune, s);
+  } else {
+    e = sc::launch_batch_sweeps<sc::kMuscl>(ba, b->tiles, flux, visc, b->tune, s);
+  }
+  return hip_check(e, "batch sweep kernels");
+}
+
+sc_status sc_batch_max_wave_speed(sc_batch* b, const float* const* q, float cfl,
+                                  float* speed_dev, float* dt_dev, void* stream) {
+  if (b == nullptr || q == nullptr || speed_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_batch_max_wave_speed: null argument");
+  float* qb[SC_MAX_FIELDS] = {};
+  for (int f = 0; f < 5; ++f) qb[f] = const_cast<float*>(q[f]);
+  return hip_check(sc::launch_batch_max_speed(qb, b->hinst_dev, b->h, b->stiles_dev, b->nstiles,
+                                              grid_for(b->nstiles), b->n_inst,
+                                              (unsigned*)speed_dev, cfl, b->hmin_dev, dt_dev,
+                                              (hipStream_t)stream),
+                   "batch max speed kernel");
+}
+
+}  // extern "C"

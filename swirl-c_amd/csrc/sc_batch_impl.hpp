@@ -1,0 +1,148 @@
+// Batched sweeps: many independent grids of different sizes advanced by one
+// launch per sweep direction over a packed tile list.
+//
+// Instance i's field f occupies [off_i, off_i + N_i) of arena f (N_i its
+// halo-padded cell count); all instances share the physics configuration.
+// A tile is one block of the plain launch geometry of one instance; the
+// kernels walk the tile list with a grid-stride loop, so one launch of
+// ~(CUs x resident blocks) blocks covers every instance (no per-instance
+// launches, no idle tail for small grids).  Per-instance dt comes from a
+// device array, so a whole RK step of the ensemble can be captured in a HIP
+// graph and replayed.
+#pragma once
+
+#include "sc_sweep_impl.hpp"
+
+namespace sc {
+
+struct BatchInst {
+  int nx, ny, nz;
+  long long off;  // element offset of the instance in every arena
+  float inv_h[3];
+  float h[3];
+  float inv_two_h[3];
+};
+
+struct BatchTile {
+  int inst, bx, by, bz;
+};
+
+struct BatchArgs {
+  SweepArgs proto;  // shared configuration (flags, nu, kfac, RK form, ...)
+  CFields q;        // arena bases
+  CFields u0;
+  Fields r;
+  Fields out;
+  int nf;
+  const BatchInst* inst;
+  const BatchTile* tiles;
+  int ntiles;
+  const float* dt;  // per-instance time steps
+};
+
+__device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int dir) {
+  const BatchInst in = b.inst[id];
+  SweepArgs a = b.proto;
+#pragma unroll
+  for (int f = 0; f < kMaxFields; ++f) {
+    const bool on = f < b.nf;
+    a.q.p[f] = on && b.q.p[f] ? b.q.p[f] + in.off : nullptr;
+    a.u0.p[f] = on && b.u0.p[f] ? b.u0.p[f] + in.off : nullptr;
+    a.r.p[f] = on && b.r.p[f] ? b.r.p[f] + in.off : nullptr;
+    a.out.p[f] = on && b.out.p[f] ? b.out.p[f] + in.off : nullptr;
+  }
+  a.nx = in.nx;
+  a.ny = in.ny;
+  a.nz = in.nz;
+  a.NX = in.nx + 2 * a.h;
+  a.NY = in.ny + 2 * a.h;
+  a.NZ = in.nz + 2 * a.h;
+  a.sx = a.NY;
+  a.sz = (long long)a.NX * a.NY;
+  a.inv_hdir = in.inv_h[dir];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    a.vc.inv_h[d] = in.inv_h[d];
+    a.vc.h[d] = in.h[d];
+    a.vc.inv_two_h[d] = in.inv_two_h[d];
+  }
+  a.dt = b.dt != nullptr ? b.dt + id : nullptr;
+  return a;
+}
+
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
+__global__ void __launch_bounds__(256) batch_march_kernel(const BatchArgs b) {
+  for (int t = (int)blockIdx.x; t < b.ntiles; t += (int)gridDim.x) {
+    const BatchTile tl = b.tiles[t];
+    const SweepArgs a = tile_args(b, tl.inst, DIR);
+    const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
+                     (int)blockDim.y};
+    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE>(a, ti);
+  }
+}
+
+template <int SCHEME, int FLUX, bool VISC, int NS>
+__global__ void __launch_bounds__(256) batch_y_kernel(const BatchArgs b) {
+  for (int t = (int)blockIdx.x; t < b.ntiles; t += (int)gridDim.x) {
+    const BatchTile tl = b.tiles[t];
+    const SweepArgs a = tile_args(b, tl.inst, 1);
+    const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
+                     (int)blockDim.y};
+    sweep_y_body<SCHEME, FLUX, VISC, NS>(a, ti);
+  }
+}
+
+// Tile lists of one batch (device arrays), one per sweep direction.
+struct BatchTiles {
+  const BatchTile* t[3];
+  int n[3];
+  int grid[3];  // blocks launched per direction (grid-stride over the tiles)
+};
+
+template <int SCHEME, int FLUX, bool VISC, int NS>
+hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
+                                 const SweepTuning& tune, hipStream_t stream) {
+  b.proto.seg_len = tune.seg_len;
+  const dim3 blk_m(64, tune.by_march);
+  const dim3 blk_y(64, tune.by_y);
+  {
+    BatchArgs bx = b;
+    bx.tiles = tiles.t[0];
+    bx.ntiles = tiles.n[0];
+    hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>),
+                       dim3(tiles.grid[0]), blk_m, 0, stream, bx);
+  }
+  {
+    BatchArgs by = b;
+    by.tiles = tiles.t[1];
+    by.ntiles = tiles.n[1];
+    hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS>), dim3(tiles.grid[1]),
+                       blk_y, 0, stream, by);
+  }
+  {
+    BatchArgs bz = b;
+    bz.tiles = tiles.t[2];
+    bz.ntiles = tiles.n[2];
+    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>),
+                       dim3(tiles.grid[2]), blk_m, 0, stream, bz);
+  }
+  return hipGetLastError();
+}
+
+template <int SCHEME>
+hipError_t launch_batch_sweeps(const BatchArgs& b, const BatchTiles& tiles, int flux,
+                               bool visc, const SweepTuning& tune, hipStream_t stream);
+
+#define SC_INSTANTIATE_BATCH(SCHEME)                                                     \
+  template <>                                                                            \
+  hipError_t launch_batch_sweeps<SCHEME>(const BatchArgs& b, const BatchTiles& tiles,    \
+                                         int flux, bool visc, const SweepTuning& tune,   \
+                                         hipStream_t stream) {                           \
+    if (flux == kHllc)                                                                   \
+      return visc ? launch_batch_sweeps_t<SCHEME, kHllc, true, 0>(b, tiles, tune, stream) \
+                  : launch_batch_sweeps_t<SCHEME, kHllc, false, 0>(b, tiles, tune, stream); \
+    return visc ? launch_batch_sweeps_t<SCHEME, kHll, true, 0>(b, tiles, tune, stream)     \
+                : launch_batch_sweeps_t<SCHEME, kHll, false, 0>(b, tiles, tune, stream);   \
+  }
+
+}  // namespace sc

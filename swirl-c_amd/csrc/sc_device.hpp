@@ -63,6 +63,18 @@ __device__ __forceinline__ float dnn(float x, float y) {
 __device__ __forceinline__ float rnn(float x) {
   return x == 0.0f ? 0.0f : frcp(x);
 }
+// tf.minimum / tf.maximum: a NaN operand gives NaN (fminf/fmaxf would return
+// the other operand), so unphysical states propagate exactly as in the
+// reference instead of silently selecting a different wave speed.
+__device__ __forceinline__ float tf_min(float a, float b) {
+  const float m = fminf(a, b);
+  return a != a ? a : (b != b ? b : m);
+}
+__device__ __forceinline__ float tf_max(float a, float b) {
+  const float m = fmaxf(a, b);
+  return a != a ? a : (b != b ? b : m);
+}
+
 // divide_no_nan with an IEEE division, for the boundary-condition paths whose
 // values must match the reference bit for bit (cheap: halo slabs only).
 __device__ __forceinline__ float dnn_exact(float x, float y) {
@@ -213,8 +225,8 @@ __device__ __forceinline__ void roe_speeds(const float* L, const float* R,
   const float hs = (rh - 0.5f * ((ru * ru + rv * rv) + rw * rw)) - pe;
   const float c = fsqrt(kGm1 * hs);
   const float un = D == 0 ? ru : (D == 1 ? rv : rw);
-  sl = fminf(L[kU + D] - cl, un - c);
-  sr = fmaxf(R[kU + D] + cr, un + c);
+  sl = tf_min(L[kU + D] - cl, un - c);
+  sr = tf_max(R[kU + D] + cr, un + c);
 }
 
 // HLL flux: hll.py:36-175.
@@ -226,7 +238,7 @@ __device__ __forceinline__ void hll_flux(const float* L, const float* R, float p
   roe_speeds<D>(L, R, pe, sl, sr);
   const float ds = sr - sl;
   const float rds = ds == 0.0f ? 0.0f : frcp(ds);
-  const float t1 = (fminf(sr, 0.0f) - fminf(sl, 0.0f)) * rds;
+  const float t1 = (tf_min(sr, 0.0f) - tf_min(sl, 0.0f)) * rds;
   const float t2 = 1.0f - t1;
   const float t3 = (sr * fabsf(sl) - sl * fabsf(sr)) * (0.5f * rds);
   float fl[NF], fr[NF], ul[NF], ur[NF];

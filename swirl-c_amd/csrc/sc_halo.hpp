@@ -34,6 +34,28 @@ struct HaloArgs {
 hipError_t launch_halo_dim(const HaloArgs& a, int d, hipStream_t stream);
 hipError_t launch_halo_prim_finish(const HaloArgs& a, hipStream_t stream);
 
+// Batched halo fill: `cfg` carries the shared BC tables (its field pointers
+// and extents are unused); instance geometry comes from `inst` (nx, ny, nz,
+// arena offset) and the tile list (instance, block) of each dim.
+struct BatchHaloInst {
+  int n[3];
+  long long off;
+};
+struct BatchHaloTile {
+  int inst, blk;
+};
+hipError_t launch_batch_halo_dim(const HaloArgs& cfg, float* const* q_base,
+                                 const BatchHaloInst* inst, const BatchHaloTile* tiles,
+                                 int ntiles, int grid, int d, hipStream_t stream);
+
+// Batched wave speed: tiles (instance, chunk of kSpeedChunk interior cells);
+// bits[i] = max speed of instance i, then dt[i] = cfl * hmin[i] / max.
+constexpr int kSpeedChunk = 4096;
+hipError_t launch_batch_max_speed(float* const* q_base, const BatchHaloInst* inst, int h,
+                                  const BatchHaloTile* tiles, int ntiles, int grid,
+                                  int n_inst, unsigned* bits, float cfl,
+                                  const float* hmin, float* dt, hipStream_t stream);
+
 struct SpeedArgs {
   const float* q[5];
   int nx, ny, nz, h;

@@ -7,13 +7,6 @@
 
 namespace sc {
 
-// Launch geometry of the sweeps (the tunable knobs).
-struct SweepTuning {
-  int seg_len = 32;  // cells per thread along a marching sweep
-  int by_march = 4;  // waves per block, marching sweeps
-  int by_y = 4;      // waves per block, y sweep
-};
-
 // Launches the x, y and z sweeps of one RHS evaluation (plus the folded RK
 // update when a.rk_form != 0) for interpolation scheme SCHEME.
 // only_dir = -1 launches all three; 0, 1 or 2 launches that sweep alone.

@@ -27,6 +27,13 @@ struct CFields {
   const float* p[kMaxFields];
 };
 
+// Launch geometry of the sweeps (the tunable knobs).
+struct SweepTuning {
+  int seg_len = 32;  // cells per thread along a marching sweep
+  int by_march = 4;  // waves per block, marching sweeps
+  int by_y = 4;      // waves per block, y sweep
+};
+
 struct SweepArgs {
   CFields q;    // stage state uk, halos filled
   CFields u0;   // RK base state
@@ -184,19 +191,26 @@ __device__ __forceinline__ void store_last(const SweepArgs& a, int f, long long 
 // seg_len cells along the sweep.  PHASE 0: first sweep (writes R = src - div);
 // PHASE 2: last sweep (R - div -> R or RK update).
 // ---------------------------------------------------------------------------
+// Block geometry of one tile: block indices (bx, by, bz) of the plain launch
+// and the thread's (tx, ty) with by_dim waves per block.
+struct TileIdx {
+  int bx, by, bz;
+  int tx, ty, by_dim;
+};
+
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
-__global__ void __launch_bounds__(256)
-sweep_march_kernel(const SweepArgs a) {
+__device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
   constexpr int NV = NVar<NS>::value;
   constexpr int NF = 5 + NS;
-  const int j = a.h + (int)blockIdx.x * 64 + (int)threadIdx.x;
-  const int o = a.h + (int)blockIdx.y * (int)blockDim.y + (int)threadIdx.y;
+  const int j = a.h + ti.bx * 64 + ti.tx;
+  const int o = a.h + ti.by * ti.by_dim + ti.ty;
   const int n_march = DIR == 0 ? a.nx : a.nz;
   const int n_other = DIR == 0 ? a.nz : a.nx;
   if (j >= a.h + a.ny || o >= a.h + n_other) return;
-  const int c_begin = a.h + (int)blockIdx.z * a.seg_len;
+  const int c_begin = a.h + ti.bz * a.seg_len;
+  if (c_begin >= a.h + n_march) return;
   const int c_end = min(c_begin + a.seg_len, a.h + n_march);
   const long long sm = DIR == 0 ? a.sx : a.sz;
   const long long so = DIR == 0 ? a.sz : a.sx;
@@ -285,6 +299,14 @@ sweep_march_kernel(const SweepArgs a) {
   }
 }
 
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
+__global__ void __launch_bounds__(256)
+sweep_march_kernel(const SweepArgs a) {
+  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE>(a, ti);
+}
+
 // ---------------------------------------------------------------------------
 // y sweep (contiguous direction).  Block (64, BY): one wave per (z, x) row
 // chunk; lane l holds cell y0 - K + l and the stencil comes from shuffles.
@@ -298,19 +320,19 @@ struct YGeom {
 };
 
 template <int SCHEME, int FLUX, bool VISC, int NS>
-__global__ void __launch_bounds__(256)
-sweep_y_kernel(const SweepArgs a) {
+__device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = YGeom<SCHEME>::RAD;
   constexpr int K = YGeom<SCHEME>::K;
   constexpr int OUTW = YGeom<SCHEME>::OUTW;
   constexpr int WN = 2 * RAD + 1;
   constexpr int NV = NVar<NS>::value;
   constexpr int NF = 5 + NS;
-  const int lane = (int)threadIdx.x;
-  const int i = a.h + (int)blockIdx.y * (int)blockDim.y + (int)threadIdx.y;
-  const int k = a.h + (int)blockIdx.z;
-  if (i >= a.h + a.nx) return;  // uniform per wave
-  const int y0 = a.h + (int)blockIdx.x * OUTW;
+  const int lane = ti.tx;
+  const int i = a.h + ti.by * ti.by_dim + ti.ty;
+  const int k = a.h + ti.bz;
+  if (i >= a.h + a.nx || k >= a.h + a.nz) return;  // uniform per wave
+  const int y0 = a.h + ti.bx * OUTW;
+  if (y0 >= a.h + a.ny) return;
   const int y = y0 - K + lane;
   const int yc = min(y, a.NY - 1);
   const long long idx = (long long)k * a.sz + (long long)i * a.sx + yc;
@@ -357,6 +379,14 @@ sweep_y_kernel(const SweepArgs a) {
       a.r.p[f][idx] = rin[f] - (Fn[f] - F[f]) * a.inv_hdir;
     }
   }
+}
+
+template <int SCHEME, int FLUX, bool VISC, int NS>
+__global__ void __launch_bounds__(256)
+sweep_y_kernel(const SweepArgs a) {
+  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  sweep_y_body<SCHEME, FLUX, VISC, NS>(a, ti);
 }
 
 // ---------------------------------------------------------------------------

@@ -26,7 +26,8 @@ SC_RK_EULER, SC_RK_BLEND, SC_RK_AVG = 1, 2, 3
 EXPORTED = (
     'sc_ctx_create', 'sc_ctx_destroy', 'sc_fill_halos', 'sc_rhs',
     'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes', 'sc_sweep_only',
-    'sc_last_error', 'sc_version',
+    'sc_batch_create', 'sc_batch_destroy', 'sc_batch_fill_halos', 'sc_batch_rk_stage',
+    'sc_batch_max_wave_speed', 'sc_last_error', 'sc_version',
 )
 
 
@@ -65,6 +66,14 @@ class ScParams(ctypes.Structure):
   ]
 
 
+class ScInstance(ctypes.Structure):
+  _fields_ = [
+      ('nx', ctypes.c_int), ('ny', ctypes.c_int), ('nz', ctypes.c_int),
+      ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('dz', ctypes.c_double),
+      ('offset', ctypes.c_longlong),
+  ]
+
+
 class NativeError(RuntimeError):
   pass
 
@@ -92,11 +101,18 @@ def lib():
     L.sc_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_batch_create.argtypes = [ctypes.POINTER(ScParams), i, i, ctypes.POINTER(ScInstance),
+                                  ctypes.POINTER(vp)]
+    L.sc_batch_destroy.argtypes = [vp]
+    L.sc_batch_fill_halos.argtypes = [vp, fpp, vp]
+    L.sc_batch_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_batch_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
     L.sc_last_error.restype = ctypes.c_char_p
     L.sc_version.restype = ctypes.c_char_p
     for name in ('sc_ctx_create', 'sc_ctx_destroy', 'sc_fill_halos', 'sc_rhs',
                  'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes',
-                 'sc_sweep_only'):
+                 'sc_sweep_only', 'sc_batch_create', 'sc_batch_destroy',
+                 'sc_batch_fill_halos', 'sc_batch_rk_stage', 'sc_batch_max_wave_speed'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

@@ -174,3 +174,38 @@ def riemann(shape, rho, p, vel, axis, offset, h=3):
   return cfg, _state_from_prim(pick(rho[0], rho[1]), pick(vel[0][0], vel[1][0]),
                                pick(vel[0][1], vel[1][1]), pick(vel[0][2], vel[1][2]),
                                pick(p[0], p[1]), h)
+
+
+ENSEMBLE_SHAPES = ((32, 32, 32), (64, 64, 64), (96, 96, 96), (128, 128, 64), (160, 64, 64))
+
+
+def riemann_ensemble(n=256, seed=0, shapes=ENSEMBLE_SHAPES, h=3):
+  """Batched workload: n independent 3-D Riemann problems.
+
+  Sizes (nx, ny, nz) drawn with default_rng(seed) from `shapes`; for each of
+  the two states rho ~ logU[0.1, 2], p ~ logU[0.1, 10], velocity ~ U[-1, 1]^3;
+  the interface is normal to a random axis at a random offset in [0.3, 0.7].
+  WENO_5 + HLLC, inviscid, Neumann(0) walls.  Returns (cfg, [(grid, state)]).
+  """
+  rng = np.random.default_rng(seed)
+  members = []
+  cfg = None
+  for _ in range(n):
+    shape = shapes[int(rng.integers(len(shapes)))]
+    rho = np.exp(rng.uniform(np.log(0.1), np.log(2.0), 2))
+    p = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+    vel = rng.uniform(-1.0, 1.0, (2, 3))
+    axis = int(rng.integers(3))
+    offset = float(rng.uniform(0.3, 0.7))
+    c, st = riemann(shape, rho, p, vel, axis, offset, h)
+    cfg = c if cfg is None else cfg
+    members.append(((shape[0], shape[1], shape[2], c['dx'], c['dy'], c['dz']), st))
+  return cfg, members
+
+
+def member_cfg(cfg, grid):
+  """The single-grid configuration of one ensemble member (grid keys replaced)."""
+  nx, ny, nz, dx, dy, dz = grid
+  out = dict(cfg)
+  out.update(_base_cfg(nx, ny, nz, dx * nx, dy * ny, dz * nz, cfg['halo_width']))
+  return out

@@ -51,7 +51,10 @@ def validate_models(cfg):
 class NativeContext:
   """One native RHS/RK context for a fixed grid and configuration."""
 
-  def __init__(self, cfg, device=None):
+  def __init__(self, cfg, device=None, create=True):
+    """Validates cfg and builds the native parameters; `create=False` stops
+    before the native context is made (the parameters are reused by a batch)."""
+    self._ctx = None
     validate_models(cfg)
     scheme = _get(cfg, 'interpolation_scheme')
     if scheme not in _INTERP:
@@ -108,7 +111,7 @@ class NativeContext:
         self._keep.append(arr)
         setattr(grid, attr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
 
-    params = N.ScParams()
+    params = self._params = N.ScParams()
     params.n_fields = self.nf
     params.interpolation = _INTERP[scheme]
     params.numeric_flux = _FLUX[flux]
@@ -140,6 +143,8 @@ class NativeContext:
     params.prim_bc_mask = mask
     self.prim_bc_mask = mask
 
+    if not create:
+      return
     L = N.lib()
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):

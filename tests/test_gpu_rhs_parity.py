@@ -228,3 +228,22 @@ def test_rk_step_matches_oracle(scheme_name):
   errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in ref}
   _report(f'{scheme_name} step', errs)
   assert max(errs.values()) <= TOL, errs
+
+
+def test_unphysical_ghost_states_follow_reference_nan_semantics():
+  """Periodic-y density with Neumann-y energy gives negative-pressure ghost
+  cells and NaN wave speeds; tf.minimum/maximum propagate NaN (selecting the
+  F_R branch), and the kernels must do the same."""
+  n, h = (14, 11, 9), 3
+  cfg = _cfg(n, h, 'WENO_5', 'HLLC', True, 'neumann')
+  for v in ('rho', 'rho_u'):
+    cfg['bc']['cell_averages'][v]['y'] = {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)}
+  st = random_state(n, h, seed=21)
+  with np.errstate(all='ignore'):
+    ref = O.rhs(st, {}, oracle_cfg(cfg))
+  got = _gpu_rhs(cfg, st)
+  for k in ref:
+    a, b = interior(got[k], h), interior(ref[k], h)
+    assert np.array_equal(np.isnan(a), np.isnan(b)), k
+    fin = ~np.isnan(b)
+    assert rel_linf(a[fin], b[fin]) <= TOL, k
