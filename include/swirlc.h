@@ -158,6 +158,38 @@ sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0,
                         float* const* uk, float* const* out, void* stream);
 
+/* ---- Adjoint (no reference counterpart) ------------------------------------
+ * The reference differentiates its TensorFlow graph with tf.GradientTape;
+ * these entry points are hand-written reverse-mode counterparts of sc_rhs and
+ * sc_rk_stage, for gradients of losses of RK rollouts.  Every face's forward
+ * arithmetic is recomputed and differentiated exactly as executed (HLLC
+ * branch, wave-speed min/max and MUSCL limiter selections replayed; WENO
+ * smoothness-indicator and weight derivatives included).  dt is a constant
+ * (no derivative through a CFL time step).  Restrictions: 5 fields, no
+ * primitive-variable BCs.  Cotangents ("lam") are full-size field arrays of
+ * which only interior cells are read; outputs have zero halo cells. */
+
+/* Allocates the context's adjoint work arrays (14 full-size fp32 fields);
+ * required once before the calls below, which allocate nothing. */
+sc_status sc_ctx_enable_adjoint(sc_ctx* ctx);
+
+/* lq = (d rhs / d q)^T g: the vector-Jacobian product of sc_rhs at q
+ * (equations/compressible_navier_stokes.py:69-268 differentiated, halo fill
+ * included, so lq is the adjoint of q's interior cells).  Fills q's halos in
+ * place. */
+sc_status sc_rhs_vjp(sc_ctx* ctx, float* const* q, const float* const* g,
+                     float* const* lq, void* stream);
+
+/* Reverse of one sc_rk_stage (form SC_RK_EULER or SC_RK_BLEND) given the
+ * adjoint lam_out of its output:
+ *   lam_uk  = c1 * (lam_out + dt * (d rhs / d uk)^T lam_out)   (c1 = 1 for EULER)
+ *   lam_u0 += c0 * lam_out                                     (BLEND, if not NULL)
+ * lam_uk may alias lam_out.  Fills uk's halos in place. */
+sc_status sc_rk_stage_vjp(sc_ctx* ctx, int form, float c0, float c1,
+                          const float* dt_dev, float* const* uk,
+                          const float* const* lam_out, float* const* lam_uk,
+                          float* const* lam_u0, void* stream);
+
 /* ---- Batched ensembles (no reference counterpart) ------------------------
  * Many independent grids of different sizes sharing one configuration,
  * advanced together: each phase (halo fill per dim, x / y / z sweep, wave

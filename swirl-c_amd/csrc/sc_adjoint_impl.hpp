@@ -1,0 +1,440 @@
+// Adjoint sweeps: the vector-Jacobian product (VJP) of the RHS,
+//   lq = (dR/dq)^T g,
+// for a state q (halos filled) and a cotangent g of the RHS on the interior.
+//
+// The forward RHS is R[c] = src(q[c]) - sum_d (F_d[c+1/2] - F_d[c-1/2]) / h_d
+// with every face flux a function of the primitives of the cells of its
+// stencil.  Each adjoint pass handles one direction d and is written as a
+// gather: a thread owns a run of cells and recomputes every face whose
+// stencil reaches them, so each cell's adjoint is summed by one thread, in a
+// fixed order (deterministic, no atomics).  Per face:
+//   lF = (g[c] - g[c-1]) / h_d                     (flux adjoint)
+//   Riemann adjoint (the forward branch and min/max arguments replayed)
+//     -> adjoints of the reconstructed left / right face primitives,
+//   viscous adjoint -> adjoints of the two cell primitives and of the
+//     transverse velocity derivatives of both cells,
+//   reconstruction adjoint (WENO smoothness indicators and weights included)
+//     -> adjoints of the stencil cells' primitives.
+// A cell's primitive adjoints are converted to conserved adjoints once all
+// of its faces are done.  Transverse velocity derivatives are central
+// differences, so their adjoints are stored per cell (lG) and gathered by a
+// separate kernel; the halo fill is then transposed (halo adjoints summed
+// into the interior cells they were copied from, z, y, x order).
+#pragma once
+
+#include "sc_adjoint_device.hpp"
+#include "sc_halo.hpp"
+#include "sc_sweep_impl.hpp"
+
+namespace sc {
+
+// lG[gidx(i, k)]: adjoint of the central derivative d(u_i)/dx_k at a cell.
+__host__ __device__ constexpr int gidx(int i, int k) { return 3 * i + k; }
+
+struct AdjArgs {
+  SweepArgs s;    // the state (halos filled), geometry and physics
+  CFields g;      // cotangent of the RHS (interior cells are read)
+  Fields lq;      // state adjoint, full grid, accumulated with +=
+  float* lG[9];   // velocity-gradient adjoints (viscous), full grid, +=
+};
+
+// Conserved adjoint of one cell from its primitive adjoint (and the gravity
+// source adjoint of the x pass), accumulated into lq.
+__device__ __forceinline__ void retire_cell(const AdjArgs& A, long long idx, const float* W,
+                                            float pe, const float* lw, bool src,
+                                            bool interior) {
+  float lq[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  cons_to_prim_adj(W, pe, lw, lq);
+  const SweepArgs& a = A.s;
+  if (src && interior && a.grav_src) {
+    // R[1+d] += (rho * G) * g_d on the selected momenta.
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (a.grav_mask & (1 << d)) lq[0] += A.g.p[1 + d][idx] * (kG * a.gdir[d]);
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) A.lq.p[f][idx] += lq[f];
+}
+
+template <int D>
+__device__ __forceinline__ void add_lG(const AdjArgs& A, long long idx, const float (*lt)[2]) {
+  int t = 0;
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) {
+    if (dd == D) continue;
+    A.lG[gidx(dd, dd)][idx] += lt[t][0];
+    A.lG[gidx(D, dd)][idx] += lt[t][1];
+    ++t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Marching adjoint pass along x (DIR 0) or z (DIR 2).  Block (64, BY): lane ->
+// y, threadIdx.y -> the other transverse index, blockIdx.z -> a run of
+// seg_len owned cells of the extended range [h-1-RAD, h+n+1+RAD) (every cell
+// a face stencil of the direction reaches).  Iteration c handles face c (the
+// face between cells c-1 and c) and the reconstruction adjoint of cell c-1.
+// ---------------------------------------------------------------------------
+template <int DIR, int SCHEME, int FLUX, bool VISC>
+__device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& ti) {
+  constexpr int RAD = Radius<SCHEME>::value;
+  constexpr int WN = 2 * RAD + 1;
+  constexpr int NW = WN + 1;  // window: cells c-1-RAD .. c+RAD
+  constexpr int NV = 7;
+  const SweepArgs& a = A.s;
+  const int j = a.h + ti.bx * 64 + ti.tx;
+  const int o = a.h + ti.by * ti.by_dim + ti.ty;
+  const int n = DIR == 0 ? a.nx : a.nz;
+  const int n_other = DIR == 0 ? a.nz : a.nx;
+  if (j >= a.h + a.ny || o >= a.h + n_other) return;
+  const int ext_hi = a.h + n + 1 + RAD;
+  const int o_lo = a.h - 1 - RAD + ti.bz * a.seg_len;
+  if (o_lo >= ext_hi) return;
+  const int o_hi = min(o_lo + a.seg_len, ext_hi);
+  const int f_hi = a.h + n;  // faces h .. h+n are the interior faces
+  const int c_start = max(o_lo - RAD, a.h);
+  const int c_last = min(o_hi + RAD, f_hi + 1);
+  const int NM = DIR == 0 ? a.NX : a.NZ;
+  const long long sm = DIR == 0 ? a.sx : a.sz;
+  const long long so = DIR == 0 ? a.sz : a.sx;
+  const long long base = (long long)o * so + j;
+  auto kk = [&](int c) { return DIR == 0 ? o : c; };
+  auto ii = [&](int c) { return DIR == 0 ? c : o; };
+  auto interior = [&](int c) { return c >= a.h && c < a.h + n; };
+  auto owned = [&](int c) { return c >= o_lo && c < o_hi; };
+  auto load_g = [&](int c, float* gv) {
+    const bool in = interior(c);
+    const long long idx = base + (long long)(in ? c : a.h) * sm;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gv[f] = in ? A.g.p[f][idx] : 0.0f;
+  };
+
+  float W[NW][NV];
+  float ACC[WN][NV];
+#pragma unroll
+  for (int w = 0; w < WN; ++w) {
+    const int c = c_start - 1 - RAD + w;
+    load_prim<0>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
+                 W[w]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ACC[w][v] = 0.0f;
+  }
+  // Left state of face c_start (reconstruction of cell c_start - 1).
+  float fpPrev[NV], lRprev[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    float s[WN], dummy;
+#pragma unroll
+    for (int w = 0; w < WN; ++w) s[w] = W[w][v];
+    recon<SCHEME>(s, fpPrev[v], dummy);
+    lRprev[v] = 0.0f;
+  }
+  float gPrev[5];
+  load_g(c_start - 1, gPrev);
+  float tPrev[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+  float ltPrev[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+  if constexpr (VISC) transverse_grads<DIR>(a, kk(c_start - 1), ii(c_start - 1), j, tPrev);
+
+  for (int c = c_start; c <= c_last; ++c) {
+    {
+      const int cn = min(c + RAD, NM - 1);
+      load_prim<0>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
+                   pe_at(a, kk(cn), ii(cn), j), W[NW - 1]);
+    }
+    const bool face = c <= f_hi;
+    // Reconstruction of cell c: right state of face c, left state of c+1.
+    float fpC[NV], fmC[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      float s[WN];
+#pragma unroll
+      for (int w = 0; w < WN; ++w) s[w] = W[w + 1][v];
+      recon<SCHEME>(s, fpC[v], fmC[v]);
+    }
+    float gC[5];
+    load_g(c, gC);
+    float lL[NV], lR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lL[v] = lR[v] = 0.0f;
+    float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+    float ltCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+    if (face) {
+      float lF[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) lF[f] = (gC[f] - gPrev[f]) * a.inv_hdir;
+      riemann_adj<FLUX, DIR>(fpPrev, fmC, pe_at(a, kk(c), ii(c), j), lF, lL, lR);
+      if constexpr (VISC) {
+        transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
+        // Cells c-1 and c sit at ACC slots RAD and RAD+1.
+        float lA[NV], lB[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lA[v] = lB[v] = 0.0f;
+        viscous_adj<DIR>(W[RAD], W[RAD + 1], tPrev, tCur, a.vc, lF, lA, lB, ltPrev, ltCur);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          ACC[RAD][v] += lA[v];
+          ACC[RAD + 1][v] += lB[v];
+        }
+      }
+    }
+    // Reconstruction adjoint of cell c-1: left state of face c, right state
+    // of face c-1.
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      float s[WN], la[WN];
+#pragma unroll
+      for (int w = 0; w < WN; ++w) {
+        s[w] = W[w][v];
+        la[w] = 0.0f;
+      }
+      recon_adj<SCHEME>(s, lL[v], lRprev[v], la);
+#pragma unroll
+      for (int w = 0; w < WN; ++w) ACC[w][v] += la[w];
+    }
+    if constexpr (VISC) {
+      // Cell c-1 has seen both of its faces.
+      if (owned(c - 1) && c - 1 <= f_hi) add_lG<DIR>(A, base + (long long)(c - 1) * sm, ltPrev);
+    }
+    // Cell c-1-RAD is complete.
+    {
+      const int x = c - 1 - RAD;
+      if (owned(x))
+        retire_cell(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), ACC[0],
+                    DIR == 0, interior(x));
+    }
+#pragma unroll
+    for (int w = 0; w < WN - 1; ++w) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ACC[w][v] = ACC[w + 1][v];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ACC[WN - 1][v] = 0.0f;
+#pragma unroll
+    for (int w = 0; w < NW - 1; ++w) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) W[w][v] = W[w + 1][v];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      fpPrev[v] = fpC[v];
+      lRprev[v] = lR[v];
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) gPrev[f] = gC[f];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      tPrev[t][0] = tCur[t][0];
+      tPrev[t][1] = tCur[t][1];
+      ltPrev[t][0] = ltCur[t][0];
+      ltPrev[t][1] = ltCur[t][1];
+    }
+  }
+  // Remaining cells of the window: c_last - RAD .. c_last + RAD - 1.
+#pragma unroll
+  for (int w = 0; w < WN - 1; ++w) {
+    const int x = c_last - RAD + w;
+    if (owned(x))
+      retire_cell(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), ACC[w],
+                  DIR == 0, interior(x));
+  }
+}
+
+template <int DIR, int SCHEME, int FLUX, bool VISC>
+__global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
+  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  adj_march_body<DIR, SCHEME, FLUX, VISC>(A, ti);
+}
+
+// ---------------------------------------------------------------------------
+// y adjoint pass (contiguous direction).  One wave per run of y cells: lane l
+// holds cell y0 - KA + l; lane l computes face l (cells l-1, l) and the
+// reconstruction adjoint of its own cell; a cell's adjoint gathers, through
+// shuffles, the contributions of the RAD lanes on either side.  Lanes
+// KA .. KA + OUTA - 1 write.
+// ---------------------------------------------------------------------------
+template <int SCHEME>
+struct YAdjGeom {
+  static constexpr int RAD = Radius<SCHEME>::value;
+  static constexpr int KA = 2 * RAD + 1;
+  static constexpr int OUTA = 62 - 4 * RAD;
+};
+
+template <int SCHEME, int FLUX, bool VISC>
+__device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) {
+  constexpr int RAD = YAdjGeom<SCHEME>::RAD;
+  constexpr int KA = YAdjGeom<SCHEME>::KA;
+  constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
+  constexpr int WN = 2 * RAD + 1;
+  constexpr int NV = 7;
+  const SweepArgs& a = A.s;
+  const int lane = ti.tx;
+  const int i = a.h + ti.by * ti.by_dim + ti.ty;
+  const int k = a.h + ti.bz;
+  if (i >= a.h + a.nx || k >= a.h + a.nz) return;  // uniform per wave
+  const int ext_hi = a.h + a.ny + 1 + RAD;
+  const int y0 = a.h - 1 - RAD + ti.bx * OUTA;
+  if (y0 >= ext_hi) return;
+  const int y = y0 - KA + lane;
+  const int yc = min(max(y, 0), a.NY - 1);
+  const long long row = (long long)k * a.sz + (long long)i * a.sx;
+  const long long idx = row + yc;
+  const bool out_lane = lane >= KA && lane < KA + OUTA && y < ext_hi;
+  const bool face = y >= a.h && y <= a.h + a.ny;
+  const bool rcell = y >= a.h - 1 && y <= a.h + a.ny;
+  const bool inner = y >= a.h && y < a.h + a.ny;
+  const float pe = pe_at(a, k, i, yc);
+
+  float W0[NV];
+  load_prim<0>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
+  float S[NV][WN], fp[NV], fm[NV], Am[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+#pragma unroll
+    for (int w = 0; w < WN; ++w) {
+      const int off = w - RAD;
+      S[v][w] = off == 0 ? W0[v] : __shfl(W0[v], lane + off, 64);
+    }
+    Am[v] = S[v][RAD - 1];
+    recon<SCHEME>(S[v], fp[v], fm[v]);
+  }
+  float gC[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) gC[f] = inner ? A.g.p[f][idx] : 0.0f;
+  float L[NV], gP[5];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) L[v] = __shfl(fp[v], lane - 1, 64);
+#pragma unroll
+  for (int f = 0; f < 5; ++f) gP[f] = __shfl(gC[f], lane - 1, 64);
+  float lL[NV], lR[NV], lA[NV], lB[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) lL[v] = lR[v] = lA[v] = lB[v] = 0.0f;
+  float lta[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, ltb[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+  float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, tPrev[2][2];
+  if constexpr (VISC) transverse_grads<1>(a, k, i, yc, tCur);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    tPrev[t][0] = __shfl(tCur[t][0], lane - 1, 64);
+    tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
+  }
+  if (face) {
+    float lF[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) lF[f] = (gC[f] - gP[f]) * a.inv_hdir;
+    riemann_adj<FLUX, 1>(L, fm, pe, lF, lL, lR);
+    if constexpr (VISC) viscous_adj<1>(Am, W0, tPrev, tCur, a.vc, lF, lA, lB, lta, ltb);
+  }
+  // Reconstruction adjoint of the own cell: left state of face y+1 (lane + 1)
+  // and right state of face y.
+  float la[NV][WN];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const float lfp = __shfl(lL[v], lane + 1, 64);
+#pragma unroll
+    for (int w = 0; w < WN; ++w) la[v][w] = 0.0f;
+    if (rcell) recon_adj<SCHEME>(S[v], lfp, lR[v], la[v]);
+  }
+  // Gather: contributions of lanes l - RAD .. l + RAD (reconstruction) and of
+  // face l + 1 (viscous, as its cell A).
+  float tot[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    float s = la[v][RAD] + lB[v];
+#pragma unroll
+    for (int w = 0; w < WN; ++w) {
+      const int off = w - RAD;  // lane l - off holds the contribution at w
+      if (off != 0) s += __shfl(la[v][w], lane - off, 64);
+    }
+    tot[v] = s + __shfl(lA[v], lane + 1, 64);
+  }
+  float lt[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    lt[t][0] = ltb[t][0] + __shfl(lta[t][0], lane + 1, 64);
+    lt[t][1] = ltb[t][1] + __shfl(lta[t][1], lane + 1, 64);
+  }
+  if (out_lane) {
+    retire_cell(A, idx, W0, pe, tot, false, inner);
+    if constexpr (VISC) {
+      if (rcell) add_lG<1>(A, idx, lt);
+    }
+  }
+}
+
+template <int SCHEME, int FLUX, bool VISC>
+__global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
+  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  adj_y_body<SCHEME, FLUX, VISC>(A, ti);
+}
+
+// Launches the three adjoint passes (x, y, z) for one scheme.
+template <int SCHEME, int FLUX, bool VISC>
+hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& tune,
+                               hipStream_t stream) {
+  constexpr int RAD = Radius<SCHEME>::value;
+  const SweepArgs& a = A.s;
+  const int seg = tune.seg_len;
+  A.s.seg_len = seg;
+  const dim3 blk_m(64, tune.by_march);
+  const dim3 blk_y(64, tune.by_y);
+  {
+    AdjArgs ax = A;
+    ax.s.inv_hdir = 1.0f / h[0];
+    const int ext = a.nx + 2 * RAD + 2;
+    const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_march - 1) / tune.by_march,
+                    (ext + seg - 1) / seg);
+    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC>), grid, blk_m, 0, stream, ax);
+  }
+  {
+    AdjArgs ay = A;
+    ay.s.inv_hdir = 1.0f / h[1];
+    constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
+    const int ext = a.ny + 2 * RAD + 2;
+    const dim3 grid((ext + OUTA - 1) / OUTA, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
+    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC>), grid, blk_y, 0, stream, ay);
+  }
+  {
+    AdjArgs az = A;
+    az.s.inv_hdir = 1.0f / h[2];
+    const int ext = a.nz + 2 * RAD + 2;
+    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
+                    (ext + seg - 1) / seg);
+    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC>), grid, blk_m, 0, stream, az);
+  }
+  return hipGetLastError();
+}
+
+template <int SCHEME>
+hipError_t launch_adj_sweeps(const AdjArgs& A, const float h[3], int flux, bool visc,
+                             const SweepTuning& tune, hipStream_t stream);
+
+#define SC_INSTANTIATE_ADJOINT(SCHEME)                                                    \
+  template <>                                                                             \
+  hipError_t launch_adj_sweeps<SCHEME>(const AdjArgs& A, const float h[3], int flux,      \
+                                       bool visc, const SweepTuning& tune,                \
+                                       hipStream_t stream) {                              \
+    if (flux == kHllc)                                                                    \
+      return visc ? launch_adj_sweeps_t<SCHEME, kHllc, true>(A, h, tune, stream)          \
+                  : launch_adj_sweeps_t<SCHEME, kHllc, false>(A, h, tune, stream);        \
+    return visc ? launch_adj_sweeps_t<SCHEME, kHll, true>(A, h, tune, stream)             \
+                : launch_adj_sweeps_t<SCHEME, kHll, false>(A, h, tune, stream);           \
+  }
+
+// Non-templated kernels of the adjoint (sc_adjoint.hip).
+hipError_t launch_adj_zero(float* const* p, int np, long long n, hipStream_t stream);
+hipError_t launch_adj_visc_gather(const AdjArgs& A, hipStream_t stream);
+// Transpose of the halo fill along d (lq in a.q); call for d = 2, 1, 0.
+hipError_t launch_halo_adj_dim(const HaloArgs& a, int d, hipStream_t stream);
+// Grid extents for the element-wise adjoint kernels.
+struct AdjGeom {
+  int n[3], N[3], h;
+};
+// Interior cells: lam_k = c1 * (lam_next + dt * lq), and lam0 += c0 * lam_next
+// when lam0 is set; halo cells of lam_k are zeroed.  lam_k may alias lam_next.
+hipError_t launch_rk_adj_combine(const float* const* lam_next, const float* const* lq,
+                                 float* const* lam_k, float* const* lam0, int nf,
+                                 const AdjGeom& g, float c0, float c1, const float* dt,
+                                 hipStream_t stream);
+
+}  // namespace sc

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/swirlc.h"
+#include "sc_adjoint_impl.hpp"
 #include "sc_halo.hpp"
 #include "sc_launch.hpp"
 
@@ -27,6 +28,10 @@ struct sc_ctx {
   // scratch RHS accumulator (device), one full-size array per field
   float* rhs_scratch[SC_MAX_FIELDS];
   unsigned* speed_bits;
+  // adjoint work arrays (sc_ctx_enable_adjoint): state adjoint and the nine
+  // velocity-gradient adjoints
+  float* adj_lq[5];
+  float* adj_lG[9];
   sc::HaloArgs halo;  // BC tables; field pointers filled per call
   sc::SweepArgs sweep;
   sc::SweepTuning tune;
@@ -253,6 +258,10 @@ sc_status sc_ctx_destroy(sc_ctx* c) {
   for (int f = 0; f < SC_MAX_FIELDS; ++f)
     if (c->rhs_scratch[f]) hipFree(c->rhs_scratch[f]);
   if (c->speed_bits) hipFree(c->speed_bits);
+  for (int f = 0; f < 5; ++f)
+    if (c->adj_lq[f]) hipFree(c->adj_lq[f]);
+  for (int f = 0; f < 9; ++f)
+    if (c->adj_lG[f]) hipFree(c->adj_lG[f]);
   delete c;
   return SC_OK;
 }
@@ -405,6 +414,148 @@ sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_
     e = sc::launch_branch_codes<sc::kMuscl>(a, dim, ns, codes, (hipStream_t)stream);
   }
   return hip_check(e, "branch code kernel");
+}
+
+sc_status sc_ctx_enable_adjoint(sc_ctx* c) {
+  if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_enable_adjoint: null context");
+  if (c->nf != 5)
+    return fail(SC_ERR_UNSUPPORTED, "the adjoint supports the 5 base conserved fields only");
+  if (c->params.prim_bc_mask != 0)
+    return fail(SC_ERR_UNSUPPORTED, "the adjoint does not support primitive-variable BCs");
+  if (c->adj_lq[0] != nullptr) return SC_OK;
+  sc_status st = SC_OK;
+  const size_t bytes = sizeof(float) * c->cells_full;
+  for (int f = 0; f < 5 && st == SC_OK; ++f)
+    st = hip_check(hipMalloc((void**)&c->adj_lq[f], bytes), "hipMalloc adjoint");
+  for (int f = 0; f < 9 && st == SC_OK; ++f)
+    st = hip_check(hipMalloc((void**)&c->adj_lG[f], bytes), "hipMalloc adjoint");
+  if (st != SC_OK) {
+    for (int f = 0; f < 5; ++f) {
+      if (c->adj_lq[f]) hipFree(c->adj_lq[f]);
+      c->adj_lq[f] = nullptr;
+    }
+    for (int f = 0; f < 9; ++f) {
+      if (c->adj_lG[f]) hipFree(c->adj_lG[f]);
+      c->adj_lG[f] = nullptr;
+    }
+  }
+  return st;
+}
+
+}  // extern "C"
+
+namespace {
+
+// lq = (dR/dq)^T g over the full grid (zero halo cells); q's halos are filled.
+sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* const* lq,
+                      hipStream_t s) {
+  sc_status st = sc_fill_halos(c, q, s);
+  if (st != SC_OK) return st;
+  const bool visc = c->params.include_diffusion != 0;
+  float* zero[14];
+  int nz = 0;
+  for (int f = 0; f < 5; ++f) zero[nz++] = lq[f];
+  if (visc)
+    for (int f = 0; f < 9; ++f) zero[nz++] = c->adj_lG[f];
+  st = hip_check(sc::launch_adj_zero(zero, nz, c->cells_full, s), "adjoint clear");
+  if (st != SC_OK) return st;
+  sc::AdjArgs A;
+  memset(&A, 0, sizeof(A));
+  A.s = c->sweep;
+  for (int f = 0; f < 5; ++f) {
+    A.s.q.p[f] = q[f];
+    A.g.p[f] = g[f];
+    A.lq.p[f] = lq[f];
+  }
+  for (int f = 0; f < 9; ++f) A.lG[f] = c->adj_lG[f];
+  A.s.rk_form = 0;
+  hipError_t e;
+  const int scheme = c->params.interpolation;
+  const int flux = c->params.numeric_flux;
+  if (scheme == SC_WENO_5) {
+    e = sc::launch_adj_sweeps<sc::kWeno5>(A, c->h, flux, visc, c->tune, s);
+  } else if (scheme == SC_WENO_3) {
+    e = sc::launch_adj_sweeps<sc::kWeno3>(A, c->h, flux, visc, c->tune, s);
+  } else {
+    e = sc::launch_adj_sweeps<sc::kMuscl>(A, c->h, flux, visc, c->tune, s);
+  }
+  st = hip_check(e, "adjoint sweep kernels");
+  if (st != SC_OK) return st;
+  if (visc) {
+    st = hip_check(sc::launch_adj_visc_gather(A, s), "adjoint viscous gather");
+    if (st != SC_OK) return st;
+  }
+  sc::HaloArgs ha = c->halo;
+  for (int f = 0; f < 5; ++f) ha.q[f] = lq[f];
+  for (int d = 2; d >= 0; --d) {
+    st = hip_check(sc::launch_halo_adj_dim(ha, d, s), "adjoint halo kernel");
+    if (st != SC_OK) return st;
+  }
+  return SC_OK;
+}
+
+sc_status check_adjoint_ready(sc_ctx* c, const char* who) {
+  if (c->adj_lq[0] == nullptr)
+    return fail(SC_ERR_INVALID, std::string(who) + ": call sc_ctx_enable_adjoint first");
+  return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sc_status sc_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* const* lq,
+                     void* stream) {
+  if (c == nullptr || q == nullptr || g == nullptr || lq == nullptr)
+    return fail(SC_ERR_INVALID, "sc_rhs_vjp: null argument");
+  sc_status st = check_adjoint_ready(c, "sc_rhs_vjp");
+  if (st != SC_OK) return st;
+  for (int f = 0; f < 5; ++f) {
+    if (q[f] == nullptr || g[f] == nullptr || lq[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_rhs_vjp: null field pointer");
+    for (int k = 0; k < 5; ++k)
+      if (lq[f] == q[k] || lq[f] == g[k])
+        return fail(SC_ERR_INVALID, "sc_rhs_vjp: lq must not alias q or g");
+  }
+  return run_rhs_vjp(c, q, g, lq, (hipStream_t)stream);
+}
+
+sc_status sc_rk_stage_vjp(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
+                          float* const* uk, const float* const* lam_out, float* const* lam_uk,
+                          float* const* lam_u0, void* stream) {
+  if (c == nullptr || uk == nullptr || lam_out == nullptr || lam_uk == nullptr ||
+      dt_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: null argument");
+  sc_status st = check_adjoint_ready(c, "sc_rk_stage_vjp");
+  if (st != SC_OK) return st;
+  if (form != SC_RK_EULER && form != SC_RK_BLEND)
+    return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: form must be SC_RK_EULER or SC_RK_BLEND");
+  for (int f = 0; f < 5; ++f) {
+    if (uk[f] == nullptr || lam_out[f] == nullptr || lam_uk[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: null field pointer");
+    if (lam_u0 != nullptr && lam_u0[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: null lam_u0 field pointer");
+    for (int k = 0; k < 5; ++k) {
+      if (lam_uk[f] == uk[k] || (lam_u0 != nullptr && (lam_u0[f] == uk[k] ||
+                                                        lam_u0[f] == lam_out[k] ||
+                                                        lam_u0[f] == lam_uk[k])))
+        return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: overlapping output arrays");
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  st = run_rhs_vjp(c, uk, lam_out, c->adj_lq, s);
+  if (st != SC_OK) return st;
+  sc::AdjGeom geo;
+  geo.h = c->grid.halo;
+  geo.n[0] = c->grid.nx;
+  geo.n[1] = c->grid.ny;
+  geo.n[2] = c->grid.nz;
+  for (int d = 0; d < 3; ++d) geo.N[d] = c->N[d];
+  const bool blend = form == SC_RK_BLEND;
+  return hip_check(sc::launch_rk_adj_combine(lam_out, c->adj_lq, lam_uk,
+                                             blend ? lam_u0 : nullptr, 5, geo,
+                                             blend ? c0 : 0.0f, blend ? c1 : 1.0f, dt_dev, s),
+                   "adjoint RK combine");
 }
 
 }  // extern "C"

@@ -27,7 +27,8 @@ EXPORTED = (
     'sc_ctx_create', 'sc_ctx_destroy', 'sc_fill_halos', 'sc_rhs',
     'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes', 'sc_sweep_only',
     'sc_batch_create', 'sc_batch_destroy', 'sc_batch_fill_halos', 'sc_batch_rk_stage',
-    'sc_batch_max_wave_speed', 'sc_last_error', 'sc_version',
+    'sc_batch_max_wave_speed', 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
+    'sc_last_error', 'sc_version',
 )
 
 
@@ -107,12 +108,16 @@ def lib():
     L.sc_batch_fill_halos.argtypes = [vp, fpp, vp]
     L.sc_batch_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_batch_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
+    L.sc_ctx_enable_adjoint.argtypes = [vp]
+    L.sc_rhs_vjp.argtypes = [vp, fpp, fpp, fpp, vp]
+    L.sc_rk_stage_vjp.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, fpp, vp]
     L.sc_last_error.restype = ctypes.c_char_p
     L.sc_version.restype = ctypes.c_char_p
     for name in ('sc_ctx_create', 'sc_ctx_destroy', 'sc_fill_halos', 'sc_rhs',
                  'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes',
                  'sc_sweep_only', 'sc_batch_create', 'sc_batch_destroy',
-                 'sc_batch_fill_halos', 'sc_batch_rk_stage', 'sc_batch_max_wave_speed'):
+                 'sc_batch_fill_halos', 'sc_batch_rk_stage', 'sc_batch_max_wave_speed',
+                 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

@@ -268,3 +268,28 @@ class NativeContext:
     N.check(N.lib().sc_hllc_branch_codes(
         self._ctx, N.ptr_array(q), int(dim), ctypes.c_void_p(codes.data_ptr()),
         N.stream_handle(stream)), 'sc_hllc_branch_codes')
+
+  # ---- Adjoint -------------------------------------------------------------
+
+  def enable_adjoint(self):
+    """Allocates the context's adjoint work arrays (once)."""
+    if not getattr(self, '_adjoint', False):
+      N.check(N.lib().sc_ctx_enable_adjoint(self._ctx), 'sc_ctx_enable_adjoint')
+      self._adjoint = True
+
+  def rhs_vjp(self, q, g, lq, stream=None):
+    """lq = (d rhs / d q)^T g (interior adjoint of q; zero halo cells)."""
+    self.enable_adjoint()
+    N.check(N.lib().sc_rhs_vjp(self._ctx, N.ptr_array(q), N.ptr_array(g), N.ptr_array(lq),
+                               N.stream_handle(stream)), 'sc_rhs_vjp')
+
+  def rk_stage_vjp(self, form, c0, c1, dt_dev, uk, lam_out, lam_uk, lam_u0=None,
+                   stream=None):
+    """Reverse of rk_stage: lam_uk = c1 (lam_out + dt (d rhs/d uk)^T lam_out),
+    lam_u0 += c0 lam_out (BLEND form, if given)."""
+    self.enable_adjoint()
+    N.check(N.lib().sc_rk_stage_vjp(
+        self._ctx, int(form), ctypes.c_float(c0), ctypes.c_float(c1),
+        ctypes.c_void_p(dt_dev.data_ptr()), N.ptr_array(uk), N.ptr_array(lam_out),
+        N.ptr_array(lam_uk), N.ptr_array(lam_u0) if lam_u0 is not None else None,
+        N.stream_handle(stream)), 'sc_rk_stage_vjp')
