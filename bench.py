@@ -26,9 +26,10 @@ for _p in (ROOT, os.path.join(ROOT, 'swirl-c_amd')):
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # Algorithmic bytes per interior cell per launch of each sweep kernel (fp32,
-# 5 conserved fields): x reads U, writes R; y reads U and R, writes R; z in an
-# RK3 stage 2/3 reads U, R and U0 and writes the new state.
-SWEEP_BYTES = {'x': 40, 'y': 60, 'z': 80}
+# 5 conserved fields), in launch order: x reads U, writes R; z reads U and R,
+# writes R; y (last) in an RK3 stage 2/3 reads U, R and U0 and writes the new
+# state.
+SWEEP_BYTES = {'x': 40, 'y': 80, 'z': 60}
 # Per RK3 step (viscous, per-direction-sweep accounting): 700 B per cell.
 STEP_BYTES_VISCOUS = 700
 
@@ -41,12 +42,19 @@ def parse():
   ap.add_argument('--n', type=int, default=256)
   ap.add_argument('--kernel-reps', type=int, default=10)
   ap.add_argument('--no-cpu', action='store_true')
-  ap.add_argument('--cpu-n', type=int, default=64)
-  ap.add_argument('--cpu-steps', type=int, default=2)
+  ap.add_argument('--cpu-n', type=int, default=128)
+  ap.add_argument('--cpu-steps', type=int, default=3)
   ap.add_argument('--batch-instances', type=int, default=256)
   ap.add_argument('--batch-steps', type=int, default=10)
   ap.add_argument('--batch-loop-steps', type=int, default=3)
   ap.add_argument('--no-batch', action='store_true')
+  ap.add_argument('--adjoint-n', type=int, default=128)
+  ap.add_argument('--adjoint-steps', type=int, default=200)
+  ap.add_argument('--adjoint-every', type=int, default=20)
+  ap.add_argument('--no-adjoint', action='store_true')
+  ap.add_argument('--pmc-probe', action='store_true',
+                  help='launch a 256 MiB copy kernel, then each sweep kernel twice on the '
+                       'state, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes')
   return ap.parse_args()
 
 
@@ -71,6 +79,34 @@ def time_sweeps(solver, reps):
     ev1.synchronize()
     out[name] = ev0.elapsed_time(ev1) / 1e3 / reps
   return out
+
+
+def pmc_probe(n):
+  """Kernel launches for HBM-traffic counters: a copy of 64 Mi floats (256 MiB
+  read + 256 MiB written: the counters' calibration), then the x, z and y
+  sweeps (RK3 stage-2 form) twice each on the n^3 Kelvin-Helmholtz state."""
+  import torch
+  from swirlc_amd import _native as N
+  from swirlc_amd import configs
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.core.solver import Solver
+  src = torch.ones(64 << 20, device='cuda')
+  dst = torch.empty_like(src)
+  torch.mul(src, 1.0, out=dst)
+  torch.cuda.synchronize()
+  del src, dst
+  cfg_d, st = configs.kh(n=n)
+  solver = Solver(SwirlCParameters(cfg_d), {k: torch.from_numpy(v).cuda() for k, v in st.items()},
+                  cfl=0.5, use_graph=False)
+  del st
+  solver.step(1)
+  ctx = solver.ctx
+  ctx.fill_halos(solver.u)
+  for d in (0, 2, 1):
+    for _ in range(2):
+      ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.u, solver.u, solver.s2)
+  torch.cuda.synchronize()
+  print(f'[pmc-probe] copy 268435456 B each way; sweeps x, z, y x2 at {n}^3', flush=True)
 
 
 def bench_batch(n_inst, steps, loop_steps):
@@ -133,6 +169,54 @@ def bench_batch(n_inst, steps, loop_steps):
           'finite': finite}
 
 
+def bench_adjoint(n, steps, every):
+  """Gradient of J = sum 0.5 rho |u|^2 + 0.1 sum (rho - rho_target)^2 after a
+  `steps`-step SSP-RK3 rollout of the viscous Kelvin-Helmholtz setup at n^3
+  (fixed dt = the CFL-0.5 step of the initial state), with respect to all
+  five initial conserved fields; RK states checkpointed every `every` steps
+  and recomputed.  Reports forward and backward (adjoint incl. recompute)
+  times and their ratio."""
+  import numpy as np
+  import torch
+  from swirlc_amd import configs
+  from swirlc_amd.core import adjoint as A
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.equations.compressible_navier_stokes import get_rhs_fn
+  cfg_d, st = configs.kh(n=n)
+  cfg = SwirlCParameters(cfg_d)
+  nctx = get_rhs_fn(None, cfg).native_context
+  h = cfg_d['halo_width']
+  u0 = [torch.from_numpy(st[k]).cuda() for k in nctx.names]
+  del st
+  speed = torch.zeros(4, device='cuda')
+  dt = torch.zeros(1, device='cuda')
+  nctx.max_wave_speed(u0, speed, 0.5, dt)
+  rho_t = torch.from_numpy(np.random.default_rng(1).uniform(
+      0.9, 2.1, tuple(u0[0].shape)).astype(np.float32)).cuda()
+  # Warm-up (kernels loaded, adjoint buffers allocated).
+  w = [t.clone().requires_grad_(True) for t in u0]
+  A.kinetic_density_loss(A.rollout(nctx, dt, 2, *w, every=1), rho_t, h).backward()
+  torch.cuda.synchronize()
+  x = [t.clone().requires_grad_(True) for t in u0]
+  t0 = time.perf_counter()
+  out = A.rollout(nctx, dt, steps, *x, every=every)
+  torch.cuda.synchronize()
+  t_fwd = time.perf_counter() - t0
+  J = A.kinetic_density_loss(out, rho_t, h)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  J.backward()
+  torch.cuda.synchronize()
+  t_bwd = time.perf_counter() - t0
+  finite = all(bool(torch.isfinite(t.grad).all()) for t in x)
+  gnorm = float(sum((t.grad.double() ** 2).sum() for t in x) ** 0.5)
+  return {'workload': f'kelvin_helmholtz_{n}^3_viscous_ssprk3_{steps}steps_ckpt{every}',
+          'dt': float(dt[0]), 'loss': float(J), 'grad_norm': gnorm, 'finite': finite,
+          'forward_s': round(t_fwd, 4), 'adjoint_s': round(t_bwd, 4),
+          'adjoint_over_forward': round(t_bwd / t_fwd, 3),
+          'forward_Mcell_updates_per_s': round(n ** 3 * steps / t_fwd / 1e6, 1)}
+
+
 def cpu_baseline(n, steps):
   """The fp32 numpy oracle on an n^3 sample of the same workload (1 thread)."""
   import numpy as np
@@ -160,6 +244,9 @@ def main():
     print(f'bench.py runs on one GPU only (asked for --gpus {args.gpus}); the '
           'multi-GPU decomposition is not part of this build.', file=sys.stderr)
     sys.exit(2)
+  if args.pmc_probe:
+    pmc_probe(args.n)
+    return
   import torch
   from swirlc_amd import configs
   from swirlc_amd.core.parameter import SwirlCParameters
@@ -206,6 +293,14 @@ def main():
     batched = bench_batch(args.batch_instances, args.batch_steps, args.batch_loop_steps)
     print(f'[bench] batched: {batched}', file=sys.stderr, flush=True)
 
+  adjoint = None
+  if not args.no_adjoint:
+    print(f'[bench] adjoint: {args.adjoint_n}^3, {args.adjoint_steps} steps', file=sys.stderr,
+          flush=True)
+    torch.cuda.empty_cache()
+    adjoint = bench_adjoint(args.adjoint_n, args.adjoint_steps, args.adjoint_every)
+    print(f'[bench] adjoint: {adjoint}', file=sys.stderr, flush=True)
+
   cpu = None
   if not args.no_cpu:
     print(f'[bench] CPU oracle sample {args.cpu_n}^3 x {args.cpu_steps} steps',
@@ -223,6 +318,7 @@ def main():
       'step_hbm_frac_700B': round(step_frac, 4),
       'cpu_baseline': cpu,
       'batched': batched,
+      'adjoint': adjoint,
       'finite': finite,
   }
   print(json.dumps(line), flush=True)

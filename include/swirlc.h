@@ -150,8 +150,9 @@ sc_status sc_hllc_branch_codes(sc_ctx* ctx, const float* const* q, int dim,
 
 /* Profiling: launches only the sweep kernel of direction dim (0 x, 1 y, 2 z)
  * of an sc_rk_stage call (form 0 = plain RHS into the context's scratch), on
- * uk whose halos are already filled; the x sweep writes the scratch RHS, the
- * y sweep updates it, the z sweep finishes it (and writes out for form != 0).
+ * uk whose halos are already filled.  A stage runs x, z, y in that order: the
+ * x sweep writes the scratch RHS, the z sweep updates it, the y sweep
+ * finishes it (and writes out for form != 0).
  * No reference counterpart: it exposes one kernel of the fused stage so
  * its duration can be timed in isolation. */
 sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,

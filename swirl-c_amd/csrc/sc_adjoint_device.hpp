@@ -56,35 +56,43 @@ template <>
 __device__ __forceinline__ void recon_adj<kWeno5>(const float* a, float lfp, float lfm,
                                                   float* la) {
   const float vm2 = a[0], vm1 = a[1], v0 = a[2], vp1 = a[3], vp2 = a[4];
+  // First differences of the stencil: the smoothness indicators and the
+  // differences of the candidate values are formed from them (accurate to
+  // their own size on smooth data).
+  const float dm2 = vm1 - vm2, dm1 = v0 - vm1, d0 = vp1 - v0, d1 = vp2 - vp1;
   const float c1312 = (float)(13.0 / 12.0);
-  const float t0 = v0 - 2.0f * vp1 + vp2, s0 = 3.0f * v0 - 4.0f * vp1 + vp2;
-  const float t1 = vm1 - 2.0f * v0 + vp1, s1 = vm1 - vp1;
-  const float t2 = vm2 - 2.0f * vm1 + v0, s2 = vm2 - 4.0f * vm1 + 3.0f * v0;
+  const float t0 = d1 - d0, s0 = d1 - 3.0f * d0;
+  const float t1 = d0 - dm1, s1 = -(dm1 + d0);
+  const float t2 = dm1 - dm2, s2 = 3.0f * dm1 - dm2;
   const float b0 = c1312 * (t0 * t0) + 0.25f * (s0 * s0);
   const float b1 = c1312 * (t1 * t1) + 0.25f * (s1 * s1);
   const float b2 = c1312 * (t2 * t2) + 0.25f * (s2 * s2);
   const float c13 = (float)(1.0 / 3.0), c56 = (float)(5.0 / 6.0);
   const float c16 = (float)(-1.0 / 6.0), c76 = (float)(-7.0 / 6.0);
   const float c116 = (float)(11.0 / 6.0);
-  const float qn0 = (c13 * v0 + c56 * vp1) + c16 * vp2;
-  const float qn1 = (c16 * vm1 + c56 * v0) + c13 * vp1;
-  const float qn2 = (c13 * vm2 + c76 * vm1) + c116 * v0;
-  const float qp0 = (c116 * v0 + c76 * vp1) + c13 * vp2;
-  const float qp1 = (c13 * vm1 + c56 * v0) + c16 * vp1;
-  const float qp2 = (c16 * vm2 + c56 * vm1) + c13 * v0;
   const float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
   const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1), i2 = frcp(e2 * e2);
   const float an0 = 0.3f * i0, an1 = 0.6f * i1, an2 = 0.1f * i2;
   const float ap0 = 0.1f * i0, ap1 = 0.6f * i1, ap2 = 0.3f * i2;
   const float rn = frcp((an0 + an1) + an2);
   const float rp = frcp((ap0 + ap1) + ap2);
-  const float fp = ((an0 * qn0 + an1 * qn1) + an2 * qn2) * rn;
-  const float fm = ((ap0 * qp0 + ap1 * qp1) + ap2 * qp2) * rp;
-  // f = N / S: dN = lf / S, dS = -lf * f / S.
-  const float lNn = lfp * rn, lSn = -lfp * fp * rn;
-  const float lNp = lfm * rp, lSp = -lfm * fm * rp;
-  const float lan0 = lNn * qn0 + lSn, lan1 = lNn * qn1 + lSn, lan2 = lNn * qn2 + lSn;
-  const float lap0 = lNp * qp0 + lSp, lap1 = lNp * qp1 + lSp, lap2 = lNp * qp2 + lSp;
+  // f = N / S with N = sum a_r q_r, S = sum a_r: df/da_r = (q_r - f) / S,
+  // and q_r - f = sum_s w_s (q_r - q_s) (not q_r - f in fp32, which cancels
+  // when the candidates nearly agree).
+  const float c16p = (float)(1.0 / 6.0);
+  const float n01 = c16p * ((2.0f * d0 - dm1) - d1);   // qn0 - qn1
+  const float n12 = c13 * ((dm2 - 2.0f * dm1) + d0);   // qn1 - qn2
+  const float p01 = c13 * ((dm1 - 2.0f * d0) + d1);    // qp0 - qp1
+  const float p12 = c16p * ((2.0f * dm1 - dm2) - d0);  // qp1 - qp2
+  const float wn0 = an0 * rn, wn1 = an1 * rn, wn2 = an2 * rn;
+  const float wp0 = ap0 * rp, wp1 = ap1 * rp, wp2 = ap2 * rp;
+  const float lNn = lfp * rn, lNp = lfm * rp;
+  const float lan0 = lNn * ((wn1 + wn2) * n01 + wn2 * n12);
+  const float lan1 = lNn * (wn2 * n12 - wn0 * n01);
+  const float lan2 = lNn * (-(wn0 * (n01 + n12) + wn1 * n12));
+  const float lap0 = lNp * ((wp1 + wp2) * p01 + wp2 * p12);
+  const float lap1 = lNp * (wp2 * p12 - wp0 * p01);
+  const float lap2 = lNp * (-(wp0 * (p01 + p12) + wp1 * p12));
   const float lqn0 = lNn * an0, lqn1 = lNn * an1, lqn2 = lNn * an2;
   const float lqp0 = lNp * ap0, lqp1 = lNp * ap1, lqp2 = lNp * ap2;
   // alpha = d * i, i = (eps + beta)^-2: d i / d beta = -2 i / (eps + beta).
@@ -131,12 +139,12 @@ __device__ __forceinline__ void recon_adj<kWeno3>(const float* a, float lfp, flo
   const float an0 = d0 * i0, an1 = d1 * i1;
   const float ap0 = d1 * i0, ap1 = d0 * i1;
   const float rn = frcp(an0 + an1), rp = frcp(ap0 + ap1);
-  const float fp = (an0 * qn0 + an1 * qn1) * rn;
-  const float fm = (ap0 * qp0 + ap1 * qp1) * rp;
-  const float lNn = lfp * rn, lSn = -lfp * fp * rn;
-  const float lNp = lfm * rp, lSp = -lfm * fm * rp;
-  const float lan0 = lNn * qn0 + lSn, lan1 = lNn * qn1 + lSn;
-  const float lap0 = lNp * qp0 + lSp, lap1 = lNp * qp1 + lSp;
+  // df/da_r = (q_r - f) / S with q_r - f from the stencil differences.
+  const float n01 = 0.5f * (t0 - t1);  // qn0 - qn1
+  const float p01 = 0.5f * (t1 - t0);  // qp0 - qp1
+  const float lNn = lfp * rn, lNp = lfm * rp;
+  const float lan0 = lNn * (an1 * rn) * n01, lan1 = -lNn * (an0 * rn) * n01;
+  const float lap0 = lNp * (ap1 * rp) * p01, lap1 = -lNp * (ap0 * rp) * p01;
   const float lqn0 = lNn * an0, lqn1 = lNn * an1, lqp0 = lNp * ap0, lqp1 = lNp * ap1;
   const float li0 = d0 * lan0 + d1 * lap0, li1 = d1 * lan1 + d0 * lap1;
   const float lb0 = -2.0f * li0 * i0 * frcp(e0), lb1 = -2.0f * li1 * i1 * frcp(e1);

@@ -113,18 +113,18 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
                        dim3(tiles.grid[0]), blk_m, 0, stream, bx);
   }
   {
+    BatchArgs bz = b;
+    bz.tiles = tiles.t[2];
+    bz.ntiles = tiles.n[2];
+    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1>),
+                       dim3(tiles.grid[2]), blk_m, 0, stream, bz);
+  }
+  {
     BatchArgs by = b;
     by.tiles = tiles.t[1];
     by.ntiles = tiles.n[1];
     hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS>), dim3(tiles.grid[1]),
                        blk_y, 0, stream, by);
-  }
-  {
-    BatchArgs bz = b;
-    bz.tiles = tiles.t[2];
-    bz.ntiles = tiles.n[2];
-    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>),
-                       dim3(tiles.grid[2]), blk_m, 0, stream, bz);
   }
   return hipGetLastError();
 }

@@ -7,8 +7,8 @@
 
 namespace sc {
 
-// Launches the x, y and z sweeps of one RHS evaluation (plus the folded RK
-// update when a.rk_form != 0) for interpolation scheme SCHEME.
+// Launches the x, z and y sweeps of one RHS evaluation (plus the RK update
+// folded into the y sweep when a.rk_form != 0) for interpolation scheme SCHEME.
 // only_dir = -1 launches all three; 0, 1 or 2 launches that sweep alone.
 template <int SCHEME>
 hipError_t launch_rhs_sweeps(const SweepArgs& a, const float h[3], int flux,
@@ -37,7 +37,16 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>), grid,
                        blk_m, 0, stream, ax);
   }
-  // y sweep (middle): R -= div_y.
+  // z sweep (middle): R -= div_z.
+  if (only_dir < 0 || only_dir == 2) {
+    SweepArgs az = a;
+    az.inv_hdir = 1.0f / h[2];
+    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
+                    (a.nz + tune.seg_len - 1) / tune.seg_len);
+    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1>), grid,
+                       blk_m, 0, stream, az);
+  }
+  // y sweep (last): R -= div_y, then store R or the RK update.
   if (only_dir < 0 || only_dir == 1) {
     SweepArgs ay = a;
     ay.inv_hdir = 1.0f / h[1];
@@ -45,15 +54,6 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     const dim3 grid((a.ny + OUTW - 1) / OUTW, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
     hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS>), grid, blk_y, 0,
                        stream, ay);
-  }
-  // z sweep (last): R -= div_z, then store R or the RK update.
-  if (only_dir < 0 || only_dir == 2) {
-    SweepArgs az = a;
-    az.inv_hdir = 1.0f / h[2];
-    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
-                    (a.nz + tune.seg_len - 1) / tune.seg_len);
-    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>), grid,
-                       blk_m, 0, stream, az);
   }
   return hipGetLastError();
 }

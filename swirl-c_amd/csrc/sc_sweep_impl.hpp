@@ -1,15 +1,19 @@
 // Fused per-direction sweep kernels of the RHS (gfx950).
 //
-// One launch per direction d in x, y, z (the reference's order of flux
-// subtraction, equations/compressible_navier_stokes.py:255-264).  Each launch
-// reads the conserved fields of the stage state once, converts them to
-// primitives, reconstructs the face states (WENO / MUSCL), solves the Riemann
-// problem (HLL / HLLC), adds the viscous and heat flux, takes the flux
-// difference and accumulates it into the RHS:
+// One launch per direction.  Each launch reads the conserved fields of the
+// stage state once, converts them to primitives, reconstructs the face states
+// (WENO / MUSCL), solves the Riemann problem (HLL / HLLC), adds the viscous
+// and heat flux, takes the flux difference and accumulates it into the RHS:
 //   x sweep:  R  = src - (F_x[i+1] - F_x[i]) / dx          (writes R)
-//   y sweep:  R  = R   - (F_y[j+1] - F_y[j]) / dy          (reads, writes R)
-//   z sweep:  R  = R   - (F_z[k+1] - F_z[k]) / dz, then either writes R or
+//   z sweep:  R  = R   - (F_z[k+1] - F_z[k]) / dz          (reads, writes R)
+//   y sweep:  R  = R   - (F_y[j+1] - F_y[j]) / dy, then either writes R or
 //             the Runge-Kutta stage update of the state (RK axpy folded in).
+// The reference subtracts the divergences in x, y, z order
+// (equations/compressible_navier_stokes.py:255-264); here y is last because
+// its lanes run along the contiguous axis, so the RK operands (R, U0, Uk) of
+// every output cell are loaded at kernel start, off the critical path, while
+// a marching sweep would wait for them inside its loop.  The reordering only
+// changes the rounding of the three-term sum.
 // Layout [z][x][y], y contiguous.  The x and z sweeps map 64 lanes to 64
 // consecutive y cells (coalesced) and march each thread along its pencil
 // with a register sliding window; the y sweep maps lanes along y and gets
@@ -164,32 +168,22 @@ __device__ __forceinline__ float source_of(const SweepArgs& a, int f, float rho)
   return 0.0f;
 }
 
-// Last-sweep store: either the RHS value or the RK stage update
-// (rk3.py:69-83, rk2.py:67-86).
-__device__ __forceinline__ void store_last(const SweepArgs& a, int f, long long idx,
-                                           float val) {
-  if (a.rk_form == 0) {
-    a.r.p[f][idx] = val;
-    return;
-  }
-  const float dt = *a.dt;
-  const float uk = a.q.p[f][idx];
-  float o;
-  if (a.rk_form == 1) {
-    o = uk + val * dt;
-  } else if (a.rk_form == 2) {
-    o = a.u0.p[f][idx] * a.c0 + (uk + val * dt) * a.c1;
-  } else {
-    o = 0.5f * (a.u0.p[f][idx] + (uk + dt * val));
-  }
-  a.out.p[f][idx] = o;
+// Runge-Kutta stage update of one value from preloaded operands
+// (rk3.py:69-83, rk2.py:67-86): 1 out = uk + L*dt; 2 out = u0*c0 +
+// (uk + L*dt)*c1; 3 out = 0.5*(u0 + (uk + dt*L)).
+__device__ __forceinline__ float rk_update(const SweepArgs& a, float uk, float u0, float val,
+                                           float dt) {
+  if (a.rk_form == 1) return uk + val * dt;
+  if (a.rk_form == 2) return u0 * a.c0 + (uk + val * dt) * a.c1;
+  return 0.5f * (u0 + (uk + dt * val));
 }
 
 // ---------------------------------------------------------------------------
 // Marching sweep along x (DIR 0) or z (DIR 2).  Block (64, BY): lane -> y,
 // threadIdx.y -> the other transverse index, blockIdx.z -> segment of
 // seg_len cells along the sweep.  PHASE 0: first sweep (writes R = src - div);
-// PHASE 2: last sweep (R - div -> R or RK update).
+// PHASE 1: middle sweep (R -= div; the R of the next output cell is loaded
+// one iteration ahead).
 // ---------------------------------------------------------------------------
 // Block geometry of one tile: block indices (bx, by, bz) of the plain launch
 // and the thread's (tx, ty) with by_dim waves per block.
@@ -239,7 +233,20 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   // iteration ahead so the loads overlap the arithmetic of the current cell.
   float qn[NF];
   load_raw<NF>(a, base + (long long)(c_begin - 1 + RAD) * sm, qn);
+  float rpre[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) rpre[f] = 0.0f;
   for (int c = c_begin - 1; c <= c_end; ++c) {
+    float rcur[NF];
+    if constexpr (PHASE == 1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) rcur[f] = rpre[f];
+      if (c >= c_begin && c < c_end) {
+        const long long ridx = base + (long long)c * sm;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) rpre[f] = a.r.p[f][ridx];
+      }
+    }
     {
       const int cn = c + RAD;
       float qc[NF];
@@ -275,9 +282,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
           }
         } else {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            store_last(a, f, idx, a.r.p[f][idx] - (F[f] - Fprev[f]) * a.inv_hdir);
-          }
+          for (int f = 0; f < NF; ++f) a.r.p[f][idx] = rcur[f] - (F[f] - Fprev[f]) * a.inv_hdir;
         }
       }
 #pragma unroll
@@ -308,9 +313,10 @@ sweep_march_kernel(const SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// y sweep (contiguous direction).  Block (64, BY): one wave per (z, x) row
-// chunk; lane l holds cell y0 - K + l and the stencil comes from shuffles.
-// Outputs cells y0 .. y0 + OUTW - 1.  Middle sweep: R = R - div.
+// y sweep (contiguous direction, last).  Block (64, BY): one wave per (z, x)
+// row chunk; lane l holds cell y0 - K + l and the stencil comes from
+// shuffles.  Outputs cells y0 .. y0 + OUTW - 1: R - div into R (rk_form 0) or
+// the RK stage update into out.
 // ---------------------------------------------------------------------------
 template <int SCHEME>
 struct YGeom {
@@ -337,10 +343,17 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   const int yc = min(y, a.NY - 1);
   const long long idx = (long long)k * a.sz + (long long)i * a.sx + yc;
   const bool out_lane = lane >= K && lane < K + OUTW && y < a.h + a.ny;
-  // The RHS accumulator is read first so its latency overlaps the arithmetic.
-  float rin[NF];
+  // The RHS accumulator and the RK operands are read first so their latency
+  // overlaps the arithmetic.
+  const bool rk = a.rk_form != 0;
+  float rin[NF], ukv[NF], u0v[NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) rin[f] = out_lane ? a.r.p[f][idx] : 0.0f;
+  for (int f = 0; f < NF; ++f) {
+    rin[f] = out_lane ? a.r.p[f][idx] : 0.0f;
+    ukv[f] = (out_lane && rk) ? a.q.p[f][idx] : 0.0f;
+    u0v[f] = (out_lane && rk && a.rk_form != 1) ? a.u0.p[f][idx] : 0.0f;
+  }
+  const float dt = rk ? *a.dt : 0.0f;
   float W0[NV];
   load_prim<NS>(a, idx, in_core(yc, a.h, a.ny), pe_at(a, k, i, yc), W0);
   float negP[NV], posM[NV], Am[NV];
@@ -376,7 +389,12 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   if (out_lane) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      a.r.p[f][idx] = rin[f] - (Fn[f] - F[f]) * a.inv_hdir;
+      const float val = rin[f] - (Fn[f] - F[f]) * a.inv_hdir;
+      if (rk) {
+        a.out.p[f][idx] = rk_update(a, ukv[f], u0v[f], val, dt);
+      } else {
+        a.r.p[f][idx] = val;
+      }
     }
   }
 }
