@@ -66,14 +66,16 @@ def time_sweeps(solver, reps):
   stream = torch.cuda.current_stream()
   ctx.fill_halos(solver.u)
   out = {}
+  # RK3 stage-2 form with U0 (the last step's stage-1 state) distinct from Uk,
+  # as in a real stage.
   for d, name in enumerate(('x', 'y', 'z')):
-    ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.u, solver.u,
+    ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u,
                    solver.s2, stream)  # warm
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
     for _ in range(reps):
-      ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.u, solver.u,
+      ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u,
                      solver.s2, stream)
     ev1.record(stream)
     ev1.synchronize()
@@ -84,7 +86,9 @@ def time_sweeps(solver, reps):
 def pmc_probe(n):
   """Kernel launches for HBM-traffic counters: a copy of 64 Mi floats (256 MiB
   read + 256 MiB written: the counters' calibration), then the x, z and y
-  sweeps (RK3 stage-2 form) twice each on the n^3 Kelvin-Helmholtz state."""
+  sweeps (RK3 stage-2 form, U0 distinct from Uk) twice each on the n^3
+  Kelvin-Helmholtz state, in the autotuned launch geometry (the last six
+  dispatches)."""
   import torch
   from swirlc_amd import _native as N
   from swirlc_amd import configs
@@ -101,10 +105,14 @@ def pmc_probe(n):
   del st
   solver.step(1)
   ctx = solver.ctx
+  from swirlc_amd.core import autotune
+  tuning, _ = autotune.tune_context(ctx, solver.u, solver.s2, solver.dt, u0=solver.s1, reps=2)
+  torch.cuda.synchronize()
+  print(f'[pmc-probe] tuning {tuning}', flush=True)
   ctx.fill_halos(solver.u)
   for d in (0, 2, 1):
     for _ in range(2):
-      ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.u, solver.u, solver.s2)
+      ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u, solver.s2)
   torch.cuda.synchronize()
   print(f'[pmc-probe] copy 268435456 B each way; sweeps x, z, y x2 at {n}^3', flush=True)
 
@@ -118,9 +126,11 @@ def bench_batch(n_inst, steps, loop_steps):
   from swirlc_amd.core.batch import BatchSolver
   from swirlc_amd.core.parameter import SwirlCParameters
   from swirlc_amd.core.solver import Solver
+  from swirlc_amd.core import autotune
   cfg, members = configs.riemann_ensemble(n_inst, seed=0)
   cells = sum(g[0] * g[1] * g[2] for g, _ in members)
   batch = BatchSolver(SwirlCParameters(dict(cfg)), members, cfl=0.5, use_graph=True)
+  tuning, _ = autotune.tune_batch(batch, reps=2)
   batch.step(2)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -135,6 +145,7 @@ def bench_batch(n_inst, steps, loop_steps):
                     use_graph=False) for g, st in members]
   del members
   for s in solvers:
+    s.ctx.set_tuning(tuning)  # same geometry as the batched kernels
     s.step(1)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -166,7 +177,7 @@ def bench_batch(n_inst, steps, loop_steps):
           'loop_graph_ms_per_step': round(t_loop_graph * 1e3, 3),
           'speedup_vs_loop': round(t_loop / t_batch, 2),
           'speedup_vs_graphed_loop': round(t_loop_graph / t_batch, 2),
-          'finite': finite}
+          'finite': finite, 'tuning': tuning}
 
 
 def bench_adjoint(n, steps, every):
@@ -188,6 +199,11 @@ def bench_adjoint(n, steps, every):
   h = cfg_d['halo_width']
   u0 = [torch.from_numpy(st[k]).cuda() for k in nctx.names]
   del st
+  from swirlc_amd.core import autotune
+  scratch = [torch.zeros_like(t) for t in u0]
+  dt0 = torch.full((1,), 1e-4, device='cuda')
+  autotune.tune_context(nctx, [t.clone() for t in u0], scratch, dt0, reps=3)
+  del scratch
   speed = torch.zeros(4, device='cuda')
   dt = torch.zeros(1, device='cuda')
   nctx.max_wave_speed(u0, speed, 0.5, dt)
@@ -259,8 +275,10 @@ def main():
   del st
   solver = Solver(cfg, state, cfl=0.5, use_graph=True)
   del state
-  print(f'[bench] KH {n}^3 state on GPU; warmup {args.warmup} steps', file=sys.stderr,
-        flush=True)
+  from swirlc_amd.core import autotune
+  tuning, _ = autotune.tune_context(solver.ctx, solver.u, solver.s2, solver.dt, u0=solver.s1)
+  print(f'[bench] KH {n}^3 state on GPU; tuning {tuning}; warmup {args.warmup} steps',
+        file=sys.stderr, flush=True)
   solver.step(args.warmup)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -313,7 +331,7 @@ def main():
       'warmup': args.warmup, 'ms_per_step': round(ms, 4), 'higher_is_better': True,
       'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
       'config': {'workload': f'kelvin_helmholtz_{n}^3_weno5_hllc_viscous_ssprk3_cfl0.5',
-                 'cells': cells, 'parallelism': 'single-gpu'},
+                 'cells': cells, 'parallelism': 'single-gpu', 'tuning': tuning},
       'roofline': roofline,
       'step_hbm_frac_700B': round(step_frac, 4),
       'cpu_baseline': cpu,

@@ -88,6 +88,24 @@ typedef struct sc_grid {
   const float* z_full;
 } sc_grid;
 
+/* A Dirichlet value on a boundary face of the domain (the faces between halo
+ * and interior cells): cfg.bc['intercell_fluxes'][type][var][dim][side]
+ * (boundary/boundary.py:89-183) and cfg.bc['cell_faces'][var][dim][side]
+ * (boundary/boundary.py:230-461), scalar values. */
+#define SC_MAX_BOUNDARY_VALUES 64
+#define SC_MAX_FACE_VALUES 8      /* cell-face values per (dim, side) */
+#define SC_BV_FLUX_CONVECTIVE 0   /* var: conserved field index */
+#define SC_BV_FLUX_DIFFUSIVE 1
+#define SC_BV_FLUX_TOTAL 2
+#define SC_BV_FACE_CONSERVED 3    /* var: conserved field index 1.. (not rho) */
+#define SC_BV_FACE_PRIMITIVE 4    /* var: 1 u, 2 v, 3 w, 4 e, 5 p, 6 h, 7+s scalar s */
+typedef struct sc_boundary_value {
+  int kind;                 /* SC_BV_* */
+  int dim, side;            /* 0 x, 1 y, 2 z; 0 low, 1 high */
+  int var;
+  float value;
+} sc_boundary_value;
+
 typedef struct sc_params {
   int n_fields;             /* 5 + number of transported scalars */
   int interpolation;        /* SC_WENO_3 | SC_WENO_5 | SC_MUSCL */
@@ -101,6 +119,11 @@ typedef struct sc_params {
   /* bit f set: the BC of field f is given on its primitive (e.g. 'u' for
    * rho_u), boundary.py:516-531. */
   int prim_bc_mask;
+  /* Face and flux values; cell-face values of one (dim, side) are applied
+   * in list order, rho never (the reference's face update then drops p and
+   * h).  Cell-face values require include_diffusion == 0. */
+  int n_boundary_values;
+  sc_boundary_value boundary_values[SC_MAX_BOUNDARY_VALUES];
 } sc_params;
 
 typedef struct sc_ctx sc_ctx;
@@ -158,6 +181,22 @@ sc_status sc_hllc_branch_codes(sc_ctx* ctx, const float* const* q, int dim,
 sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0,
                         float* const* uk, float* const* out, void* stream);
+
+/* Launch geometry of the sweep kernels (no reference counterpart: the
+ * reference leaves scheduling to XLA).  seg_*: cells per thread of a marching
+ * sweep (>= 4); by_*: 64-lane waves per block (1..4); waves_*: 4 selects the
+ * build of the marching kernel limited to 128 VGPRs (4 waves per SIMD), 0 the
+ * default build.  Chosen per grid by swirlc_amd.core.autotune, or set by the
+ * caller; results do not depend on it (bit for bit). */
+typedef struct sc_tuning {
+  int seg_x, by_x, waves_x;
+  int seg_z, by_z, waves_z;
+  int by_y;
+  int xcd_remap;            /* 1: XCD-aware block order (single-grid sweeps) */
+} sc_tuning;
+
+sc_status sc_ctx_get_tuning(const sc_ctx* ctx, sc_tuning* out);
+sc_status sc_ctx_set_tuning(sc_ctx* ctx, const sc_tuning* tuning);
 
 /* ---- Adjoint (no reference counterpart) ------------------------------------
  * The reference differentiates its TensorFlow graph with tf.GradientTape;
@@ -217,6 +256,10 @@ sc_status sc_batch_fill_halos(sc_batch* batch, float* const* q, void* stream);
 sc_status sc_batch_rk_stage(sc_batch* batch, int form, float c0, float c1,
                             const float* dt_dev, const float* const* u0,
                             float* const* uk, float* const* out, void* stream);
+/* Tuning of the batched sweeps (sc_tuning above).  Setting it rebuilds the
+ * tile lists (allocates; not allowed during graph capture). */
+sc_status sc_batch_get_tuning(const sc_batch* batch, sc_tuning* out);
+sc_status sc_batch_set_tuning(sc_batch* batch, const sc_tuning* tuning);
 /* Per-instance maximum wave speed (speed_dev[i]) and, if dt_dev is not NULL,
  * dt_dev[i] = cfl * min(dx_i, dy_i, dz_i) / speed_dev[i]. */
 sc_status sc_batch_max_wave_speed(sc_batch* batch, const float* const* q, float cfl,

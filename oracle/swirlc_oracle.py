@@ -520,8 +520,9 @@ def _face_boundary_update_1d(cons, prim, spec, dim, plane):
     r = cons['rho'].copy()
     r[sel(r)] = spec['rho']
     cons['rho'] = r
-    new_prim = conservative_to_primitive(cons)
-    prim.update(new_prim)
+    # boundary.py:284-293 replaces the whole primitive dict: p and h are gone
+    # afterwards (a later flux evaluation then fails on the missing 'p').
+    prim = conservative_to_primitive(cons)
   for var, val in spec.items():
     if var == 'rho':
       continue

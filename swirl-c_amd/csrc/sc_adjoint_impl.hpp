@@ -368,22 +368,27 @@ __global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
   adj_y_body<SCHEME, FLUX, VISC>(A, ti);
 }
 
+// Launch geometry of the adjoint passes: owned cells per thread of the
+// marching passes and waves per block.
+constexpr int kAdjSeg = 32;
+constexpr int kAdjBy = 4;
+
 // Launches the three adjoint passes (x, y, z) for one scheme.
 template <int SCHEME, int FLUX, bool VISC>
 hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& tune,
                                hipStream_t stream) {
   constexpr int RAD = Radius<SCHEME>::value;
   const SweepArgs& a = A.s;
-  const int seg = tune.seg_len;
+  (void)tune;
+  const int seg = kAdjSeg;
   A.s.seg_len = seg;
-  const dim3 blk_m(64, tune.by_march);
-  const dim3 blk_y(64, tune.by_y);
+  const dim3 blk_m(64, kAdjBy);
+  const dim3 blk_y(64, kAdjBy);
   {
     AdjArgs ax = A;
     ax.s.inv_hdir = 1.0f / h[0];
     const int ext = a.nx + 2 * RAD + 2;
-    const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_march - 1) / tune.by_march,
-                    (ext + seg - 1) / seg);
+    const dim3 grid((a.ny + 63) / 64, (a.nz + kAdjBy - 1) / kAdjBy, (ext + seg - 1) / seg);
     hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC>), grid, blk_m, 0, stream, ax);
   }
   {
@@ -391,15 +396,14 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
     ay.s.inv_hdir = 1.0f / h[1];
     constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
     const int ext = a.ny + 2 * RAD + 2;
-    const dim3 grid((ext + OUTA - 1) / OUTA, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
+    const dim3 grid((ext + OUTA - 1) / OUTA, (a.nx + kAdjBy - 1) / kAdjBy, a.nz);
     hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC>), grid, blk_y, 0, stream, ay);
   }
   {
     AdjArgs az = A;
     az.s.inv_hdir = 1.0f / h[2];
     const int ext = a.nz + 2 * RAD + 2;
-    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
-                    (ext + seg - 1) / seg);
+    const dim3 grid((a.ny + 63) / 64, (a.nx + kAdjBy - 1) / kAdjBy, (ext + seg - 1) / seg);
     hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC>), grid, blk_m, 0, stream, az);
   }
   return hipGetLastError();

@@ -28,6 +28,9 @@ struct sc_ctx {
   // scratch RHS accumulator (device), one full-size array per field
   float* rhs_scratch[SC_MAX_FIELDS];
   unsigned* speed_bits;
+  // boundary face / flux values per [dim][side] (device) and dims using them
+  sc::BoundaryBc* bbc_dev;
+  int bbc_dims;
   // adjoint work arrays (sc_ctx_enable_adjoint): state adjoint and the nine
   // velocity-gradient adjoints
   float* adj_lq[5];
@@ -71,10 +74,84 @@ void fill_sweep_state(sc_ctx* c, const float* const* q) {
   for (int f = 0; f < c->nf; ++f) c->sweep.q.p[f] = q[f];
 }
 
-int env_int(const char* name, int dflt) {
-  const char* s = getenv(name);
-  if (s == nullptr || *s == '\0') return dflt;
-  return atoi(s);
+// Blocks are 64 x by threads and the kernels are built for at most 256.
+sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
+  if (t == nullptr) return fail(SC_ERR_INVALID, "null tuning");
+  const int seg[2] = {t->seg_x, t->seg_z};
+  const int by[3] = {t->by_x, t->by_z, t->by_y};
+  const int waves[2] = {t->waves_x, t->waves_z};
+  for (int v : seg)
+    if (v < 4 || v > (1 << 20)) return fail(SC_ERR_INVALID, "tuning: seg_* must be in [4, 2^20]");
+  for (int v : by)
+    if (v < 1 || v > 4) return fail(SC_ERR_INVALID, "tuning: by_* must be in [1, 4]");
+  for (int v : waves)
+    if (v != 0 && v != 4) return fail(SC_ERR_INVALID, "tuning: waves_* must be 0 or 4");
+  if (t->xcd_remap != 0 && t->xcd_remap != 1)
+    return fail(SC_ERR_INVALID, "tuning: xcd_remap must be 0 or 1");
+  out->seg_x = t->seg_x;
+  out->by_x = t->by_x;
+  out->waves_x = t->waves_x;
+  out->seg_z = t->seg_z;
+  out->by_z = t->by_z;
+  out->waves_z = t->waves_z;
+  out->by_y = t->by_y;
+  out->xcd_remap = t->xcd_remap;
+  return SC_OK;
+}
+
+// Per-(dim, side) device tables of the boundary face / flux values.
+sc_status build_boundary_tables(const sc_params& p, sc::BoundaryBc (*t)[2], int* dims) {
+  memset(t, 0, sizeof(sc::BoundaryBc) * 6);
+  *dims = 0;
+  if (p.n_boundary_values < 0 || p.n_boundary_values > SC_MAX_BOUNDARY_VALUES)
+    return fail(SC_ERR_INVALID, "n_boundary_values out of range");
+  for (int k = 0; k < p.n_boundary_values; ++k) {
+    const sc_boundary_value& v = p.boundary_values[k];
+    if (v.dim < 0 || v.dim > 2 || v.side < 0 || v.side > 1)
+      return fail(SC_ERR_INVALID, "boundary value: dim must be 0..2 and side 0..1");
+    sc::BoundaryBc& b = t[v.dim][v.side];
+    if (v.kind >= SC_BV_FLUX_CONVECTIVE && v.kind <= SC_BV_FLUX_TOTAL) {
+      if (v.var < 0 || v.var >= p.n_fields)
+        return fail(SC_ERR_INVALID, "flux boundary value: var must be a conserved field index");
+      b.flux_mask |= 1 << (8 * v.kind + v.var);
+      b.flux_val[v.kind][v.var] = v.value;
+    } else if (v.kind == SC_BV_FACE_CONSERVED || v.kind == SC_BV_FACE_PRIMITIVE) {
+      if (p.include_diffusion)
+        return fail(SC_ERR_UNSUPPORTED,
+                    "Face boundary conditions are not supported with diffusion enabled at"
+                    " this time.");
+      int var = v.var;
+      if (v.kind == SC_BV_FACE_CONSERVED) {
+        if (var == 0)
+          return fail(SC_ERR_UNSUPPORTED,
+                      "a density face value replaces the face primitives without p and h"
+                      " (boundary.py:284-293), which the flux then lacks");
+        if (var < 1 || var >= p.n_fields)
+          return fail(SC_ERR_INVALID, "face boundary value: invalid conserved field index");
+        if (var >= 5) var = 7 + (var - 5);  // scalar primitive slot
+      } else if (var < 1 || var >= 7 + (p.n_fields - 5)) {
+        return fail(SC_ERR_INVALID, "face boundary value: invalid primitive index");
+      }
+      if (b.n_face >= SC_MAX_FACE_VALUES)
+        return fail(SC_ERR_INVALID, "too many face boundary values on one boundary");
+      b.face[b.n_face++] = {v.kind == SC_BV_FACE_CONSERVED ? 1 : 2, var, v.value};
+    } else {
+      return fail(SC_ERR_INVALID, "boundary value: unknown kind");
+    }
+    *dims |= 1 << v.dim;
+  }
+  return SC_OK;
+}
+
+void from_tuning(const sc::SweepTuning& t, sc_tuning* out) {
+  out->seg_x = t.seg_x;
+  out->by_x = t.by_x;
+  out->waves_x = t.waves_x;
+  out->seg_z = t.seg_z;
+  out->by_z = t.by_z;
+  out->waves_z = t.waves_z;
+  out->by_y = t.by_y;
+  out->xcd_remap = t.xcd_remap;
 }
 
 }  // namespace
@@ -131,20 +208,18 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   }
   if (p.prim_bc_mask & 1)
     return fail(SC_ERR_INVALID, "the density BC cannot be given on a primitive");
+  sc::BoundaryBc bbc[3][2];
+  int bbc_dims = 0;
+  {
+    const sc_status st = build_boundary_tables(p, bbc, &bbc_dims);
+    if (st != SC_OK) return st;
+  }
   if (gdir != -1 && (grid->x_full == nullptr || grid->y_full == nullptr || grid->z_full == nullptr))
     return fail(SC_ERR_INVALID, "coordinates are required when the gravity vector is set");
 
   sc_ctx* c = new sc_ctx();
   memset(c, 0, sizeof(sc_ctx));
   c->tune = sc::SweepTuning();
-  c->tune.seg_len = env_int("SWIRLC_SEG_LEN", c->tune.seg_len);
-  c->tune.by_march = env_int("SWIRLC_BY_MARCH", c->tune.by_march);
-  c->tune.by_y = env_int("SWIRLC_BY_Y", c->tune.by_y);
-  // Blocks are 64 x by threads; the kernels are built for at most 256.
-  auto clamp = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
-  c->tune.by_march = clamp(c->tune.by_march, 1, 4);
-  c->tune.by_y = clamp(c->tune.by_y, 1, 4);
-  c->tune.seg_len = clamp(c->tune.seg_len, 4, 1 << 20);
   c->grid = *grid;
   c->params = p;
   c->nf = p.n_fields;
@@ -182,6 +257,13 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   for (int f = 0; f < c->nf && st == SC_OK; ++f)
     check(hipMalloc((void**)&c->rhs_scratch[f], sizeof(float) * c->cells_full), "hipMalloc rhs");
   if (st == SC_OK) check(hipMalloc((void**)&c->speed_bits, sizeof(unsigned) * 4), "hipMalloc speed");
+  c->bbc_dims = bbc_dims;
+  if (st == SC_OK && bbc_dims) {
+    check(hipMalloc((void**)&c->bbc_dev, sizeof(bbc)), "hipMalloc boundary values");
+    if (st == SC_OK)
+      check(hipMemcpy(c->bbc_dev, bbc, sizeof(bbc), hipMemcpyHostToDevice),
+            "hipMemcpy boundary values");
+  }
   if (st == SC_OK) {
     for (int f = 0; f < c->nf; ++f)
       check(hipMemset(c->rhs_scratch[f], 0, sizeof(float) * c->cells_full), "hipMemset rhs");
@@ -241,6 +323,8 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
     a.vc.h[d] = c->h[d];
     a.vc.inv_h[d] = 1.0f / c->h[d];
   }
+  a.bbc_dims = c->bbc_dims;
+  a.bbc = c->bbc_dev;
   a.vc.inv_two_h[0] = 1.0f / (float)(2.0 * grid->dx);
   a.vc.inv_two_h[1] = 1.0f / (float)(2.0 * grid->dy);
   a.vc.inv_two_h[2] = 1.0f / (float)(2.0 * grid->dz);
@@ -258,6 +342,7 @@ sc_status sc_ctx_destroy(sc_ctx* c) {
   for (int f = 0; f < SC_MAX_FIELDS; ++f)
     if (c->rhs_scratch[f]) hipFree(c->rhs_scratch[f]);
   if (c->speed_bits) hipFree(c->speed_bits);
+  if (c->bbc_dev) hipFree(c->bbc_dev);
   for (int f = 0; f < 5; ++f)
     if (c->adj_lq[f]) hipFree(c->adj_lq[f]);
   for (int f = 0; f < 9; ++f)
@@ -416,12 +501,28 @@ sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_
   return hip_check(e, "branch code kernel");
 }
 
+sc_status sc_ctx_get_tuning(const sc_ctx* c, sc_tuning* out) {
+  if (c == nullptr || out == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_get_tuning: null argument");
+  from_tuning(c->tune, out);
+  return SC_OK;
+}
+
+sc_status sc_ctx_set_tuning(sc_ctx* c, const sc_tuning* t) {
+  if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_set_tuning: null context");
+  sc::SweepTuning nt;
+  const sc_status st = to_tuning(t, &nt);
+  if (st == SC_OK) c->tune = nt;
+  return st;
+}
+
 sc_status sc_ctx_enable_adjoint(sc_ctx* c) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_enable_adjoint: null context");
   if (c->nf != 5)
     return fail(SC_ERR_UNSUPPORTED, "the adjoint supports the 5 base conserved fields only");
   if (c->params.prim_bc_mask != 0)
     return fail(SC_ERR_UNSUPPORTED, "the adjoint does not support primitive-variable BCs");
+  if (c->bbc_dims != 0)
+    return fail(SC_ERR_UNSUPPORTED, "the adjoint does not support face or flux boundary values");
   if (c->adj_lq[0] != nullptr) return SC_OK;
   sc_status st = SC_OK;
   const size_t bytes = sizeof(float) * c->cells_full;
@@ -594,6 +695,7 @@ struct sc_batch {
   sc::HaloArgs halo_cfg;
   sc::SweepArgs proto;
   sc::SweepTuning tune;
+  std::vector<sc_instance> inst_host;  // instance geometry, for rebuilding tiles
 };
 
 namespace {
@@ -612,9 +714,58 @@ sc_status upload(const std::vector<T>& v, T** dst) {
 
 int grid_for(int ntiles) { return ntiles < 2048 ? (ntiles > 0 ? ntiles : 1) : 2048; }
 
+// Sweep tile lists of a batch for its current tuning: x sweep (y chunks of
+// 64, z groups, x segments); y sweep (y chunks, x groups, z); z sweep (y
+// chunks, x groups, z segments).  Replaces any previous lists.
+sc_status build_sweep_tiles(sc_batch* b) {
+  const int rad = radius_of(b->params.interpolation);
+  const int outw = 64 - 2 * rad - 2;
+  const sc::SweepTuning& t = b->tune;
+  std::vector<sc::BatchTile> tl[3];
+  for (int i = 0; i < b->n_inst; ++i) {
+    const sc_instance& g = b->inst_host[i];
+    for (int bz = 0; bz < (g.nx + t.seg_x - 1) / t.seg_x; ++bz)
+      for (int by = 0; by < (g.nz + t.by_x - 1) / t.by_x; ++by)
+        for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[0].push_back({i, bx, by, bz});
+    for (int bz = 0; bz < g.nz; ++bz)
+      for (int by = 0; by < (g.nx + t.by_y - 1) / t.by_y; ++by)
+        for (int bx = 0; bx < (g.ny + outw - 1) / outw; ++bx) tl[1].push_back({i, bx, by, bz});
+    for (int bz = 0; bz < (g.nz + t.seg_z - 1) / t.seg_z; ++bz)
+      for (int by = 0; by < (g.nx + t.by_z - 1) / t.by_z; ++by)
+        for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[2].push_back({i, bx, by, bz});
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (b->tiles_dev[d]) hipFree(b->tiles_dev[d]);
+    b->tiles_dev[d] = nullptr;
+    const sc_status st = upload(tl[d], &b->tiles_dev[d]);
+    if (st != SC_OK) return st;
+    b->tiles.t[d] = b->tiles_dev[d];
+    b->tiles.n[d] = (int)tl[d].size();
+    b->tiles.grid[d] = grid_for(b->tiles.n[d]);
+  }
+  return SC_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
+
+sc_status sc_batch_get_tuning(const sc_batch* b, sc_tuning* out) {
+  if (b == nullptr || out == nullptr)
+    return fail(SC_ERR_INVALID, "sc_batch_get_tuning: null argument");
+  from_tuning(b->tune, out);
+  return SC_OK;
+}
+
+sc_status sc_batch_set_tuning(sc_batch* b, const sc_tuning* t) {
+  if (b == nullptr) return fail(SC_ERR_INVALID, "sc_batch_set_tuning: null batch");
+  sc::SweepTuning nt;
+  sc_status st = to_tuning(t, &nt);
+  if (st != SC_OK) return st;
+  b->tune = nt;
+  return build_sweep_tiles(b);
+}
 
 sc_status sc_batch_destroy(sc_batch* b) {
   if (b == nullptr) return SC_OK;
@@ -644,6 +795,8 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support a gravity vector");
   if (p.prim_bc_mask != 0)
     return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support primitive-variable BCs");
+  if (p.n_boundary_values != 0)
+    return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support face or flux boundary values");
   // Validate the shared configuration with a throwaway single-grid check of
   // every instance shape (schemes, halo width, BC types).
   long long end = 0;
@@ -690,20 +843,11 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   b->h = halo;
   b->n_inst = n;
   b->total = end;
-  b->tune.seg_len = env_int("SWIRLC_SEG_LEN", b->tune.seg_len);
-  b->tune.by_march = env_int("SWIRLC_BY_MARCH", b->tune.by_march);
-  b->tune.by_y = env_int("SWIRLC_BY_Y", b->tune.by_y);
-  auto clamp = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
-  b->tune.by_march = clamp(b->tune.by_march, 1, 4);
-  b->tune.by_y = clamp(b->tune.by_y, 1, 4);
-  b->tune.seg_len = clamp(b->tune.seg_len, 4, 1 << 20);
+  b->inst_host.assign(inst, inst + n);
 
-  const int rad = radius_of(p.interpolation);
-  const int outw = 64 - 2 * rad - 2;
   std::vector<sc::BatchInst> bi(n);
   std::vector<sc::BatchHaloInst> hi(n);
   std::vector<float> hmin(n);
-  std::vector<sc::BatchTile> tl[3];
   std::vector<sc::BatchHaloTile> ht[3], stl;
   for (int i = 0; i < n; ++i) {
     const sc_instance& g = inst[i];
@@ -726,18 +870,6 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     if (e.h[1] < hm) hm = e.h[1];
     if (e.h[2] < hm) hm = e.h[2];
     hmin[i] = hm;
-    const int seg = b->tune.seg_len, bym = b->tune.by_march, byy = b->tune.by_y;
-    // x sweep: (y chunks of 64, z groups, x segments); y sweep: (y chunks,
-    // x groups, z); z sweep: (y chunks, x groups, z segments).
-    for (int bz = 0; bz < (g.nx + seg - 1) / seg; ++bz)
-      for (int by = 0; by < (g.nz + bym - 1) / bym; ++by)
-        for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[0].push_back({i, bx, by, bz});
-    for (int bz = 0; bz < g.nz; ++bz)
-      for (int by = 0; by < (g.nx + byy - 1) / byy; ++by)
-        for (int bx = 0; bx < (g.ny + outw - 1) / outw; ++bx) tl[1].push_back({i, bx, by, bz});
-    for (int bz = 0; bz < (g.nz + seg - 1) / seg; ++bz)
-      for (int by = 0; by < (g.nx + bym - 1) / bym; ++by)
-        for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[2].push_back({i, bx, by, bz});
     const long long N[3] = {g.nx + 2LL * halo, g.ny + 2LL * halo, g.nz + 2LL * halo};
     const long long plane[3] = {N[2] * N[1], N[2] * N[0], N[0] * N[1]};
     for (int d = 0; d < 3; ++d) {
@@ -751,12 +883,9 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   sc_status st = upload(bi, &b->inst_dev);
   if (st == SC_OK) st = upload(hi, &b->hinst_dev);
   if (st == SC_OK) st = upload(hmin, &b->hmin_dev);
+  if (st == SC_OK) st = build_sweep_tiles(b);
   for (int d = 0; d < 3 && st == SC_OK; ++d) {
-    st = upload(tl[d], &b->tiles_dev[d]);
-    b->tiles.t[d] = b->tiles_dev[d];
-    b->tiles.n[d] = (int)tl[d].size();
-    b->tiles.grid[d] = grid_for(b->tiles.n[d]);
-    if (st == SC_OK) st = upload(ht[d], &b->htiles_dev[d]);
+    st = upload(ht[d], &b->htiles_dev[d]);
     b->nhtiles[d] = (int)ht[d].size();
   }
   if (st == SC_OK) st = upload(stl, &b->stiles_dev);

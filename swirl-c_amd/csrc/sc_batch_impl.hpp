@@ -70,8 +70,8 @@ __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int d
   return a;
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
-__global__ void __launch_bounds__(256) batch_march_kernel(const BatchArgs b) {
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, int MINB>
+__global__ void __launch_bounds__(256, MINB) batch_march_kernel(const BatchArgs b) {
   for (int t = (int)blockIdx.x; t < b.ntiles; t += (int)gridDim.x) {
     const BatchTile tl = b.tiles[t];
     const SweepArgs a = tile_args(b, tl.inst, DIR);
@@ -102,29 +102,38 @@ struct BatchTiles {
 template <int SCHEME, int FLUX, bool VISC, int NS>
 hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
                                  const SweepTuning& tune, hipStream_t stream) {
-  b.proto.seg_len = tune.seg_len;
-  const dim3 blk_m(64, tune.by_march);
-  const dim3 blk_y(64, tune.by_y);
   {
     BatchArgs bx = b;
+    bx.proto.seg_len = tune.seg_x;
     bx.tiles = tiles.t[0];
     bx.ntiles = tiles.n[0];
-    hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>),
-                       dim3(tiles.grid[0]), blk_m, 0, stream, bx);
+    if (tune.waves_x == 4) {
+      hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 4>),
+                         dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
+    } else {
+      hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 1>),
+                         dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
+    }
   }
   {
     BatchArgs bz = b;
+    bz.proto.seg_len = tune.seg_z;
     bz.tiles = tiles.t[2];
     bz.ntiles = tiles.n[2];
-    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1>),
-                       dim3(tiles.grid[2]), blk_m, 0, stream, bz);
+    if (tune.waves_z == 4) {
+      hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 4>),
+                         dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
+    } else {
+      hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 1>),
+                         dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
+    }
   }
   {
     BatchArgs by = b;
     by.tiles = tiles.t[1];
     by.ntiles = tiles.n[1];
     hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS>), dim3(tiles.grid[1]),
-                       blk_y, 0, stream, by);
+                       dim3(64, tune.by_y), 0, stream, by);
   }
   return hipGetLastError();
 }

@@ -25,26 +25,36 @@ template <int SCHEME, int FLUX, bool VISC, int NS>
 hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
                                const SweepTuning& tune, hipStream_t stream,
                                int only_dir) {
-  const dim3 blk_m(64, tune.by_march);
-  const dim3 blk_y(64, tune.by_y);
-  a.seg_len = tune.seg_len;
+  a.xcd_remap = tune.xcd_remap;
   // x sweep (first): R = src - div_x.
   if (only_dir < 0 || only_dir == 0) {
     SweepArgs ax = a;
     ax.inv_hdir = 1.0f / h[0];
-    const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_march - 1) / tune.by_march,
-                    (a.nx + tune.seg_len - 1) / tune.seg_len);
-    hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>), grid,
-                       blk_m, 0, stream, ax);
+    ax.seg_len = tune.seg_x;
+    const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_x - 1) / tune.by_x,
+                    (a.nx + tune.seg_x - 1) / tune.seg_x);
+    if (tune.waves_x == 4) {
+      hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 4>), grid,
+                         dim3(64, tune.by_x), 0, stream, ax);
+    } else {
+      hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 1>), grid,
+                         dim3(64, tune.by_x), 0, stream, ax);
+    }
   }
   // z sweep (middle): R -= div_z.
   if (only_dir < 0 || only_dir == 2) {
     SweepArgs az = a;
     az.inv_hdir = 1.0f / h[2];
-    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_march - 1) / tune.by_march,
-                    (a.nz + tune.seg_len - 1) / tune.seg_len);
-    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1>), grid,
-                       blk_m, 0, stream, az);
+    az.seg_len = tune.seg_z;
+    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
+                    (a.nz + tune.seg_z - 1) / tune.seg_z);
+    if (tune.waves_z == 4) {
+      hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 4>), grid,
+                         dim3(64, tune.by_z), 0, stream, az);
+    } else {
+      hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 1>), grid,
+                         dim3(64, tune.by_z), 0, stream, az);
+    }
   }
   // y sweep (last): R -= div_y, then store R or the RK update.
   if (only_dir < 0 || only_dir == 1) {
@@ -52,8 +62,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     ay.inv_hdir = 1.0f / h[1];
     constexpr int OUTW = YGeom<SCHEME>::OUTW;
     const dim3 grid((a.ny + OUTW - 1) / OUTW, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
-    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS>), grid, blk_y, 0,
-                       stream, ay);
+    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS>), grid, dim3(64, tune.by_y),
+                       0, stream, ay);
   }
   return hipGetLastError();
 }

@@ -31,11 +31,38 @@ struct CFields {
   const float* p[kMaxFields];
 };
 
-// Launch geometry of the sweeps (the tunable knobs).
+// Launch geometry of the sweeps (the tunable knobs, set per direction).
 struct SweepTuning {
-  int seg_len = 32;  // cells per thread along a marching sweep
-  int by_march = 4;  // waves per block, marching sweeps
-  int by_y = 4;      // waves per block, y sweep
+  int seg_x = 32;   // cells per thread, x sweep
+  int by_x = 4;     // waves per block, x sweep
+  int waves_x = 0;  // 4: x kernel built for 4 waves/SIMD (<= 128 VGPRs); 0: default
+  int seg_z = 32;
+  int by_z = 4;
+  int waves_z = 0;
+  int by_y = 4;     // waves per block, y sweep
+  int xcd_remap = 0;  // 1: XCD-aware block -> tile order (single-grid sweeps)
+};
+
+// Dirichlet values on the domain's boundary faces of one sweep direction
+// (boundary/boundary.py:89-183 intercell fluxes, :230-461 cell faces).
+constexpr int kMaxFaceEntries = 8;
+struct FaceEntry {
+  int kind;   // 1: conserved field (face primitive = value / rho), 2: primitive
+  int var;    // index into the reconstructed primitives (1 u .. 6 h, 7+s scalar)
+  float val;
+};
+struct BoundaryBc {
+  int n_face;                         // face-state entries, applied in order
+  FaceEntry face[kMaxFaceEntries];
+  int flux_mask;                      // bit 8*t + f: flux type t of field f set
+  float flux_val[3][kMaxFields];      // t: 0 convective, 1 diffusive, 2 total
+};
+
+// Block geometry of one tile: block indices (bx, by, bz) of the plain launch
+// and the thread's (tx, ty) with by_dim waves per block.
+struct TileIdx {
+  int bx, by, bz;
+  int tx, ty, by_dim;
 };
 
 struct SweepArgs {
@@ -60,7 +87,32 @@ struct SweepArgs {
   float c0, c1;
   const float* dt;
   ViscConst vc;
+  int bbc_dims;             // bit d: boundary face/flux values along dim d
+  const BoundaryBc* bbc;    // device table [dim][side]
+  int xcd_remap;            // XCD-aware block order (see tile_of_block)
 };
+
+// The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
+// XCD has its own L2.  With xcd_remap the linear block id is permuted so XCD k
+// runs the k-th contiguous range of tiles (x-fastest tile order): neighbouring
+// tiles, which share halo rows and transverse-neighbour planes, then hit the
+// same L2.  The map is a bijection on [0, nblocks).
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ TileIdx tile_of_block(int remap) {
+  int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
+  if (remap) {
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int n = gx * gy * (int)gridDim.z;
+    const int lin = bx + gx * (by + gy * bz);
+    const int q = n / kNumXcd, r = n % kNumXcd;
+    const int xcd = lin % kNumXcd, slot = lin / kNumXcd;
+    const int t = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
+  }
+  return TileIdx{bx, by, bz, (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+}
 
 __device__ __forceinline__ float pe_at(const SweepArgs& a, int k, int i, int j) {
   return a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
@@ -178,6 +230,61 @@ __device__ __forceinline__ float rk_update(const SweepArgs& a, float uk, float u
   return 0.5f * (u0 + (uk + dt * val));
 }
 
+// Face-state overwrite of boundary_face_update_1d on one side's reconstructed
+// primitives: a conserved value gives primitive = divide_no_nan(value, rho),
+// a primitive value is taken as is.
+template <int NS>
+__device__ __forceinline__ void apply_face_bc(const BoundaryBc& b, float* W) {
+  constexpr int NV = NVar<NS>::value;
+  for (int e = 0; e < b.n_face; ++e) {
+    const FaceEntry fe = b.face[e];
+    const float v = fe.kind == 1 ? dnn_exact(fe.val, W[kRho]) : fe.val;
+#pragma unroll
+    for (int k = 1; k < NV; ++k)
+      if (k == fe.var) W[k] = v;
+  }
+}
+
+template <int NF>
+__device__ __forceinline__ void apply_flux_bc(const BoundaryBc& b, int t, float* F) {
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+    if (b.flux_mask & (1 << (8 * t + f))) F[f] = b.flux_val[t][f];
+}
+
+// Total flux F of one face: Riemann flux of the face states L, R plus the
+// viscous and heat flux.  On a boundary face with Dirichlet values (b not
+// null) the face states are overwritten on both sides first, and the
+// convective flux, the diffusive flux (viscous runs only) and their sum are
+// overwritten in the reference's order (compressible_navier_stokes.py:147-234).
+template <int DIR, int FLUX, bool VISC, int NS>
+__device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* R, float pe,
+                                          const float* A, const float* B,
+                                          const float (*ta)[2], const float (*tb)[2],
+                                          const ViscConst& vc, float* F) {
+  constexpr int NF = 5 + NS;
+  if (b != nullptr) {
+    apply_face_bc<NS>(*b, L);
+    apply_face_bc<NS>(*b, R);
+  }
+  riemann<FLUX, DIR, NS>(L, R, pe, F);
+  if (b != nullptr) apply_flux_bc<NF>(*b, 0, F);
+  if constexpr (VISC) {
+    if (b != nullptr) {
+      float Fd[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Fd[f] = 0.0f;
+      viscous_flux<DIR, NS>(A, B, ta, tb, vc, Fd);
+      apply_flux_bc<NF>(*b, 1, Fd);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) F[f] = F[f] + Fd[f];
+    } else {
+      viscous_flux<DIR, NS>(A, B, ta, tb, vc, F);
+    }
+  }
+  if (b != nullptr) apply_flux_bc<NF>(*b, 2, F);
+}
+
 // ---------------------------------------------------------------------------
 // Marching sweep along x (DIR 0) or z (DIR 2).  Block (64, BY): lane -> y,
 // threadIdx.y -> the other transverse index, blockIdx.z -> segment of
@@ -185,12 +292,6 @@ __device__ __forceinline__ float rk_update(const SweepArgs& a, float uk, float u
 // PHASE 1: middle sweep (R -= div; the R of the next output cell is loaded
 // one iteration ahead).
 // ---------------------------------------------------------------------------
-// Block geometry of one tile: block indices (bx, by, bz) of the plain launch
-// and the thread's (tx, ty) with by_dim waves per block.
-struct TileIdx {
-  int bx, by, bz;
-  int tx, ty, by_dim;
-};
 
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileIdx& ti) {
@@ -268,10 +369,11 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
     if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
     if (c >= c_begin) {
       float F[NF];
-      riemann<FLUX, DIR, NS>(negPrev, posM, pe_at(a, kk(c), ii(c), j), F);
-      if constexpr (VISC) {
-        viscous_flux<DIR, NS>(W[RAD - 1], W[RAD], tPrev, tCur, a.vc, F);
-      }
+      const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
+      const BoundaryBc* bb =
+          (((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
+      face_flux<DIR, FLUX, VISC, NS>(bb, negPrev, posM, pe_at(a, kk(c), ii(c), j), W[RAD - 1],
+                                     W[RAD], tPrev, tCur, a.vc, F);
       if (c > c_begin) {
         const long long idx = base + (long long)(c - 1) * sm;
         if constexpr (PHASE == 0) {
@@ -304,11 +406,12 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   }
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
-__global__ void __launch_bounds__(256)
+// MINB = 4 asks the compiler for 4 resident blocks (16 waves) per CU, i.e.
+// at most 128 VGPRs per lane; MINB = 1 leaves the register budget open.
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, int MINB>
+__global__ void __launch_bounds__(256, MINB)
 sweep_march_kernel(const SweepArgs a) {
-  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
-                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  const TileIdx ti = tile_of_block(a.xcd_remap);
   sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE>(a, ti);
 }
 
@@ -372,17 +475,17 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
 #pragma unroll
   for (int v = 0; v < NV; ++v) negPrev[v] = __shfl(negP[v], lane - 1, 64);
   float F[NF];
-  riemann<FLUX, 1, NS>(negPrev, posM, pe_at(a, k, i, yc), F);
-  if constexpr (VISC) {
-    float tCur[2][2], tPrev[2][2];
-    transverse_grads<1>(a, k, i, yc, tCur);
+  float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, tPrev[2][2];
+  if constexpr (VISC) transverse_grads<1>(a, k, i, yc, tCur);
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      tPrev[t][0] = __shfl(tCur[t][0], lane - 1, 64);
-      tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
-    }
-    viscous_flux<1, NS>(Am, W0, tPrev, tCur, a.vc, F);
+  for (int t = 0; t < 2; ++t) {
+    tPrev[t][0] = __shfl(tCur[t][0], lane - 1, 64);
+    tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
   }
+  const int side = y == a.h ? 0 : (y == a.h + a.ny ? 1 : -1);
+  const BoundaryBc* bb = (((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
+  face_flux<1, FLUX, VISC, NS>(bb, negPrev, posM, pe_at(a, k, i, yc), Am, W0, tPrev, tCur,
+                               a.vc, F);
   float Fn[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) Fn[f] = __shfl(F[f], lane + 1, 64);
@@ -402,8 +505,7 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
 template <int SCHEME, int FLUX, bool VISC, int NS>
 __global__ void __launch_bounds__(256)
 sweep_y_kernel(const SweepArgs a) {
-  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
-                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  const TileIdx ti = tile_of_block(a.xcd_remap);
   sweep_y_body<SCHEME, FLUX, VISC, NS>(a, ti);
 }
 
@@ -447,6 +549,11 @@ branch_codes_kernel(const SweepArgs a, uint8_t* codes) {
     }
     recon<SCHEME>(sl, Wl[v], dummy);
     recon<SCHEME>(sr, dummy, Wr[v]);
+  }
+  const int side = pc == a.h ? 0 : (pc == a.h + n ? 1 : -1);
+  if (((a.bbc_dims >> D) & 1) && side >= 0) {
+    apply_face_bc<NS>(a.bbc[2 * D + side], Wl);
+    apply_face_bc<NS>(a.bbc[2 * D + side], Wr);
   }
   float F[5 + NS];
   codes[c] = (uint8_t)hllc_flux<D, NS>(Wl, Wr, pe_at(a, k, i, j), F);

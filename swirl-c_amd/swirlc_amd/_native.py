@@ -28,6 +28,7 @@ EXPORTED = (
     'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes', 'sc_sweep_only',
     'sc_batch_create', 'sc_batch_destroy', 'sc_batch_fill_halos', 'sc_batch_rk_stage',
     'sc_batch_max_wave_speed', 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
+    'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning', 'sc_batch_set_tuning',
     'sc_last_error', 'sc_version',
 )
 
@@ -53,6 +54,19 @@ class ScGrid(ctypes.Structure):
   ]
 
 
+SC_MAX_BOUNDARY_VALUES = 64
+SC_MAX_FACE_VALUES = 8
+(SC_BV_FLUX_CONVECTIVE, SC_BV_FLUX_DIFFUSIVE, SC_BV_FLUX_TOTAL, SC_BV_FACE_CONSERVED,
+ SC_BV_FACE_PRIMITIVE) = range(5)
+
+
+class ScBoundaryValue(ctypes.Structure):
+  _fields_ = [
+      ('kind', ctypes.c_int), ('dim', ctypes.c_int), ('side', ctypes.c_int),
+      ('var', ctypes.c_int), ('value', ctypes.c_float),
+  ]
+
+
 class ScParams(ctypes.Structure):
   _fields_ = [
       ('n_fields', ctypes.c_int),
@@ -64,6 +78,8 @@ class ScParams(ctypes.Structure):
       ('g', ctypes.c_double * 3),
       ('bc', ((ScBcFace * 2) * 3) * SC_MAX_FIELDS),
       ('prim_bc_mask', ctypes.c_int),
+      ('n_boundary_values', ctypes.c_int),
+      ('boundary_values', ScBoundaryValue * SC_MAX_BOUNDARY_VALUES),
   ]
 
 
@@ -73,6 +89,24 @@ class ScInstance(ctypes.Structure):
       ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('dz', ctypes.c_double),
       ('offset', ctypes.c_longlong),
   ]
+
+
+class ScTuning(ctypes.Structure):
+  _fields_ = [
+      ('seg_x', ctypes.c_int), ('by_x', ctypes.c_int), ('waves_x', ctypes.c_int),
+      ('seg_z', ctypes.c_int), ('by_z', ctypes.c_int), ('waves_z', ctypes.c_int),
+      ('by_y', ctypes.c_int), ('xcd_remap', ctypes.c_int),
+  ]
+
+  def as_dict(self):
+    return {name: getattr(self, name) for name, _ in self._fields_}
+
+  @classmethod
+  def from_dict(cls, d):
+    t = cls()
+    for name, _ in cls._fields_:
+      setattr(t, name, int(d[name]))
+    return t
 
 
 class NativeError(RuntimeError):
@@ -108,6 +142,10 @@ def lib():
     L.sc_batch_fill_halos.argtypes = [vp, fpp, vp]
     L.sc_batch_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_batch_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
+    L.sc_ctx_get_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
+    L.sc_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
+    L.sc_batch_get_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
+    L.sc_batch_set_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
     L.sc_ctx_enable_adjoint.argtypes = [vp]
     L.sc_rhs_vjp.argtypes = [vp, fpp, fpp, fpp, vp]
     L.sc_rk_stage_vjp.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, fpp, vp]
@@ -117,7 +155,9 @@ def lib():
                  'sc_rk_stage', 'sc_max_wave_speed', 'sc_hllc_branch_codes',
                  'sc_sweep_only', 'sc_batch_create', 'sc_batch_destroy',
                  'sc_batch_fill_halos', 'sc_batch_rk_stage', 'sc_batch_max_wave_speed',
-                 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp'):
+                 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
+                 'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning',
+                 'sc_batch_set_tuning'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

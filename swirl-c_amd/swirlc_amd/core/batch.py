@@ -104,6 +104,18 @@ class BatchSolver:
     except Exception:  # interpreter shutdown
       pass
 
+  def get_tuning(self):
+    """Launch geometry of the batched sweeps (dict of the sc_tuning fields)."""
+    t = N.ScTuning()
+    N.check(N.lib().sc_batch_get_tuning(self._b, ctypes.byref(t)), 'sc_batch_get_tuning')
+    return t.as_dict()
+
+  def set_tuning(self, tuning):
+    """Sets the launch geometry (rebuilds the tile lists; drops a captured step)."""
+    t = N.ScTuning.from_dict(tuning)
+    N.check(N.lib().sc_batch_set_tuning(self._b, ctypes.byref(t)), 'sc_batch_set_tuning')
+    self._graph = None
+
   def _stage(self, form, c0, c1, u0, uk, out):
     N.check(N.lib().sc_batch_rk_stage(
         self._b, form, ctypes.c_float(c0), ctypes.c_float(c1),

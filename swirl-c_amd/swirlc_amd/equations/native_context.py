@@ -71,14 +71,10 @@ class NativeContext:
           'multi-replica domain decomposition is not supported: one GPU runs'
           ' one replica (cx = cy = cz = 1)')
     bc = _get(cfg, 'bc')
-    if 'cell_faces' in bc:
-      if _get(cfg, 'include_diffusion'):
-        raise NotImplementedError(
-            'Face boundary conditions are not supported with diffusion enabled'
-            ' at this time.')
-      raise NotImplementedError("'cell_faces' boundary conditions are not supported yet")
-    if 'intercell_fluxes' in bc:
-      raise NotImplementedError("'intercell_fluxes' boundary conditions are not supported yet")
+    if 'cell_faces' in bc and _get(cfg, 'include_diffusion'):
+      raise NotImplementedError(
+          'Face boundary conditions are not supported with diffusion enabled'
+          ' at this time.')
 
     self.names = list(_get(cfg, 'conservative_variable_names'))
     if self.names[:5] != list(types.BASE_CONSERVATIVE):
@@ -86,6 +82,15 @@ class NativeContext:
           f'conservative_variable_names must start with {list(types.BASE_CONSERVATIVE)}')
     if len(self.names) > 6:
       raise NotImplementedError('at most one transported scalar is supported')
+    # The reference rebuilds face conserved states from the primitives listed
+    # in primitive_variable_names (compressible_navier_stokes.py:131-144), so a
+    # scalar whose primitive is not listed has no face state there.
+    prims = list(_get(cfg, 'primitive_variable_names', types.BASE_PRIMITIVES))
+    for name in self.names[5:]:
+      if types.conservative_to_primitive_name(name) not in prims:
+        raise ValueError(
+            f'transported scalar {name} needs its primitive '
+            f'{types.conservative_to_primitive_name(name)} in primitive_variable_names')
     self.nf = len(self.names)
     self.hw = int(_get(cfg, 'halo_width'))
     self.n = (int(_get(cfg, 'core_nx')), int(_get(cfg, 'core_ny')), int(_get(cfg, 'core_nz')))
@@ -142,6 +147,7 @@ class NativeContext:
         self._fill_dim_bc(params.bc[f][d], spec[dim], d)
     params.prim_bc_mask = mask
     self.prim_bc_mask = mask
+    self._fill_boundary_values(params, bc)
 
     if not create:
       return
@@ -153,6 +159,71 @@ class NativeContext:
     self._ctx = handle
 
   # -- BC translation ------------------------------------------------------
+
+  @staticmethod
+  def _dirichlet_along(spec, dim, side):
+    """boundary/boundary.py:186-227: (var -> value) of the DIRICHLET entries
+    on one boundary; `None` types are skipped."""
+    out = {}
+    for var, d in spec.items():
+      t, val = d[dim][side]
+      if t is None:
+        continue
+      if t != bc_types.BoundaryCondition.DIRICHLET:
+        raise AssertionError(
+            f'Unsupported interface (face or flux) BC type {t}. Only "DIRICHLET"'
+            ' or `None` is supported.')
+      if not isinstance(val, (int, float, np.floating, np.integer)):
+        raise NotImplementedError('face and flux boundary values must be scalars')
+      out[var] = float(val)
+    return out
+
+  def _fill_boundary_values(self, params, bc):
+    """cfg.bc['intercell_fluxes'] and cfg.bc['cell_faces'] -> the C table."""
+    entries = []
+    kinds = {'convective': N.SC_BV_FLUX_CONVECTIVE, 'diffusive': N.SC_BV_FLUX_DIFFUSIVE,
+             'total': N.SC_BV_FLUX_TOTAL}
+    for ftype, spec in (bc.get('intercell_fluxes') or {}).items():
+      if ftype not in kinds:
+        raise ValueError(f'unknown flux boundary type {ftype}')
+      bad = set(spec) - set(self.names)
+      if bad:
+        raise AssertionError(
+            f'{bad} cannot be specified by a flux boundary condition. Only conservative'
+            f' variables {set(self.names)} can be specified with a flux boundary'
+            ' condition.')
+      for d, dim in enumerate(types.DIMS):
+        for side in (0, 1):
+          for var, val in self._dirichlet_along(spec, dim, side).items():
+            entries.append((kinds[ftype], d, side, self.names.index(var), val))
+    prim_index = {'u': 1, 'v': 2, 'w': 3, 'e': 4, 'p': 5, 'h': 6}
+    for s, name in enumerate(self.names[5:]):
+      prim_index[types.conservative_to_primitive_name(name)] = 7 + s
+    faces = bc.get('cell_faces') or {}
+    for d, dim in enumerate(types.DIMS):
+      for side in (0, 1):
+        spec = self._dirichlet_along(faces, dim, side)
+        if 'rho' in spec:
+          # boundary.py:284-293 replaces the face primitives by
+          # conservative_to_primitive_variables (no p, h), so the reference's
+          # flux step fails on the missing pressure.
+          raise KeyError('p')
+        for var, val in spec.items():
+          if var in self.names:
+            entries.append((N.SC_BV_FACE_CONSERVED, d, side, self.names.index(var), val))
+          elif var in prim_index:
+            entries.append((N.SC_BV_FACE_PRIMITIVE, d, side, prim_index[var], val))
+          else:
+            raise ValueError(
+                f'"{var}" is not a valid primitive or conservative variable name for the'
+                ' face boundary condition update.')
+    if len(entries) > N.SC_MAX_BOUNDARY_VALUES:
+      raise NotImplementedError(
+          f'at most {N.SC_MAX_BOUNDARY_VALUES} face / flux boundary values are supported')
+    params.n_boundary_values = len(entries)
+    for k, (kind, d, side, var, val) in enumerate(entries):
+      e = params.boundary_values[k]
+      e.kind, e.dim, e.side, e.var, e.value = kind, d, side, var, val
 
   def _plane_shape(self, d):
     nz, nx, ny = self.shape
@@ -268,6 +339,18 @@ class NativeContext:
     N.check(N.lib().sc_hllc_branch_codes(
         self._ctx, N.ptr_array(q), int(dim), ctypes.c_void_p(codes.data_ptr()),
         N.stream_handle(stream)), 'sc_hllc_branch_codes')
+
+  # ---- Launch geometry -----------------------------------------------------
+
+  def get_tuning(self):
+    """The sweep kernels' launch geometry (dict of the sc_tuning fields)."""
+    t = N.ScTuning()
+    N.check(N.lib().sc_ctx_get_tuning(self._ctx, ctypes.byref(t)), 'sc_ctx_get_tuning')
+    return t.as_dict()
+
+  def set_tuning(self, tuning):
+    t = N.ScTuning.from_dict(tuning)
+    N.check(N.lib().sc_ctx_set_tuning(self._ctx, ctypes.byref(t)), 'sc_ctx_set_tuning')
 
   # ---- Adjoint -------------------------------------------------------------
 

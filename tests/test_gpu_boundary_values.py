@@ -1,0 +1,112 @@
+"""Face and flux Dirichlet values on the domain boundary (cfg.bc['cell_faces']
+and cfg.bc['intercell_fluxes'], boundary/boundary.py:89-461) through the GPU
+RHS, against the fp32 oracle (relative L-infinity <= 1e-5)."""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import swirlc_oracle as O
+from swirlc_amd.boundary import bc_types
+from swirlc_amd.core.parameter import SwirlCParameters
+from swirlc_amd.equations import compressible_navier_stokes as cns
+
+from helpers import grid_cfg, interior, oracle_cfg, random_state, rel_linf
+
+pytestmark = pytest.mark.gpu
+BC = bc_types.BoundaryCondition
+D, NONE = BC.DIRICHLET, (None, None)
+NAMES = ('rho', 'rho_u', 'rho_v', 'rho_w', 'rho_e')
+TOL = 1e-5
+
+
+def _cell_bcs():
+  wall = {'rho': 1.0, 'rho_u': 0.0, 'rho_v': 0.05, 'rho_w': 0.0, 'rho_e': 6.0}
+  return {k: {'x': {0: (BC.NEUMANN, 0.0), 1: (BC.DIRICHLET, wall[k])},
+              'y': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
+              'z': {0: (BC.DIRICHLET, wall[k]), 1: (BC.NEUMANN, 0.0)}} for k in NAMES}
+
+
+def _cfg(n, scheme, flux, visc, faces=None, fluxes=None):
+  cfg = grid_cfg(n, 3)
+  bc = {'cell_averages': _cell_bcs()}
+  if faces is not None:
+    bc['cell_faces'] = faces
+  if fluxes is not None:
+    bc['intercell_fluxes'] = fluxes
+  cfg.update({'interpolation_scheme': scheme, 'numeric_flux_scheme': flux,
+              'include_diffusion': visc, 'transport_parameters': {'nu': 2e-3, 'pr': 0.7},
+              'dt': 2e-3, 'bc': bc})
+  return cfg
+
+
+def _flux_spec():
+  return {
+      'convective': {
+          'rho': {'x': {0: (D, 0.0), 1: NONE}, 'y': {0: NONE, 1: NONE},
+                  'z': {0: (D, 0.1), 1: (D, -0.2)}},
+          'rho_w': {'x': {0: NONE, 1: NONE}, 'y': {0: NONE, 1: NONE},
+                    'z': {0: (D, 1.0), 1: NONE}},
+      },
+      'diffusive': {
+          'rho_e': {'x': {0: (D, 0.5), 1: (D, -0.5)}, 'y': {0: NONE, 1: NONE},
+                    'z': {0: NONE, 1: (D, 0.25)}},
+      },
+      'total': {
+          'rho_u': {'x': {0: NONE, 1: (D, 0.3)}, 'y': {0: (D, 0.0), 1: NONE},
+                    'z': {0: NONE, 1: NONE}},
+      },
+  }
+
+
+def _face_spec():
+  spec = {}
+  spec['rho_u'] = {'x': {0: (D, 0.0), 1: (D, 0.0)}, 'y': {0: NONE, 1: NONE},
+                   'z': {0: NONE, 1: NONE}}
+  spec['v'] = {'x': {0: NONE, 1: NONE}, 'y': {0: (D, 0.2), 1: NONE},
+               'z': {0: (D, -0.1), 1: NONE}}
+  spec['p'] = {'x': {0: (D, 1.5), 1: NONE}, 'y': {0: NONE, 1: NONE},
+               'z': {0: NONE, 1: (D, 0.8)}}
+  spec['rho_e'] = {'x': {0: NONE, 1: (D, 4.0)}, 'y': {0: NONE, 1: NONE},
+                   'z': {0: NONE, 1: NONE}}
+  return spec
+
+
+def _check(cfg, seed, tag):
+  n, h = (cfg['nx'], cfg['ny'], cfg['nz']), 3
+  st = random_state(n, h, seed=seed)
+  ref = O.rhs(st, {}, oracle_cfg(cfg))
+  fn = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
+  out = fn(0, {k: torch.from_numpy(v.copy()).cuda() for k, v in st.items()}, {})
+  torch.cuda.synchronize()
+  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in NAMES}
+  print(f'[bvalues] {tag}: ' + ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
+  assert max(errs.values()) <= TOL, errs
+
+
+@pytest.mark.parametrize('scheme,flux,visc', [('WENO_5', 'HLLC', True), ('MUSCL', 'HLL', True),
+                                              ('WENO_3', 'HLLC', False)])
+def test_flux_values_match_oracle(scheme, flux, visc):
+  cfg = _cfg((12, 10, 9), scheme, flux, visc, fluxes=_flux_spec())
+  _check(cfg, 31, f'fluxes {scheme}/{flux}/visc={visc}')
+
+
+@pytest.mark.parametrize('scheme,flux', [('WENO_5', 'HLLC'), ('WENO_5', 'HLL'), ('MUSCL', 'HLLC')])
+def test_face_values_match_oracle(scheme, flux):
+  cfg = _cfg((12, 10, 9), scheme, flux, False, faces=_face_spec(), fluxes=_flux_spec())
+  _check(cfg, 32, f'faces {scheme}/{flux}')
+
+
+def test_face_value_errors_follow_the_reference():
+  faces = _face_spec()
+  with pytest.raises(NotImplementedError):
+    cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(
+        _cfg((8, 8, 8), 'WENO_5', 'HLLC', True, faces=faces))))
+  faces['rho'] = {'x': {0: (D, 1.0), 1: NONE}, 'y': {0: NONE, 1: NONE}, 'z': {0: NONE, 1: NONE}}
+  cfg = _cfg((8, 8, 8), 'WENO_5', 'HLLC', False, faces=faces)
+  with pytest.raises(KeyError):
+    cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
+  with pytest.raises(KeyError):
+    O.rhs(random_state((8, 8, 8), 3, seed=1), {}, oracle_cfg(cfg))

@@ -1,0 +1,93 @@
+"""Launch-geometry tuning changes timings only: every tuning gives bit-identical
+results (single grid and batched), and the autotuner returns a valid one."""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from swirlc_amd import _native as N
+from swirlc_amd import configs
+from swirlc_amd.core import autotune
+from swirlc_amd.core.batch import BatchSolver
+from swirlc_amd.core.parameter import SwirlCParameters
+from swirlc_amd.equations import compressible_navier_stokes as cns
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [
+    dict(seg_x=16, by_x=2, waves_x=4, seg_z=64, by_z=1, waves_z=0, by_y=2, xcd_remap=1),
+    dict(seg_x=64, by_x=4, waves_x=0, seg_z=16, by_z=4, waves_z=4, by_y=1, xcd_remap=0),
+    dict(seg_x=5, by_x=3, waves_x=4, seg_z=7, by_z=3, waves_z=4, by_y=3, xcd_remap=1),
+]
+
+
+def _stage(ctx, q, dt):
+  out = [torch.zeros_like(t) for t in q]
+  ctx.rk_stage(N.SC_RK_BLEND, 0.75, 0.25, dt, q, q, out)
+  torch.cuda.synchronize()
+  return out
+
+
+def test_tuning_is_bitwise_neutral_single_grid():
+  cfg_d, st = configs.kh(n=40)
+  cfg_d = copy.deepcopy(cfg_d)
+  ctx = cns.get_rhs_fn(None, SwirlCParameters(cfg_d)).native_context
+  q = [torch.from_numpy(st[k]).cuda() for k in ctx.names]
+  dt = torch.full((1,), 1e-3, device='cuda')
+  base = ctx.get_tuning()
+  ref = _stage(ctx, q, dt)
+  for t in TUNINGS:
+    print(f'[tuning] single grid {t}', flush=True)
+    ctx.set_tuning(t)
+    assert ctx.get_tuning() == t
+    got = _stage(ctx, q, dt)
+    for a, b in zip(got, ref):
+      assert torch.equal(a, b)
+  ctx.set_tuning(base)
+  with pytest.raises(N.NativeError):
+    ctx.set_tuning(dict(base, by_y=8))
+  with pytest.raises(N.NativeError):
+    ctx.set_tuning(dict(base, waves_x=2))
+
+
+def test_autotuner_picks_a_valid_geometry():
+  cfg_d, st = configs.kh(n=48)
+  ctx = cns.get_rhs_fn(None, SwirlCParameters(cfg_d)).native_context
+  q = [torch.from_numpy(st[k]).cuda() for k in ctx.names]
+  out = [torch.zeros_like(t) for t in q]
+  dt = torch.full((1,), 1e-3, device='cuda')
+  best, timings = autotune.tune_context(ctx, q, out, dt, reps=2)
+  print(f'[tuning] autotuned {best}', flush=True)
+  assert ctx.get_tuning() == best
+  assert len(timings) == 2 * len(autotune.MARCH_SPACE) + len(autotune.Y_SPACE) + 2
+  assert all(ms > 0 for _, _, ms in timings)
+
+
+def test_tuning_is_bitwise_neutral_batched():
+  cfg, members = configs.riemann_ensemble(6, seed=3, shapes=((20, 18, 16), (33, 24, 17)))
+  batch = BatchSolver(SwirlCParameters(dict(cfg)), members, cfl=0.5, use_graph=False)
+  batch.step(1)
+  torch.cuda.synchronize()
+  start = [t.clone() for t in batch.u]
+
+  def run():
+    for t, s in zip(batch.u, start):
+      t.copy_(s)
+    batch.step(2)
+    torch.cuda.synchronize()
+    return [t.clone() for t in batch.u]
+
+  ref = run()
+  for t in TUNINGS:
+    print(f'[tuning] batched {t}', flush=True)
+    batch.set_tuning(t)
+    got = run()
+    for a, b in zip(got, ref):
+      assert torch.equal(a, b)
+  best, _ = autotune.tune_batch(batch, reps=1)
+  assert batch.get_tuning() == best
+  got = run()
+  for a, b in zip(got, ref):
+    assert torch.equal(a, b)
