@@ -52,6 +52,8 @@ def parse():
   ap.add_argument('--adjoint-steps', type=int, default=200)
   ap.add_argument('--adjoint-every', type=int, default=20)
   ap.add_argument('--no-adjoint', action='store_true')
+  ap.add_argument('--no-tune', action='store_true',
+                  help='keep the default launch geometry (no autotuning)')
   ap.add_argument('--pmc-probe', action='store_true',
                   help='launch a 256 MiB copy kernel, then each sweep kernel twice on the '
                        'state, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes')
@@ -276,7 +278,10 @@ def main():
   solver = Solver(cfg, state, cfl=0.5, use_graph=True)
   del state
   from swirlc_amd.core import autotune
-  tuning, _ = autotune.tune_context(solver.ctx, solver.u, solver.s2, solver.dt, u0=solver.s1)
+  if args.no_tune:
+    tuning = solver.ctx.get_tuning()
+  else:
+    tuning, _ = autotune.tune_context(solver.ctx, solver.u, solver.s2, solver.dt, u0=solver.s1)
   print(f'[bench] KH {n}^3 state on GPU; tuning {tuning}; warmup {args.warmup} steps',
         file=sys.stderr, flush=True)
   solver.step(args.warmup)

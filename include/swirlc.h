@@ -157,6 +157,16 @@ sc_status sc_rk_stage(sc_ctx* ctx, int form, float c0, float c1,
                       const float* dt_dev, const float* const* u0,
                       float* const* uk, float* const* out, void* stream);
 
+/* Hydrostatic reference state (helper_var 'p_ref' and 'rho_ref',
+ * equations/compressible_navier_stokes.py:273-335): full-size device fields
+ * used by the following sc_rhs / sc_rk_stage / sc_sweep_only /
+ * sc_hllc_branch_codes calls until cleared with two NULLs.  p_dyn = p - p_ref
+ * (zero in every halo cell) is reconstructed to the faces and replaces p in
+ * the momentum flux (and, for HLLC, in the wave speeds and star states,
+ * hllc.py:109-120); d_rho = rho - rho_ref replaces rho in the gravity source.
+ * The caller keeps the fields alive.  5 fields only. */
+sc_status sc_ctx_set_reference_state(sc_ctx* ctx, const float* p_ref, const float* rho_ref);
+
 /* Maximum over interior cells of max_d(|u_d| + sqrt(gamma p / rho)) written to
  * *out_dev, and dt = cfl * min(dx,dy,dz) / max written to *dt_dev (if not
  * NULL).  No reference counterpart (the reference takes a fixed cfg.dt); the
@@ -184,13 +194,12 @@ sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
 
 /* Launch geometry of the sweep kernels (no reference counterpart: the
  * reference leaves scheduling to XLA).  seg_*: cells per thread of a marching
- * sweep (>= 4); by_*: 64-lane waves per block (1..4); waves_*: 4 selects the
- * build of the marching kernel limited to 128 VGPRs (4 waves per SIMD), 0 the
- * default build.  Chosen per grid by swirlc_amd.core.autotune, or set by the
- * caller; results do not depend on it (bit for bit). */
+ * sweep (>= 4); by_*: 64-lane waves per block (1..4).  Chosen per grid by
+ * swirlc_amd.core.autotune, or set by the caller; results do not depend on
+ * it (bit for bit). */
 typedef struct sc_tuning {
-  int seg_x, by_x, waves_x;
-  int seg_z, by_z, waves_z;
+  int seg_x, by_x;
+  int seg_z, by_z;
   int by_y;
   int xcd_remap;            /* 1: XCD-aware block order (single-grid sweeps) */
 } sc_tuning;

@@ -79,21 +79,16 @@ sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
   if (t == nullptr) return fail(SC_ERR_INVALID, "null tuning");
   const int seg[2] = {t->seg_x, t->seg_z};
   const int by[3] = {t->by_x, t->by_z, t->by_y};
-  const int waves[2] = {t->waves_x, t->waves_z};
   for (int v : seg)
     if (v < 4 || v > (1 << 20)) return fail(SC_ERR_INVALID, "tuning: seg_* must be in [4, 2^20]");
   for (int v : by)
     if (v < 1 || v > 4) return fail(SC_ERR_INVALID, "tuning: by_* must be in [1, 4]");
-  for (int v : waves)
-    if (v != 0 && v != 4) return fail(SC_ERR_INVALID, "tuning: waves_* must be 0 or 4");
   if (t->xcd_remap != 0 && t->xcd_remap != 1)
     return fail(SC_ERR_INVALID, "tuning: xcd_remap must be 0 or 1");
   out->seg_x = t->seg_x;
   out->by_x = t->by_x;
-  out->waves_x = t->waves_x;
   out->seg_z = t->seg_z;
   out->by_z = t->by_z;
-  out->waves_z = t->waves_z;
   out->by_y = t->by_y;
   out->xcd_remap = t->xcd_remap;
   return SC_OK;
@@ -146,10 +141,8 @@ sc_status build_boundary_tables(const sc_params& p, sc::BoundaryBc (*t)[2], int*
 void from_tuning(const sc::SweepTuning& t, sc_tuning* out) {
   out->seg_x = t.seg_x;
   out->by_x = t.by_x;
-  out->waves_x = t.waves_x;
   out->seg_z = t.seg_z;
   out->by_z = t.by_z;
-  out->waves_z = t.waves_z;
   out->by_y = t.by_y;
   out->xcd_remap = t.xcd_remap;
 }
@@ -501,6 +494,17 @@ sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_
   return hip_check(e, "branch code kernel");
 }
 
+sc_status sc_ctx_set_reference_state(sc_ctx* c, const float* p_ref, const float* rho_ref) {
+  if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_set_reference_state: null context");
+  if ((p_ref == nullptr) != (rho_ref == nullptr))
+    return fail(SC_ERR_INVALID, "sc_ctx_set_reference_state: give both p_ref and rho_ref, or neither");
+  if (p_ref != nullptr && c->nf != 5)
+    return fail(SC_ERR_UNSUPPORTED, "a reference state is supported without transported scalars only");
+  c->sweep.p_ref = p_ref;
+  c->sweep.rho_ref = rho_ref;
+  return SC_OK;
+}
+
 sc_status sc_ctx_get_tuning(const sc_ctx* c, sc_tuning* out) {
   if (c == nullptr || out == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_get_tuning: null argument");
   from_tuning(c->tune, out);
@@ -598,6 +602,8 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
 sc_status check_adjoint_ready(sc_ctx* c, const char* who) {
   if (c->adj_lq[0] == nullptr)
     return fail(SC_ERR_INVALID, std::string(who) + ": call sc_ctx_enable_adjoint first");
+  if (c->sweep.p_ref != nullptr)
+    return fail(SC_ERR_UNSUPPORTED, std::string(who) + ": no adjoint with a reference state");
   return SC_OK;
 }
 

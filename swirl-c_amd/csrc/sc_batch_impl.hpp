@@ -70,8 +70,8 @@ __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int d
   return a;
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, int MINB>
-__global__ void __launch_bounds__(256, MINB) batch_march_kernel(const BatchArgs b) {
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
+__global__ void __launch_bounds__(256) batch_march_kernel(const BatchArgs b) {
   for (int t = (int)blockIdx.x; t < b.ntiles; t += (int)gridDim.x) {
     const BatchTile tl = b.tiles[t];
     const SweepArgs a = tile_args(b, tl.inst, DIR);
@@ -107,26 +107,16 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
     bx.proto.seg_len = tune.seg_x;
     bx.tiles = tiles.t[0];
     bx.ntiles = tiles.n[0];
-    if (tune.waves_x == 4) {
-      hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 4>),
-                         dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
-    } else {
-      hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 1>),
-                         dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
-    }
+    hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>),
+                       dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
   }
   {
     BatchArgs bz = b;
     bz.proto.seg_len = tune.seg_z;
     bz.tiles = tiles.t[2];
     bz.ntiles = tiles.n[2];
-    if (tune.waves_z == 4) {
-      hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 4>),
-                         dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
-    } else {
-      hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 1>),
-                         dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
-    }
+    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1>),
+                       dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
   }
   {
     BatchArgs by = b;

@@ -196,13 +196,17 @@ __device__ __forceinline__ void recon<kMuscl>(const float* a, float& fp, float& 
 
 // Physical flux of one side: equations/compressible_navier_stokes.py:338-427,
 // with the face conserved momentum m = rho_f * u_f (utils.py:186-216).
-template <int D, int NS>
+// PD: the face state carries the dynamic pressure p - p_ref at index 7 + NS,
+// which replaces p in the aligned momentum flux (helper_var 'p_dyn',
+// compressible_navier_stokes.py:366-372).
+template <int D, int NS, bool PD = false>
 __device__ __forceinline__ void physical_flux(const float* s, float* f) {
   const float m = s[kRho] * s[kU + D];
+  const float pm = PD ? s[7 + NS] : s[kP];
   f[0] = m;
-  f[1] = D == 0 ? m * s[kU] + s[kP] : m * s[kU];
-  f[2] = D == 1 ? m * s[kV] + s[kP] : m * s[kV];
-  f[3] = D == 2 ? m * s[kW] + s[kP] : m * s[kW];
+  f[1] = D == 0 ? m * s[kU] + pm : m * s[kU];
+  f[2] = D == 1 ? m * s[kV] + pm : m * s[kV];
+  f[3] = D == 2 ? m * s[kW] + pm : m * s[kW];
   f[4] = m * (s[kE] + s[kP] * frcp(s[kRho]));
 #pragma unroll
   for (int k = 0; k < NS; ++k) f[5 + k] = m * s[7 + k];
@@ -230,7 +234,7 @@ __device__ __forceinline__ void roe_speeds(const float* L, const float* R,
 }
 
 // HLL flux: hll.py:36-175.
-template <int D, int NS>
+template <int D, int NS, bool PD = false>
 __device__ __forceinline__ void hll_flux(const float* L, const float* R, float pe,
                                          float* F) {
   constexpr int NF = 5 + NS;
@@ -242,8 +246,8 @@ __device__ __forceinline__ void hll_flux(const float* L, const float* R, float p
   const float t2 = 1.0f - t1;
   const float t3 = (sr * fabsf(sl) - sl * fabsf(sr)) * (0.5f * rds);
   float fl[NF], fr[NF], ul[NF], ur[NF];
-  physical_flux<D, NS>(L, fl);
-  physical_flux<D, NS>(R, fr);
+  physical_flux<D, NS, PD>(L, fl);
+  physical_flux<D, NS, PD>(R, fr);
   ul[0] = L[kRho];
   ur[0] = R[kRho];
   ul[1] = L[kRho] * L[kU];
@@ -289,10 +293,26 @@ __device__ __forceinline__ void hllc_star(const float* S, float sk, float sm,
 
 // HLLC flux: hllc.py:33-257 and its wave speeds hllc.py:260-347.  Returns the
 // branch code 0 F_L, 1 F*_L, 2 F*_R, 3 F_R (hllc.py:227-243).
-template <int D, int NS>
-__device__ __forceinline__ int hllc_flux(const float* L, const float* R, float pe,
+// With PD the wave speeds, the contact speed and the star states see the
+// dynamic pressure (hllc.py:109-120 swaps p for p_dyn in the primitives they
+// use); the physical fluxes keep p in the energy flux.
+template <int D, int NS, bool PD = false>
+__device__ __forceinline__ int hllc_flux(const float* L0, const float* R0, float pe,
                                          float* F) {
   constexpr int NF = 5 + NS;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
+  float Lp[NV], Rp[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    Lp[v] = L0[v];
+    Rp[v] = R0[v];
+  }
+  if constexpr (PD) {
+    Lp[kP] = L0[7 + NS];
+    Rp[kP] = R0[7 + NS];
+  }
+  const float* L = Lp;
+  const float* R = Rp;
   float sl, sr;
   roe_speeds<D>(L, R, pe, sl, sr);
   const float vl = L[kU + D], vr = R[kU + D];
@@ -312,7 +332,7 @@ __device__ __forceinline__ int hllc_flux(const float* L, const float* R, float p
   }
   float f[NF];
   if (br == 0 || br == 1) {
-    physical_flux<D, NS>(L, f);
+    physical_flux<D, NS, PD>(L0, f);
     if (br == 1) {
       hllc_star<D, NS>(L, sl, sm, f, F);
     } else {
@@ -320,7 +340,7 @@ __device__ __forceinline__ int hllc_flux(const float* L, const float* R, float p
       for (int k = 0; k < NF; ++k) F[k] = f[k];
     }
   } else {
-    physical_flux<D, NS>(R, f);
+    physical_flux<D, NS, PD>(R0, f);
     if (br == 2) {
       hllc_star<D, NS>(R, sr, sm, f, F);
     } else {
@@ -331,13 +351,13 @@ __device__ __forceinline__ int hllc_flux(const float* L, const float* R, float p
   return br;
 }
 
-template <int FLUX, int D, int NS>
+template <int FLUX, int D, int NS, bool PD = false>
 __device__ __forceinline__ int riemann(const float* L, const float* R, float pe,
                                        float* F) {
   if constexpr (FLUX == kHllc) {
-    return hllc_flux<D, NS>(L, R, pe, F);
+    return hllc_flux<D, NS, PD>(L, R, pe, F);
   } else {
-    hll_flux<D, NS>(L, R, pe, F);
+    hll_flux<D, NS, PD>(L, R, pe, F);
     return 0;
   }
 }

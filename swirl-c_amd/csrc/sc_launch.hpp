@@ -21,7 +21,9 @@ hipError_t launch_branch_codes(const SweepArgs& a, int dim, int ns,
                                uint8_t* codes, hipStream_t stream);
 
 // Generic body, instantiated once per scheme in sc_sweep_<scheme>.hip.
-template <int SCHEME, int FLUX, bool VISC, int NS>
+// PD: reference-state (p_ref / rho_ref) build; BB: boundary face / flux
+// value build.
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
 hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
                                const SweepTuning& tune, hipStream_t stream,
                                int only_dir) {
@@ -33,13 +35,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     ax.seg_len = tune.seg_x;
     const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_x - 1) / tune.by_x,
                     (a.nx + tune.seg_x - 1) / tune.seg_x);
-    if (tune.waves_x == 4) {
-      hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 4>), grid,
-                         dim3(64, tune.by_x), 0, stream, ax);
-    } else {
-      hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, 1>), grid,
-                         dim3(64, tune.by_x), 0, stream, ax);
-    }
+    hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB>), grid,
+                       dim3(64, tune.by_x), 0, stream, ax);
   }
   // z sweep (middle): R -= div_z.
   if (only_dir < 0 || only_dir == 2) {
@@ -48,13 +45,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     az.seg_len = tune.seg_z;
     const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
                     (a.nz + tune.seg_z - 1) / tune.seg_z);
-    if (tune.waves_z == 4) {
-      hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 4>), grid,
-                         dim3(64, tune.by_z), 0, stream, az);
-    } else {
-      hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, 1>), grid,
-                         dim3(64, tune.by_z), 0, stream, az);
-    }
+    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, PD, BB>), grid,
+                       dim3(64, tune.by_z), 0, stream, az);
   }
   // y sweep (last): R -= div_y, then store R or the RK update.
   if (only_dir < 0 || only_dir == 1) {
@@ -62,13 +54,30 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     ay.inv_hdir = 1.0f / h[1];
     constexpr int OUTW = YGeom<SCHEME>::OUTW;
     const dim3 grid((a.ny + OUTW - 1) / OUTW, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
-    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS>), grid, dim3(64, tune.by_y),
-                       0, stream, ay);
+    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB>), grid,
+                       dim3(64, tune.by_y), 0, stream, ay);
   }
   return hipGetLastError();
 }
 
-template <int SCHEME, int NS>
+// Picks the build for the configuration: boundary values (BB) or a reference
+// state (PD, scalars-free runs only) or neither.
+template <int SCHEME, int FLUX, bool VISC>
+hipError_t launch_rhs_variant(const SweepArgs& a, const float h[3], int ns,
+                              const SweepTuning& tune, hipStream_t stream, int od) {
+  const bool bb = a.bbc_dims != 0;
+  if (ns == 0) {
+    if (a.p_ref != nullptr)
+      return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, true, true>(a, h, tune, stream, od)
+                : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, true, false>(a, h, tune, stream, od);
+    return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, true>(a, h, tune, stream, od)
+              : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, false>(a, h, tune, stream, od);
+  }
+  return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, true>(a, h, tune, stream, od)
+            : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(a, h, tune, stream, od);
+}
+
+template <int SCHEME, int NS, bool PD = false>
 hipError_t launch_branch_codes_t(const SweepArgs& a, int dim, uint8_t* codes,
                                  hipStream_t stream) {
   const long long nf = (long long)(a.nx + (dim == 0)) * (a.ny + (dim == 1)) *
@@ -76,11 +85,11 @@ hipError_t launch_branch_codes_t(const SweepArgs& a, int dim, uint8_t* codes,
   const int blk = 256;
   const unsigned grid = (unsigned)((nf + blk - 1) / blk);
   if (dim == 0) {
-    hipLaunchKernelGGL((branch_codes_kernel<0, SCHEME, NS>), grid, blk, 0, stream, a, codes);
+    hipLaunchKernelGGL((branch_codes_kernel<0, SCHEME, NS, PD>), grid, blk, 0, stream, a, codes);
   } else if (dim == 1) {
-    hipLaunchKernelGGL((branch_codes_kernel<1, SCHEME, NS>), grid, blk, 0, stream, a, codes);
+    hipLaunchKernelGGL((branch_codes_kernel<1, SCHEME, NS, PD>), grid, blk, 0, stream, a, codes);
   } else {
-    hipLaunchKernelGGL((branch_codes_kernel<2, SCHEME, NS>), grid, blk, 0, stream, a, codes);
+    hipLaunchKernelGGL((branch_codes_kernel<2, SCHEME, NS, PD>), grid, blk, 0, stream, a, codes);
   }
   return hipGetLastError();
 }
@@ -91,22 +100,17 @@ hipError_t launch_branch_codes_t(const SweepArgs& a, int dim, uint8_t* codes,
                                        int flux, bool visc, int ns,              \
                                        const SweepTuning& tune,                  \
                                        hipStream_t stream, int od) {             \
-    if (ns == 0) {                                                               \
-      if (flux == kHllc)                                                         \
-        return visc ? launch_rhs_sweeps_t<SCHEME, kHllc, true, 0>(a, h, tune, stream, od) \
-                    : launch_rhs_sweeps_t<SCHEME, kHllc, false, 0>(a, h, tune, stream, od); \
-      return visc ? launch_rhs_sweeps_t<SCHEME, kHll, true, 0>(a, h, tune, stream, od)      \
-                  : launch_rhs_sweeps_t<SCHEME, kHll, false, 0>(a, h, tune, stream, od);    \
-    }                                                                            \
     if (flux == kHllc)                                                           \
-      return visc ? launch_rhs_sweeps_t<SCHEME, kHllc, true, 1>(a, h, tune, stream, od)   \
-                  : launch_rhs_sweeps_t<SCHEME, kHllc, false, 1>(a, h, tune, stream, od); \
-    return visc ? launch_rhs_sweeps_t<SCHEME, kHll, true, 1>(a, h, tune, stream, od)        \
-                : launch_rhs_sweeps_t<SCHEME, kHll, false, 1>(a, h, tune, stream, od);      \
+      return visc ? launch_rhs_variant<SCHEME, kHllc, true>(a, h, ns, tune, stream, od)    \
+                  : launch_rhs_variant<SCHEME, kHllc, false>(a, h, ns, tune, stream, od);  \
+    return visc ? launch_rhs_variant<SCHEME, kHll, true>(a, h, ns, tune, stream, od)        \
+                : launch_rhs_variant<SCHEME, kHll, false>(a, h, ns, tune, stream, od);      \
   }                                                                              \
   template <>                                                                    \
   hipError_t launch_branch_codes<SCHEME>(const SweepArgs& a, int dim, int ns,    \
                                          uint8_t* codes, hipStream_t stream) {   \
+    if (ns == 0 && a.p_ref != nullptr)                                           \
+      return launch_branch_codes_t<SCHEME, 0, true>(a, dim, codes, stream);      \
     return ns == 0 ? launch_branch_codes_t<SCHEME, 0>(a, dim, codes, stream)     \
                    : launch_branch_codes_t<SCHEME, 1>(a, dim, codes, stream);    \
   }

@@ -35,10 +35,8 @@ struct CFields {
 struct SweepTuning {
   int seg_x = 32;   // cells per thread, x sweep
   int by_x = 4;     // waves per block, x sweep
-  int waves_x = 0;  // 4: x kernel built for 4 waves/SIMD (<= 128 VGPRs); 0: default
   int seg_z = 32;
   int by_z = 4;
-  int waves_z = 0;
   int by_y = 4;     // waves per block, y sweep
   int xcd_remap = 0;  // 1: XCD-aware block -> tile order (single-grid sweeps)
 };
@@ -90,6 +88,12 @@ struct SweepArgs {
   int bbc_dims;             // bit d: boundary face/flux values along dim d
   const BoundaryBc* bbc;    // device table [dim][side]
   int xcd_remap;            // XCD-aware block order (see tile_of_block)
+  // Hydrostatic reference state (helper_var p_ref / rho_ref), or null:
+  // p_dyn = p - p_ref (zero in halo cells) replaces p in the momentum flux,
+  // d_rho = rho - rho_ref replaces rho in the gravity source
+  // (compressible_navier_stokes.py:273-335).
+  const float* p_ref;
+  const float* rho_ref;
 };
 
 // The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
@@ -143,13 +147,22 @@ __device__ __forceinline__ void load_cons(const SweepArgs& a, long long idx,
   prim_bc_round<NF>(a, interior, q);
 }
 
-template <int NS>
+// Dynamic pressure of a cell (slot 7 + NS): exchange_halos with zero
+// DIRICHLET values on every face leaves 0 in every halo cell.
+template <int NS, bool PD>
+__device__ __forceinline__ void set_pdyn(const SweepArgs& a, long long idx, bool interior,
+                                         float* w) {
+  if constexpr (PD) w[7 + NS] = interior ? w[kP] - a.p_ref[idx] : 0.0f;
+}
+
+template <int NS, bool PD = false>
 __device__ __forceinline__ void load_prim(const SweepArgs& a, long long idx,
                                           bool interior, float pe, float* w) {
   constexpr int NF = 5 + NS;
   float q[NF];
   load_cons<NF>(a, idx, interior, q);
   cons_to_prim<NS>(q, pe, w);
+  set_pdyn<NS, PD>(a, idx, interior, w);
 }
 
 // Velocity components u_c0 and u_c1 at a cell (for transverse derivatives).
@@ -257,7 +270,7 @@ __device__ __forceinline__ void apply_flux_bc(const BoundaryBc& b, int t, float*
 // null) the face states are overwritten on both sides first, and the
 // convective flux, the diffusive flux (viscous runs only) and their sum are
 // overwritten in the reference's order (compressible_navier_stokes.py:147-234).
-template <int DIR, int FLUX, bool VISC, int NS>
+template <int DIR, int FLUX, bool VISC, int NS, bool PD>
 __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* R, float pe,
                                           const float* A, const float* B,
                                           const float (*ta)[2], const float (*tb)[2],
@@ -267,7 +280,7 @@ __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* 
     apply_face_bc<NS>(*b, L);
     apply_face_bc<NS>(*b, R);
   }
-  riemann<FLUX, DIR, NS>(L, R, pe, F);
+  riemann<FLUX, DIR, NS, PD>(L, R, pe, F);
   if (b != nullptr) apply_flux_bc<NF>(*b, 0, F);
   if constexpr (VISC) {
     if (b != nullptr) {
@@ -293,11 +306,14 @@ __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* 
 // one iteration ahead).
 // ---------------------------------------------------------------------------
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
+// BB: build with the boundary face / flux value handling (only launched when
+// the configuration has such values, so other runs carry none of its code).
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
+          bool BB = false>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
-  constexpr int NV = NVar<NS>::value;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   constexpr int NF = 5 + NS;
   const int j = a.h + ti.bx * 64 + ti.tx;
   const int o = a.h + ti.by * ti.by_dim + ti.ty;
@@ -314,12 +330,16 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   auto kk = [&](int c) { return DIR == 0 ? o : c; };
   auto ii = [&](int c) { return DIR == 0 ? c : o; };
 
+  // Every cell of the window is converted to primitives by the same code (the
+  // loop body below, which starts 2*RAD cells early to fill the window), so a
+  // cell's primitives do not depend on where a segment starts: the compiler
+  // may contract multiply-adds differently in separately compiled copies, and
+  // results are then independent of the launch geometry bit for bit.
   float W[WN][NV];
 #pragma unroll
-  for (int w = 0; w < WN - 1; ++w) {
-    const int c = c_begin - 1 - RAD + w;
-    load_prim<NS>(a, base + (long long)c * sm, in_core(c, a.h, n_march),
-                  pe_at(a, kk(c), ii(c), j), W[w]);
+  for (int w = 0; w < WN; ++w) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) W[w][v] = 0.0f;
   }
   float negPrev[NV];
   float Fprev[NF];
@@ -332,12 +352,13 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 
   // Raw conserved values of the cell entering the window, loaded one
   // iteration ahead so the loads overlap the arithmetic of the current cell.
+  const int c_first = c_begin - 1 - 2 * RAD;  // enters cell c_begin - 1 - RAD
   float qn[NF];
-  load_raw<NF>(a, base + (long long)(c_begin - 1 + RAD) * sm, qn);
+  load_raw<NF>(a, base + (long long)(c_first + RAD) * sm, qn);
   float rpre[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) rpre[f] = 0.0f;
-  for (int c = c_begin - 1; c <= c_end; ++c) {
+  for (int c = c_first; c <= c_end; ++c) {
     float rcur[NF];
     if constexpr (PHASE == 1) {
 #pragma unroll
@@ -356,48 +377,52 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
       if (c < c_end) load_raw<NF>(a, base + (long long)(cn + 1) * sm, qn);
       prim_bc_round<NF>(a, in_core(cn, a.h, n_march), qc);
       cons_to_prim<NS>(qc, pe_at(a, kk(cn), ii(cn), j), W[WN - 1]);
+      set_pdyn<NS, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n_march), W[WN - 1]);
     }
-    float negP[NV], posM[NV];
+    if (c >= c_begin - 1) {
+      float negP[NV], posM[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      float s[WN];
+      for (int v = 0; v < NV; ++v) {
+        float s[WN];
 #pragma unroll
-      for (int w = 0; w < WN; ++w) s[w] = W[w][v];
-      recon<SCHEME>(s, negP[v], posM[v]);
-    }
-    float tCur[2][2];
-    if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
-    if (c >= c_begin) {
-      float F[NF];
-      const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
-      const BoundaryBc* bb =
-          (((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
-      face_flux<DIR, FLUX, VISC, NS>(bb, negPrev, posM, pe_at(a, kk(c), ii(c), j), W[RAD - 1],
-                                     W[RAD], tPrev, tCur, a.vc, F);
-      if (c > c_begin) {
-        const long long idx = base + (long long)(c - 1) * sm;
-        if constexpr (PHASE == 0) {
-          const float rho = W[RAD - 1][kRho];
+        for (int w = 0; w < WN; ++w) s[w] = W[w][v];
+        recon<SCHEME>(s, negP[v], posM[v]);
+      }
+      float tCur[2][2];
+      if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
+      if (c >= c_begin) {
+        float F[NF];
+        const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
+        const BoundaryBc* bb =
+            (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
+        face_flux<DIR, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, kk(c), ii(c), j),
+                                           W[RAD - 1], W[RAD], tPrev, tCur, a.vc, F);
+        if (c > c_begin) {
+          const long long idx = base + (long long)(c - 1) * sm;
+          if constexpr (PHASE == 0) {
+            // The gravity source takes d_rho = rho - rho_ref with a reference state.
+            const float rho = PD ? W[RAD - 1][kRho] - a.rho_ref[idx] : W[RAD - 1][kRho];
 #pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
+            for (int f = 0; f < NF; ++f) {
+              a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
+            }
+          } else {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) a.r.p[f][idx] = rcur[f] - (F[f] - Fprev[f]) * a.inv_hdir;
           }
-        } else {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) a.r.p[f][idx] = rcur[f] - (F[f] - Fprev[f]) * a.inv_hdir;
         }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) Fprev[f] = F[f];
       }
 #pragma unroll
-      for (int f = 0; f < NF; ++f) Fprev[f] = F[f];
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) negPrev[v] = negP[v];
-    if constexpr (VISC) {
-      tPrev[0][0] = tCur[0][0];
-      tPrev[0][1] = tCur[0][1];
-      tPrev[1][0] = tCur[1][0];
-      tPrev[1][1] = tCur[1][1];
-    }
+      for (int v = 0; v < NV; ++v) negPrev[v] = negP[v];
+      if constexpr (VISC) {
+        tPrev[0][0] = tCur[0][0];
+        tPrev[0][1] = tCur[0][1];
+        tPrev[1][0] = tCur[1][0];
+        tPrev[1][1] = tCur[1][1];
+      }
+    }  // c >= c_begin - 1
 #pragma unroll
     for (int w = 0; w < WN - 1; ++w) {
 #pragma unroll
@@ -406,13 +431,12 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   }
 }
 
-// MINB = 4 asks the compiler for 4 resident blocks (16 waves) per CU, i.e.
-// at most 128 VGPRs per lane; MINB = 1 leaves the register budget open.
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, int MINB>
-__global__ void __launch_bounds__(256, MINB)
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
+          bool BB = false>
+__global__ void __launch_bounds__(256)
 sweep_march_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
-  sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE>(a, ti);
+  sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, PD, BB>(a, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -428,13 +452,13 @@ struct YGeom {
   static constexpr int OUTW = 64 - 2 * RAD - 2;
 };
 
-template <int SCHEME, int FLUX, bool VISC, int NS>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
 __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = YGeom<SCHEME>::RAD;
   constexpr int K = YGeom<SCHEME>::K;
   constexpr int OUTW = YGeom<SCHEME>::OUTW;
   constexpr int WN = 2 * RAD + 1;
-  constexpr int NV = NVar<NS>::value;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   constexpr int NF = 5 + NS;
   const int lane = ti.tx;
   const int i = a.h + ti.by * ti.by_dim + ti.ty;
@@ -458,7 +482,7 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   }
   const float dt = rk ? *a.dt : 0.0f;
   float W0[NV];
-  load_prim<NS>(a, idx, in_core(yc, a.h, a.ny), pe_at(a, k, i, yc), W0);
+  load_prim<NS, PD>(a, idx, in_core(yc, a.h, a.ny), pe_at(a, k, i, yc), W0);
   float negP[NV], posM[NV], Am[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -483,9 +507,10 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
     tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
   }
   const int side = y == a.h ? 0 : (y == a.h + a.ny ? 1 : -1);
-  const BoundaryBc* bb = (((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
-  face_flux<1, FLUX, VISC, NS>(bb, negPrev, posM, pe_at(a, k, i, yc), Am, W0, tPrev, tCur,
-                               a.vc, F);
+  const BoundaryBc* bb =
+      (BB && ((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
+  face_flux<1, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, k, i, yc), Am, W0, tPrev,
+                                   tCur, a.vc, F);
   float Fn[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) Fn[f] = __shfl(F[f], lane + 1, 64);
@@ -502,23 +527,23 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
 __global__ void __launch_bounds__(256)
 sweep_y_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
-  sweep_y_body<SCHEME, FLUX, VISC, NS>(a, ti);
+  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB>(a, ti);
 }
 
 // ---------------------------------------------------------------------------
 // HLLC branch codes of the interior faces normal to D (parity/debug): one
 // thread per face, straightforward stencil loads.
 // ---------------------------------------------------------------------------
-template <int D, int SCHEME, int NS>
+template <int D, int SCHEME, int NS, bool PD = false>
 __global__ void __launch_bounds__(256)
 branch_codes_kernel(const SweepArgs a, uint8_t* codes) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
-  constexpr int NV = NVar<NS>::value;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   const long long nfx = a.nx + (D == 0), nfy = a.ny + (D == 1), nfz = a.nz + (D == 2);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nfx * nfy * nfz) return;
@@ -535,7 +560,7 @@ branch_codes_kernel(const SweepArgs a, uint8_t* codes) {
   for (int w = 0; w < WN + 1; ++w) {
     const int off = w - RAD - 1;  // cells c-1-RAD .. c+RAD
     const int kk = k + (D == 2) * off, ii = i + (D == 0) * off, jj = j + (D == 1) * off;
-    load_prim<NS>(a, c + off * s, in_core(pc + off, a.h, n) &&
+    load_prim<NS, PD>(a, c + off * s, in_core(pc + off, a.h, n) &&
                   in_core(ii, a.h, a.nx) && in_core(jj, a.h, a.ny) && in_core(kk, a.h, a.nz),
                   pe_at(a, kk, ii, jj), win[w]);
   }
@@ -556,7 +581,7 @@ branch_codes_kernel(const SweepArgs a, uint8_t* codes) {
     apply_face_bc<NS>(a.bbc[2 * D + side], Wr);
   }
   float F[5 + NS];
-  codes[c] = (uint8_t)hllc_flux<D, NS>(Wl, Wr, pe_at(a, k, i, j), F);
+  codes[c] = (uint8_t)hllc_flux<D, NS, PD>(Wl, Wr, pe_at(a, k, i, j), F);
 }
 
 }  // namespace sc

@@ -29,7 +29,7 @@ EXPORTED = (
     'sc_batch_create', 'sc_batch_destroy', 'sc_batch_fill_halos', 'sc_batch_rk_stage',
     'sc_batch_max_wave_speed', 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
     'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning', 'sc_batch_set_tuning',
-    'sc_last_error', 'sc_version',
+    'sc_ctx_set_reference_state', 'sc_last_error', 'sc_version',
 )
 
 
@@ -93,8 +93,8 @@ class ScInstance(ctypes.Structure):
 
 class ScTuning(ctypes.Structure):
   _fields_ = [
-      ('seg_x', ctypes.c_int), ('by_x', ctypes.c_int), ('waves_x', ctypes.c_int),
-      ('seg_z', ctypes.c_int), ('by_z', ctypes.c_int), ('waves_z', ctypes.c_int),
+      ('seg_x', ctypes.c_int), ('by_x', ctypes.c_int),
+      ('seg_z', ctypes.c_int), ('by_z', ctypes.c_int),
       ('by_y', ctypes.c_int), ('xcd_remap', ctypes.c_int),
   ]
 
@@ -142,6 +142,7 @@ def lib():
     L.sc_batch_fill_halos.argtypes = [vp, fpp, vp]
     L.sc_batch_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_batch_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
+    L.sc_ctx_set_reference_state.argtypes = [vp, vp, vp]
     L.sc_ctx_get_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
     L.sc_ctx_set_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
     L.sc_batch_get_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
@@ -157,7 +158,7 @@ def lib():
                  'sc_batch_fill_halos', 'sc_batch_rk_stage', 'sc_batch_max_wave_speed',
                  'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
                  'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning',
-                 'sc_batch_set_tuning'):
+                 'sc_batch_set_tuning', 'sc_ctx_set_reference_state'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

@@ -32,10 +32,14 @@ class InterpolationSchemes(enum.Enum):
   MUSCL = 'MUSCL'
 
 
-def _check_helpers(helper_vars):
+def _check_helpers(ctx, helper_vars):
+  """Hands the hydrostatic reference state to the context when helper_vars
+  holds both 'p_ref' and 'rho_ref' (compressible_navier_stokes.py:309), and
+  clears it otherwise."""
   if helper_vars and 'p_ref' in helper_vars and 'rho_ref' in helper_vars:
-    raise NotImplementedError(
-        'reference-state helper variables (p_ref, rho_ref) are not supported yet')
+    ctx.set_reference_state(helper_vars['p_ref'], helper_vars['rho_ref'])
+  else:
+    ctx.set_reference_state(None, None)
 
 
 def get_rhs_fn(replicas: np.ndarray, cfg, physics_models=None):
@@ -50,7 +54,7 @@ def get_rhs_fn(replicas: np.ndarray, cfg, physics_models=None):
   def rhs(replica_id, conservative, helper_vars):
     """d(conservative)/dt over the interior cells (:69-268)."""
     del replica_id
-    _check_helpers(helper_vars)
+    _check_helpers(ctx, helper_vars)
     q = ctx.fields(conservative)
     out = [torch.zeros_like(t) for t in q]
     ctx.rhs(q, out)

@@ -106,7 +106,7 @@ class NativeContext:
                        ' function is to be used.')
     self._keep = []  # device tensors referenced by the native context
 
-    grid = N.ScGrid()
+    grid = self._grid = N.ScGrid()
     grid.nx, grid.ny, grid.nz = self.n
     grid.halo = self.hw
     grid.dx, grid.dy, grid.dz = self.h
@@ -205,8 +205,16 @@ class NativeContext:
         spec = self._dirichlet_along(faces, dim, side)
         if 'rho' in spec:
           # boundary.py:284-293 replaces the face primitives by
-          # conservative_to_primitive_variables (no p, h), so the reference's
-          # flux step fails on the missing pressure.
+          # conservative_to_primitive_variables (rho, u, v, w, e, scalars: no
+          # p, h): a later primitive value of p or h is then rejected there,
+          # and otherwise the flux step fails on the missing pressure.
+          left = set(self.names) | {'rho', 'u', 'v', 'w', 'e'} | {
+              types.conservative_to_primitive_name(n) for n in self.names[5:]}
+          for var in spec:
+            if var not in left:
+              raise ValueError(
+                  f'"{var}" is not a valid primitive or conservative variable name for'
+                  ' the face boundary condition update.')
           raise KeyError('p')
         for var, val in spec.items():
           if var in self.names:
@@ -339,6 +347,26 @@ class NativeContext:
     N.check(N.lib().sc_hllc_branch_codes(
         self._ctx, N.ptr_array(q), int(dim), ctypes.c_void_p(codes.data_ptr()),
         N.stream_handle(stream)), 'sc_hllc_branch_codes')
+
+  # ---- Hydrostatic reference state ------------------------------------------
+
+  def set_reference_state(self, p_ref, rho_ref):
+    """p_ref / rho_ref fields for the following calls (None, None clears)."""
+    if (p_ref is None) != (rho_ref is None):
+      raise ValueError('give both p_ref and rho_ref, or neither')
+    if p_ref is not None:
+      for name, t in (('p_ref', p_ref), ('rho_ref', rho_ref)):
+        N.require_gpu_tensor(t, name)
+        if tuple(t.shape) != self.shape:
+          raise ValueError(f'{name} has shape {tuple(t.shape)}, expected {self.shape}')
+      self._ref_state = (p_ref, rho_ref)  # kept alive while in use
+      N.check(N.lib().sc_ctx_set_reference_state(
+          self._ctx, ctypes.c_void_p(p_ref.data_ptr()), ctypes.c_void_p(rho_ref.data_ptr())),
+              'sc_ctx_set_reference_state')
+    else:
+      self._ref_state = None
+      N.check(N.lib().sc_ctx_set_reference_state(self._ctx, None, None),
+              'sc_ctx_set_reference_state')
 
   # ---- Launch geometry -----------------------------------------------------
 

@@ -22,7 +22,7 @@ def integrate(replica_id, replicas, state_0, helper_vars, cfg, physics_models,
   dt = cfg.dt if dt is None else dt
   ctx = _stages.native_context(rhs_fn)
   if ctx is not None:
-    _stages._check_helpers(helper_vars)
+    _stages._check_helpers(ctx, helper_vars)
     u0 = ctx.fields(state_0)
     dtd = _stages.dt_tensor(dt, u0[0].device)
     s1 = _stages.empty_like_state(ctx, u0)

@@ -104,9 +104,14 @@ def test_face_value_errors_follow_the_reference():
   with pytest.raises(NotImplementedError):
     cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(
         _cfg((8, 8, 8), 'WENO_5', 'HLLC', True, faces=faces))))
-  faces['rho'] = {'x': {0: (D, 1.0), 1: NONE}, 'y': {0: NONE, 1: NONE}, 'z': {0: NONE, 1: NONE}}
-  cfg = _cfg((8, 8, 8), 'WENO_5', 'HLLC', False, faces=faces)
-  with pytest.raises(KeyError):
-    cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
-  with pytest.raises(KeyError):
-    O.rhs(random_state((8, 8, 8), 3, seed=1), {}, oracle_cfg(cfg))
+  # A density face value drops p and h from the face primitives
+  # (boundary.py:284-293): a 'p' value on the same face is then rejected
+  # (ValueError), and without one the flux step misses the pressure (KeyError).
+  rho_x0 = {'x': {0: (D, 1.0), 1: NONE}, 'y': {0: NONE, 1: NONE}, 'z': {0: NONE, 1: NONE}}
+  for spec, exc in ((dict(faces, rho=rho_x0), ValueError),
+                    ({'rho': rho_x0, 'rho_u': faces['rho_u']}, KeyError)):
+    cfg = _cfg((8, 8, 8), 'WENO_5', 'HLLC', False, faces=spec)
+    with pytest.raises(exc):
+      cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
+    with pytest.raises(exc):
+      O.rhs(random_state((8, 8, 8), 3, seed=1), {}, oracle_cfg(cfg))

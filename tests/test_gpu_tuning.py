@@ -17,9 +17,9 @@ from swirlc_amd.equations import compressible_navier_stokes as cns
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [
-    dict(seg_x=16, by_x=2, waves_x=4, seg_z=64, by_z=1, waves_z=0, by_y=2, xcd_remap=1),
-    dict(seg_x=64, by_x=4, waves_x=0, seg_z=16, by_z=4, waves_z=4, by_y=1, xcd_remap=0),
-    dict(seg_x=5, by_x=3, waves_x=4, seg_z=7, by_z=3, waves_z=4, by_y=3, xcd_remap=1),
+    dict(seg_x=16, by_x=2, seg_z=64, by_z=1, by_y=2, xcd_remap=1),
+    dict(seg_x=64, by_x=4, seg_z=16, by_z=4, by_y=1, xcd_remap=0),
+    dict(seg_x=5, by_x=3, seg_z=7, by_z=3, by_y=3, xcd_remap=1),
 ]
 
 
@@ -49,7 +49,7 @@ def test_tuning_is_bitwise_neutral_single_grid():
   with pytest.raises(N.NativeError):
     ctx.set_tuning(dict(base, by_y=8))
   with pytest.raises(N.NativeError):
-    ctx.set_tuning(dict(base, waves_x=2))
+    ctx.set_tuning(dict(base, seg_x=2))
 
 
 def test_autotuner_picks_a_valid_geometry():
@@ -61,7 +61,7 @@ def test_autotuner_picks_a_valid_geometry():
   best, timings = autotune.tune_context(ctx, q, out, dt, reps=2)
   print(f'[tuning] autotuned {best}', flush=True)
   assert ctx.get_tuning() == best
-  assert len(timings) == 2 * len(autotune.MARCH_SPACE) + len(autotune.Y_SPACE) + 2
+  assert len(timings) == 2 * len(autotune.MARCH_SPACE) + len(autotune.Y_SPACE) + 4
   assert all(ms > 0 for _, _, ms in timings)
 
 
