@@ -32,6 +32,45 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 SWEEP_BYTES = {'x': 40, 'y': 80, 'z': 60}
 # Per RK3 step (viscous, per-direction-sweep accounting): 700 B per cell.
 STEP_BYTES_VISCOUS = 700
+# Measured HBM bytes per sweep launch at 256^3 (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a copy kernel's
+# counts): the file's `traffic_bytes_per_launch <kernel> remap<0|1> <bytes>`
+# lines (XCD-aware block order off / on).
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'sweeps_kh256_hbm_traffic.txt')
+
+
+def measured_traffic(kernel, remap):
+  """HBM bytes per launch of `kernel` in block order `remap` from the
+  committed counter summary, or None if it holds no figure for it."""
+  import re
+  try:
+    text = open(TRAFFIC_FILE).read()
+  except OSError:
+    return None
+  m = re.search(r'^traffic_bytes_per_launch\s+' + re.escape(kernel) + r'\s+remap' +
+                str(int(remap)) + r'\s+(\d+)\s*$', text, re.M)
+  return int(m.group(1)) if m else None
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+  """Device-to-device copy rate (B/s: bytes read + bytes written per second)
+  of a plain copy of `nbytes`, HIP events on the current stream."""
+  import torch
+  src = torch.empty(nbytes // 4, dtype=torch.float32, device='cuda').fill_(1.0)
+  dst = torch.empty_like(src)
+  dst.copy_(src)
+  stream = torch.cuda.current_stream()
+  ev0 = torch.cuda.Event(enable_timing=True)
+  ev1 = torch.cuda.Event(enable_timing=True)
+  ev0.record(stream)
+  for _ in range(reps):
+    dst.copy_(src)
+  ev1.record(stream)
+  ev1.synchronize()
+  sec = ev0.elapsed_time(ev1) / 1e3 / reps
+  del src, dst
+  torch.cuda.empty_cache()
+  return 2 * nbytes / sec
 
 
 def parse():
@@ -89,7 +128,8 @@ def pmc_probe(n):
   """Kernel launches for HBM-traffic counters: a copy of 64 Mi floats (256 MiB
   read + 256 MiB written: the counters' calibration), then the x, z and y
   sweeps (RK3 stage-2 form, U0 distinct from Uk) twice each on the n^3
-  Kelvin-Helmholtz state, in the autotuned launch geometry (the last six
+  Kelvin-Helmholtz state in the default launch geometry with the XCD-aware
+  block order on, then the same six launches with it off (the last twelve
   dispatches)."""
   import torch
   from swirlc_amd import _native as N
@@ -107,16 +147,17 @@ def pmc_probe(n):
   del st
   solver.step(1)
   ctx = solver.ctx
-  from swirlc_amd.core import autotune
-  tuning, _ = autotune.tune_context(ctx, solver.u, solver.s2, solver.dt, u0=solver.s1, reps=2)
-  torch.cuda.synchronize()
-  print(f'[pmc-probe] tuning {tuning}', flush=True)
   ctx.fill_halos(solver.u)
-  for d in (0, 2, 1):
-    for _ in range(2):
-      ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u, solver.s2)
   torch.cuda.synchronize()
-  print(f'[pmc-probe] copy 268435456 B each way; sweeps x, z, y x2 at {n}^3', flush=True)
+  for remap in (1, 0):
+    ctx.set_tuning({'xcd_remap': remap})
+    print(f'[pmc-probe] tuning {ctx.get_tuning()}', flush=True)
+    for d in (0, 2, 1):
+      for _ in range(2):
+        ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u, solver.s2)
+    torch.cuda.synchronize()
+  print(f'[pmc-probe] copy 268435456 B each way; sweeps x, z, y x2 at {n}^3, '
+        'XCD-aware block order on, then off', flush=True)
 
 
 def bench_batch(n_inst, steps, loop_steps):
@@ -205,7 +246,11 @@ def bench_adjoint(n, steps, every):
   scratch = [torch.zeros_like(t) for t in u0]
   dt0 = torch.full((1,), 1e-4, device='cuda')
   autotune.tune_context(nctx, [t.clone() for t in u0], scratch, dt0, reps=3)
-  del scratch
+  g = [torch.randn_like(t) for t in u0]
+  lq = [torch.zeros_like(t) for t in u0]
+  q = [t.clone() for t in u0]
+  adj_tuning, _ = autotune.tune_adjoint(nctx, q, g, lq)
+  del scratch, g, lq, q
   speed = torch.zeros(4, device='cuda')
   dt = torch.zeros(1, device='cuda')
   nctx.max_wave_speed(u0, speed, 0.5, dt)
@@ -232,7 +277,8 @@ def bench_adjoint(n, steps, every):
           'dt': float(dt[0]), 'loss': float(J), 'grad_norm': gnorm, 'finite': finite,
           'forward_s': round(t_fwd, 4), 'adjoint_s': round(t_bwd, 4),
           'adjoint_over_forward': round(t_bwd / t_fwd, 3),
-          'forward_Mcell_updates_per_s': round(n ** 3 * steps / t_fwd / 1e6, 1)}
+          'forward_Mcell_updates_per_s': round(n ** 3 * steps / t_fwd / 1e6, 1),
+          'tuning': {k: adj_tuning[k] for k in ('adj_seg', 'adj_by', 'adj_by_y')}}
 
 
 def cpu_baseline(n, steps):
@@ -250,10 +296,19 @@ def cpu_baseline(n, steps):
   for _ in range(steps):
     st = O.rk3(st, {}, ocfg, dt=dt)
   el = time.perf_counter() - t0
+  model = ''
+  try:
+    for line in open('/proc/cpuinfo'):
+      if line.startswith('model name'):
+        model = line.split(':', 1)[1].strip()
+        break
+  except OSError:
+    pass
   return {'value': n ** 3 * steps / el / 1e6, 'unit': 'Mcell-updates/s', 'cores': 1,
           'kind': 'port',
           'sample': f'{steps} RK3 steps of the same workload at {n}^3 (fp32 numpy oracle, '
-                    f'single thread), {el:.1f} s'}
+                    f'single thread), {el:.1f} s',
+          'cpu_model': model}
 
 
 def main():
@@ -301,10 +356,14 @@ def main():
   dom = max(ktimes, key=ktimes.get)
   bytes_launch = SWEEP_BYTES[dom] * cells
   achieved = bytes_launch / ktimes[dom] / 1e9
+  traffic = measured_traffic(f'sweep_{dom}', tuning['xcd_remap']) if n == 256 else None
+  copy_bw = copy_bandwidth()
   roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK / 1e9,
               'unit': 'GB/s', 'frac': round(achieved * 1e9 / HBM_PEAK, 4),
-              'traffic': None, 'kernel': f'sweep_{dom}',
-              'kernel_ms': {k: round(v * 1e3, 4) for k, v in ktimes.items()}}
+              'traffic': traffic, 'kernel': f'sweep_{dom}',
+              'algorithmic_bytes': bytes_launch,
+              'kernel_ms': {k: round(v * 1e3, 4) for k, v in ktimes.items()},
+              'copy_GBps_measured': round(copy_bw / 1e9, 1)}
   step_frac = cells * STEP_BYTES_VISCOUS / (el / args.steps) / HBM_PEAK
 
   batched = None

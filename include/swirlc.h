@@ -202,6 +202,11 @@ typedef struct sc_tuning {
   int seg_z, by_z;
   int by_y;
   int xcd_remap;            /* 1: XCD-aware block order (single-grid sweeps) */
+  /* Adjoint passes (sc_rhs_vjp, sc_rk_stage_vjp): cells per thread and
+   * waves per block of the marching x / z passes, waves per block of the y
+   * pass.  Ignored by batches. */
+  int adj_seg, adj_by;
+  int adj_by_y;
 } sc_tuning;
 
 sc_status sc_ctx_get_tuning(const sc_ctx* ctx, sc_tuning* out);

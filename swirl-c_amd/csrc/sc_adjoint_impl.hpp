@@ -109,16 +109,25 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
     for (int f = 0; f < 5; ++f) gv[f] = in ? A.g.p[f][idx] : 0.0f;
   };
 
+  // Primitive adjoints of the WN cells c-1-RAD .. c-1+RAD are accumulated
+  // in LDS (dynamic, WN * NV floats per thread), a ring of WN slots per
+  // thread; a thread touches only its own column, so no barrier is needed.
+  HIP_DYNAMIC_SHARED(float, adj_acc_lds)
+  const int nthr = 64 * ti.by_dim;
+  const int tid = ti.ty * 64 + ti.tx;
+  auto acc = [&](int slot, int v) -> float& { return adj_acc_lds[(slot * NV + v) * nthr + tid]; };
   float W[NW][NV];
-  float ACC[WN][NV];
 #pragma unroll
   for (int w = 0; w < WN; ++w) {
     const int c = c_start - 1 - RAD + w;
     load_prim<0>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
                  W[w]);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) ACC[w][v] = 0.0f;
+    for (int v = 0; v < NV; ++v) acc(w, v) = 0.0f;
   }
+  // Slot of cell c-1-RAD (the oldest cell of the window).
+  int s0 = 0;
+  auto slot = [&](int w) { return s0 + w < WN ? s0 + w : s0 + w - WN; };
   // Left state of face c_start (reconstruction of cell c_start - 1).
   float fpPrev[NV], lRprev[NV];
 #pragma unroll
@@ -165,15 +174,16 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       riemann_adj<FLUX, DIR>(fpPrev, fmC, pe_at(a, kk(c), ii(c), j), lF, lL, lR);
       if constexpr (VISC) {
         transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
-        // Cells c-1 and c sit at ACC slots RAD and RAD+1.
+        // Cells c-1 and c sit at window positions RAD and RAD+1.
         float lA[NV], lB[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) lA[v] = lB[v] = 0.0f;
         viscous_adj<DIR>(W[RAD], W[RAD + 1], tPrev, tCur, a.vc, lF, lA, lB, ltPrev, ltCur);
+        const int sa = slot(RAD), sb = slot(RAD + 1);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          ACC[RAD][v] += lA[v];
-          ACC[RAD + 1][v] += lB[v];
+          acc(sa, v) += lA[v];
+          acc(sb, v) += lB[v];
         }
       }
     }
@@ -189,26 +199,26 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       }
       recon_adj<SCHEME>(s, lL[v], lRprev[v], la);
 #pragma unroll
-      for (int w = 0; w < WN; ++w) ACC[w][v] += la[w];
+      for (int w = 0; w < WN; ++w) acc(slot(w), v) += la[w];
     }
     if constexpr (VISC) {
       // Cell c-1 has seen both of its faces.
       if (owned(c - 1) && c - 1 <= f_hi) add_lG<DIR>(A, base + (long long)(c - 1) * sm, ltPrev);
     }
-    // Cell c-1-RAD is complete.
+    // Cell c-1-RAD is complete; its slot passes to cell c+RAD.
     {
       const int x = c - 1 - RAD;
+      float l0[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        l0[v] = acc(s0, v);
+        acc(s0, v) = 0.0f;
+      }
       if (owned(x))
-        retire_cell(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), ACC[0],
-                    DIR == 0, interior(x));
+        retire_cell(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), l0, DIR == 0,
+                    interior(x));
+      s0 = s0 + 1 < WN ? s0 + 1 : 0;
     }
-#pragma unroll
-    for (int w = 0; w < WN - 1; ++w) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) ACC[w][v] = ACC[w + 1][v];
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) ACC[WN - 1][v] = 0.0f;
 #pragma unroll
     for (int w = 0; w < NW - 1; ++w) {
 #pragma unroll
@@ -233,9 +243,12 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
   for (int w = 0; w < WN - 1; ++w) {
     const int x = c_last - RAD + w;
+    float l0[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) l0[v] = acc(slot(w), v);
     if (owned(x))
-      retire_cell(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), ACC[w],
-                  DIR == 0, interior(x));
+      retire_cell(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), l0, DIR == 0,
+                  interior(x));
   }
 }
 
@@ -368,10 +381,6 @@ __global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
   adj_y_body<SCHEME, FLUX, VISC>(A, ti);
 }
 
-// Launch geometry of the adjoint passes: owned cells per thread of the
-// marching passes and waves per block.
-constexpr int kAdjSeg = 32;
-constexpr int kAdjBy = 4;
 
 // Launches the three adjoint passes (x, y, z) for one scheme.
 template <int SCHEME, int FLUX, bool VISC>
@@ -379,32 +388,33 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
                                hipStream_t stream) {
   constexpr int RAD = Radius<SCHEME>::value;
   const SweepArgs& a = A.s;
-  (void)tune;
-  const int seg = kAdjSeg;
+  const int seg = tune.adj_seg, by = tune.adj_by, by_y = tune.adj_by_y;
   A.s.seg_len = seg;
-  const dim3 blk_m(64, kAdjBy);
-  const dim3 blk_y(64, kAdjBy);
+  const dim3 blk_m(64, by);
+  const dim3 blk_y(64, by_y);
+  // LDS ring of the marching passes: (2*RAD+1) cells x 7 primitives per thread.
+  const size_t lds = sizeof(float) * (2 * RAD + 1) * 7 * 64 * by;
   {
     AdjArgs ax = A;
     ax.s.inv_hdir = 1.0f / h[0];
     const int ext = a.nx + 2 * RAD + 2;
-    const dim3 grid((a.ny + 63) / 64, (a.nz + kAdjBy - 1) / kAdjBy, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC>), grid, blk_m, 0, stream, ax);
+    const dim3 grid((a.ny + 63) / 64, (a.nz + by - 1) / by, (ext + seg - 1) / seg);
+    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC>), grid, blk_m, lds, stream, ax);
   }
   {
     AdjArgs ay = A;
     ay.s.inv_hdir = 1.0f / h[1];
     constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
     const int ext = a.ny + 2 * RAD + 2;
-    const dim3 grid((ext + OUTA - 1) / OUTA, (a.nx + kAdjBy - 1) / kAdjBy, a.nz);
+    const dim3 grid((ext + OUTA - 1) / OUTA, (a.nx + by_y - 1) / by_y, a.nz);
     hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC>), grid, blk_y, 0, stream, ay);
   }
   {
     AdjArgs az = A;
     az.s.inv_hdir = 1.0f / h[2];
     const int ext = a.nz + 2 * RAD + 2;
-    const dim3 grid((a.ny + 63) / 64, (a.nx + kAdjBy - 1) / kAdjBy, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC>), grid, blk_m, 0, stream, az);
+    const dim3 grid((a.ny + 63) / 64, (a.nx + by - 1) / by, (ext + seg - 1) / seg);
+    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC>), grid, blk_m, lds, stream, az);
   }
   return hipGetLastError();
 }

@@ -77,12 +77,12 @@ void fill_sweep_state(sc_ctx* c, const float* const* q) {
 // Blocks are 64 x by threads and the kernels are built for at most 256.
 sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
   if (t == nullptr) return fail(SC_ERR_INVALID, "null tuning");
-  const int seg[2] = {t->seg_x, t->seg_z};
-  const int by[3] = {t->by_x, t->by_z, t->by_y};
+  const int seg[3] = {t->seg_x, t->seg_z, t->adj_seg};
+  const int by[5] = {t->by_x, t->by_z, t->by_y, t->adj_by, t->adj_by_y};
   for (int v : seg)
-    if (v < 4 || v > (1 << 20)) return fail(SC_ERR_INVALID, "tuning: seg_* must be in [4, 2^20]");
+    if (v < 4 || v > (1 << 20)) return fail(SC_ERR_INVALID, "tuning: seg_x, seg_z, adj_seg must be in [4, 2^20]");
   for (int v : by)
-    if (v < 1 || v > 4) return fail(SC_ERR_INVALID, "tuning: by_* must be in [1, 4]");
+    if (v < 1 || v > 4) return fail(SC_ERR_INVALID, "tuning: by_x, by_z, by_y, adj_by, adj_by_y must be in [1, 4]");
   if (t->xcd_remap != 0 && t->xcd_remap != 1)
     return fail(SC_ERR_INVALID, "tuning: xcd_remap must be 0 or 1");
   out->seg_x = t->seg_x;
@@ -91,6 +91,9 @@ sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
   out->by_z = t->by_z;
   out->by_y = t->by_y;
   out->xcd_remap = t->xcd_remap;
+  out->adj_seg = t->adj_seg;
+  out->adj_by = t->adj_by;
+  out->adj_by_y = t->adj_by_y;
   return SC_OK;
 }
 
@@ -145,6 +148,9 @@ void from_tuning(const sc::SweepTuning& t, sc_tuning* out) {
   out->by_z = t.by_z;
   out->by_y = t.by_y;
   out->xcd_remap = t.xcd_remap;
+  out->adj_seg = t.adj_seg;
+  out->adj_by = t.adj_by;
+  out->adj_by_y = t.adj_by_y;
 }
 
 }  // namespace
@@ -839,6 +845,10 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
             return fail(SC_ERR_INVALID, "per-layer DIRICHLET values must have halo_width entries");
         }
     const long long cells = (long long)(g.nx + 2 * halo) * (g.ny + 2 * halo) * (g.nz + 2 * halo);
+    const long long NN[3] = {g.nx + 2LL * halo, g.ny + 2LL * halo, g.nz + 2LL * halo};
+    for (int d = 0; d < 3; ++d)
+      if (sc::halo_dim_threads(5, halo, NN, d) > 0x7fffffffLL)
+        return fail(SC_ERR_UNSUPPORTED, "batched instance too large (halo cells of a dim >= 2^31)");
     if (g.offset < end)
       return fail(SC_ERR_INVALID, "instance offsets must be increasing and non-overlapping");
     end = g.offset + cells;
@@ -877,9 +887,8 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     if (e.h[2] < hm) hm = e.h[2];
     hmin[i] = hm;
     const long long N[3] = {g.nx + 2LL * halo, g.ny + 2LL * halo, g.nz + 2LL * halo};
-    const long long plane[3] = {N[2] * N[1], N[2] * N[0], N[0] * N[1]};
     for (int d = 0; d < 3; ++d) {
-      const long long threads = 5 * plane[d];
+      const long long threads = sc::halo_dim_threads(5, halo, N, d);
       for (long long k = 0; k < (threads + 255) / 256; ++k) ht[d].push_back({i, (int)k});
     }
     const long long cells = (long long)g.nx * g.ny * g.nz;

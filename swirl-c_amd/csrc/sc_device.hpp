@@ -84,9 +84,12 @@ __device__ __forceinline__ float dnn_exact(float x, float y) {
 // Conserved (rho, rho_u, rho_v, rho_w, rho_e, rho_s..) -> primitives
 // (rho, u, v, w, e, p, h, s..): common/utils.py:145-183 (phi = (1/rho)*q),
 // generic.py:27-88,357-389 (p = ((gamma-1)*rho) * (e - 0.5*(u^2+v^2+w^2) - pe),
-// h = e + p/rho).
+// h = e + p/rho).  The multiply-adds are written out and contraction is off,
+// so every inlined copy rounds identically: a cell's primitives do not depend
+// on which copy (a marching sweep's prologue or its loop) converted it.
 template <int NS>
 __device__ __forceinline__ void cons_to_prim(const float* q, float pe, float* w) {
+#pragma clang fp contract(off)
   const float rho = q[0];
   const float inv = frcp(rho);
   w[kRho] = rho;
@@ -96,10 +99,11 @@ __device__ __forceinline__ void cons_to_prim(const float* q, float pe, float* w)
   w[kE] = inv * q[4];
 #pragma unroll
   for (int s = 0; s < NS; ++s) w[7 + s] = inv * q[5 + s];
-  const float ke = 0.5f * ((w[kU] * w[kU] + w[kV] * w[kV]) + w[kW] * w[kW]);
+  const float ke =
+      0.5f * __builtin_fmaf(w[kW], w[kW], __builtin_fmaf(w[kV], w[kV], w[kU] * w[kU]));
   const float eint = (w[kE] - ke) - pe;
   w[kP] = (kGm1 * rho) * eint;
-  w[kH] = w[kE] + w[kP] * inv;
+  w[kH] = __builtin_fmaf(w[kP], inv, w[kE]);
 }
 
 // ---------------------------------------------------------------------------
@@ -302,17 +306,19 @@ __device__ __forceinline__ int hllc_flux(const float* L0, const float* R0, float
   constexpr int NF = 5 + NS;
   constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   float Lp[NV], Rp[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    Lp[v] = L0[v];
-    Rp[v] = R0[v];
-  }
+  const float* L = L0;
+  const float* R = R0;
   if constexpr (PD) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      Lp[v] = L0[v];
+      Rp[v] = R0[v];
+    }
     Lp[kP] = L0[7 + NS];
     Rp[kP] = R0[7 + NS];
+    L = Lp;
+    R = Rp;
   }
-  const float* L = Lp;
-  const float* R = Rp;
   float sl, sr;
   roe_speeds<D>(L, R, pe, sl, sr);
   const float vl = L[kU + D], vr = R[kU + D];

@@ -28,32 +28,37 @@ struct HaloGeom {
   long long st[3];
 };
 
-// One thread's share of the dim-d fill: thread t of nf * A * B covers one
-// (field, plane cell) and all h layers on both sides.  Configuration (BC
-// tables, nf, h, prim_mask) comes from a.
-__device__ __forceinline__ void halo_dim_body(const HaloArgs& a, const HaloGeom& g, int d,
-                                              long long t) {
+// One halo cell of the dim-D fill: field f, side (0 low, 1 high), layer,
+// and its position (ia, ib) in the plane of the two other axes.
+struct HaloCell {
+  int f, side, layer, ia, ib;
+};
+
+// Writes one halo cell.  No cell read here is written by another thread of
+// the same dim, so every thread loads its source, if any, then stores once,
+// and all of a wave's loads are in flight together.  Configuration (BC
+// tables, h, prim_mask) comes from a.
+template <int D>
+__device__ __forceinline__ void halo_cell_fill(const HaloArgs& a, const HaloGeom& g,
+                                               const HaloCell& hc) {
   // The two other axes in [z][x][y] order.
-  const int d_a = d == 0 ? 2 : (d == 1 ? 2 : 0);
-  const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
+  constexpr int d_a = D == 0 ? 2 : (D == 1 ? 2 : 0);
+  constexpr int d_b = D == 0 ? 1 : (D == 1 ? 0 : 1);
   const int A = g.N[d_a], B = g.N[d_b];
-  if (t >= (long long)a.nf * A * B) return;
-  const int f = (int)(t / ((long long)A * B));
-  const int rem = (int)(t % ((long long)A * B));
-  const int ia = rem / B, ib = rem % B;
+  const int h = a.h, N = g.N[D], n = g.n[D];
+  const int f = hc.f, side = hc.side, layer = hc.layer, ia = hc.ia, ib = hc.ib;
   float* v = g.q[f];
   const long long base = (long long)ia * g.st[d_a] + (long long)ib * g.st[d_b];
-  const long long s = g.st[d];
-  const int h = a.h, N = g.N[d], n = g.n[d];
+  const long long s = g.st[D];
   const bool prim = (a.prim_mask >> f) & 1;
   // A source cell was already written (and holds a primitive) if it lies in
   // the halo of a dim processed earlier.
   bool earlier_halo = false;
-  if (d >= 1) {
-    const int cx = d_a == 0 ? ia : ib;  // x coordinate (d = 1: d_b = 0; d = 2: d_a = 0)
+  if (D >= 1) {
+    const int cx = d_a == 0 ? ia : ib;  // x coordinate (D = 1: d_b = 0; D = 2: d_a = 0)
     earlier_halo |= (cx < h || cx >= h + g.n[0]);
   }
-  if (d == 2) {
+  if (D == 2) {
     earlier_halo |= (ib < h || ib >= h + g.n[1]);
   }
   const float* rho = g.q[0];
@@ -63,48 +68,48 @@ __device__ __forceinline__ void halo_dim_body(const HaloArgs& a, const HaloGeom&
     if (!prim || earlier_halo) return x;
     return dnn_exact(x, rho[idx]);
   };
-  auto put = [&](int c, float x) { v[base + (long long)c * s] = x; };
-
-  const BcFaceDev& lo = a.bc[f][d][0];
-  const BcFaceDev& hi = a.bc[f][d][1];
+  const BcFaceDev& lo = a.bc[f][D][0];
+  const BcFaceDev& hi = a.bc[f][D][1];
+  // Layer i of the low side is cell i, of the high side cell N - h + i;
+  // Neumann runs outward from the boundary (low side: cell h - 1 - i).
+  int dst;
+  float x;
   if (lo.type == kBcPeriodic || hi.type == kBcPeriodic) {
-    for (int i = 0; i < h; ++i) put(i, get(n + i));
-    for (int i = 0; i < h; ++i) put(N - h + i, get(h + i));
-    return;
-  }
-  for (int side = 0; side < 2; ++side) {
+    dst = side == 0 ? layer : N - h + layer;
+    x = get(side == 0 ? n + layer : h + layer);
+  } else {
     const BcFaceDev& bc = side == 0 ? lo : hi;
     if (bc.type == kBcDirichlet) {
-      for (int i = 0; i < h; ++i) {
-        float x;
-        if (bc.n_layer_values > 0) {
-          x = bc.layer_values[i];
-        } else if (bc.layer_planes != nullptr) {
-          x = bc.layer_planes[((long long)i * A + ia) * B + ib];
-        } else {
-          x = bc.value;
-        }
-        put(side == 0 ? i : N - h + i, x);
-      }
-    } else {  // Neumann
-      if (side == 0) {
-        float prev = get(h);
-        for (int i = 0; i < h; ++i) {
-          prev = prev - bc.value;
-          put(h - 1 - i, prev);
-        }
+      dst = side == 0 ? layer : N - h + layer;
+      if (bc.n_layer_values > 0) {
+        x = bc.layer_values[layer];
+      } else if (bc.layer_planes != nullptr) {
+        x = bc.layer_planes[((long long)layer * A + ia) * B + ib];
       } else {
-        float prev = get(N - h - 1);
-        for (int i = 0; i < h; ++i) {
-          prev = prev + bc.value;
-          put(N - h + i, prev);
-        }
+        x = bc.value;
       }
+    } else if (side == 0) {  // Neumann: ghost = neighbour - value, cumulatively
+      dst = h - 1 - layer;
+      x = get(h);
+      for (int i = 0; i <= layer; ++i) x = x - bc.value;
+    } else {
+      dst = N - h + layer;
+      x = get(N - h - 1);
+      for (int i = 0; i <= layer; ++i) x = x + bc.value;
     }
   }
+  v[base + (long long)dst * s] = x;
 }
 
-__global__ void __launch_bounds__(256) halo_dim_kernel(const HaloArgs a, int d) {
+// Rows of one y-fill block: each row end's 2h halo cells are adjacent
+// lanes, so a wave's stores cover whole row ends.
+__host__ __device__ constexpr int halo_y_rows(int h) { return 256 / (2 * h); }
+
+// Single grid.  D = 0, 2: grid (ceil(B / 256), A, nf * 2h), lanes along ib
+// (y, contiguous: coalesced); (f, side, layer) is uniform per block.
+// D = 1: grid (ceil(B / rows), A, nf), lanes over (row, side, layer).
+template <int D>
+__global__ void __launch_bounds__(256) halo_dim_kernel(const HaloArgs a) {
   HaloGeom g;
   g.q = a.q;
   for (int k = 0; k < 3; ++k) {
@@ -112,12 +117,40 @@ __global__ void __launch_bounds__(256) halo_dim_kernel(const HaloArgs a, int d) 
     g.N[k] = a.N[k];
     g.st[k] = a.st[k];
   }
-  halo_dim_body(a, g, d, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  constexpr int d_b = D == 0 ? 1 : (D == 1 ? 0 : 1);
+  const int B = g.N[d_b];
+  const int h = a.h;
+  HaloCell hc;
+  hc.ia = (int)blockIdx.y;
+  if (D == 1) {
+    const int rows = halo_y_rows(h);
+    const int r = (int)threadIdx.x / (2 * h);
+    const int sl = (int)threadIdx.x - r * 2 * h;
+    if (r >= rows) return;
+    hc.ib = (int)blockIdx.x * rows + r;
+    hc.side = sl / h;
+    hc.layer = sl - hc.side * h;
+    hc.f = (int)blockIdx.z;
+  } else {
+    hc.ib = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int z = (int)blockIdx.z;
+    hc.f = z / (2 * h);
+    hc.side = (z / h) & 1;
+    hc.layer = z % h;
+  }
+  if (hc.ib >= B) return;
+  halo_cell_fill<D>(a, g, hc);
 }
 
+// Batched: tile (instance, blk) covers threads blk*256 .. of the instance's
+// nf * 2h * A * B halo cells (32-bit: the tile builder checks the count),
+// decoded with ib fastest (D = 0, 2) or (side, layer) fastest (D = 1).
+template <int D>
 __global__ void __launch_bounds__(256)
 batch_halo_dim_kernel(const HaloArgs a, CFieldPtrs q_base, const BatchHaloInst* inst,
-                      const BatchHaloTile* tiles, int ntiles, int d) {
+                      const BatchHaloTile* tiles, int ntiles) {
+  constexpr int d_a = D == 0 ? 2 : (D == 1 ? 2 : 0);
+  constexpr int d_b = D == 0 ? 1 : (D == 1 ? 0 : 1);
   for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
     const BatchHaloTile tl = tiles[t];
     const BatchHaloInst in = inst[tl.inst];
@@ -133,7 +166,31 @@ batch_halo_dim_kernel(const HaloArgs a, CFieldPtrs q_base, const BatchHaloInst* 
     g.st[0] = g.N[1];
     g.st[1] = 1;
     g.st[2] = (long long)g.N[0] * g.N[1];
-    halo_dim_body(a, g, d, (long long)tl.blk * blockDim.x + threadIdx.x);
+    const unsigned A = (unsigned)g.N[d_a], B = (unsigned)g.N[d_b], h = (unsigned)a.h;
+    const unsigned id = (unsigned)tl.blk * blockDim.x + threadIdx.x;
+    if (id >= (unsigned)a.nf * 2u * h * A * B) continue;
+    HaloCell hc;
+    unsigned r;
+    if (D == 1) {
+      hc.layer = (int)(id % h);
+      r = id / h;
+      hc.side = (int)(r & 1u);
+      r >>= 1;
+      hc.ib = (int)(r % B);
+      r /= B;
+      hc.ia = (int)(r % A);
+      hc.f = (int)(r / A);
+    } else {
+      hc.ib = (int)(id % B);
+      r = id / B;
+      hc.ia = (int)(r % A);
+      r /= A;
+      hc.layer = (int)(r % h);
+      r /= h;
+      hc.side = (int)(r & 1u);
+      hc.f = (int)(r >> 1);
+    }
+    halo_cell_fill<D>(a, g, hc);
   }
 }
 
@@ -156,29 +213,45 @@ __global__ void __launch_bounds__(256) halo_prim_finish_kernel(const HaloArgs a)
 
 __global__ void speed_init_kernel(unsigned* bits) { *bits = 0u; }
 
+// Waves stride over the interior rows (k, i); the lanes of a wave cover a
+// row's y cells 64 at a time (coalesced, no per-cell index division).
 __global__ void __launch_bounds__(256) max_speed_kernel(const SpeedArgs a, unsigned* bits) {
-  const long long total = (long long)a.nx * a.ny * a.nz;
+  const int rows = a.nx * a.nz;
+  const int lane = (int)threadIdx.x & 63;
+  const int wave = (int)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+  const int nwaves = (int)(gridDim.x * (blockDim.x / 64));
   float m = 0.0f;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int j = a.h + (int)(t % a.ny);
-    const int i = a.h + (int)((t / a.ny) % a.nx);
-    const int k = a.h + (int)(t / ((long long)a.ny * a.nx));
-    const long long idx = (long long)k * a.sz + (long long)i * a.sx + j;
-    float q[5], w[7];
+  constexpr int U = 4;  // cells per lane per pass, loaded together
+  for (int r = wave; r < rows; r += nwaves) {
+    const int i = a.h + r % a.nx;
+    const int k = a.h + r / a.nx;
+    const long long row = (long long)k * a.sz + (long long)i * a.sx;
+    for (int j0 = a.h + lane; j0 < a.h + a.ny; j0 += 64 * U) {
+      float q[U][5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) q[f] = a.q[f][idx];
-    const float pe = a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
-    cons_to_prim<0>(q, pe, w);
-    const float c = sqrtf(kGamma * w[kP] / w[kRho]);
-    const float s = fmaxf(fmaxf(fabsf(w[kU]) + c, fabsf(w[kV]) + c), fabsf(w[kW]) + c);
-    m = fmaxf(m, s);
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + 64 * u;
+        const long long idx = row + min(j, a.h + a.ny - 1);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) q[u][f] = a.q[f][idx];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + 64 * u;
+        if (j >= a.h + a.ny) break;
+        float w[7];
+        const float pe = a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
+        cons_to_prim<0>(q[u], pe, w);
+        const float c = sqrtf(kGamma * w[kP] / w[kRho]);
+        const float s = fmaxf(fmaxf(fabsf(w[kU]) + c, fabsf(w[kV]) + c), fabsf(w[kW]) + c);
+        m = fmaxf(m, s);
+      }
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
   __shared__ float part[4];
-  const int wave = threadIdx.x / 64;
-  if ((threadIdx.x & 63) == 0) part[wave] = m;
+  if (lane == 0) part[threadIdx.x / 64] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     float b = part[0];
@@ -246,9 +319,19 @@ __global__ void batch_cfl_dt_kernel(const unsigned* bits, int n, float cfl,
 hipError_t launch_halo_dim(const HaloArgs& a, int d, hipStream_t stream) {
   const int d_a = d == 0 ? 2 : (d == 1 ? 2 : 0);
   const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
-  const long long n = (long long)a.nf * a.N[d_a] * a.N[d_b];
-  const unsigned grid = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(halo_dim_kernel, grid, 256, 0, stream, a, d);
+  const unsigned A = (unsigned)a.N[d_a], B = (unsigned)a.N[d_b];
+  if (d == 1) {
+    const unsigned rows = (unsigned)halo_y_rows(a.h);
+    hipLaunchKernelGGL(halo_dim_kernel<1>, dim3((B + rows - 1) / rows, A, (unsigned)a.nf), 256,
+                       0, stream, a);
+  } else {
+    const dim3 grid((B + 255) / 256, A, (unsigned)(a.nf * 2 * a.h));
+    if (d == 0) {
+      hipLaunchKernelGGL(halo_dim_kernel<0>, grid, 256, 0, stream, a);
+    } else {
+      hipLaunchKernelGGL(halo_dim_kernel<2>, grid, 256, 0, stream, a);
+    }
+  }
   return hipGetLastError();
 }
 
@@ -257,8 +340,16 @@ hipError_t launch_batch_halo_dim(const HaloArgs& cfg, float* const* q_base,
                                  int ntiles, int grid, int d, hipStream_t stream) {
   CFieldPtrs qb;
   for (int f = 0; f < kMaxFields; ++f) qb.p[f] = f < cfg.nf ? q_base[f] : nullptr;
-  hipLaunchKernelGGL(batch_halo_dim_kernel, dim3(grid), 256, 0, stream, cfg, qb, inst, tiles,
-                     ntiles, d);
+  if (d == 0) {
+    hipLaunchKernelGGL(batch_halo_dim_kernel<0>, dim3(grid), 256, 0, stream, cfg, qb, inst,
+                       tiles, ntiles);
+  } else if (d == 1) {
+    hipLaunchKernelGGL(batch_halo_dim_kernel<1>, dim3(grid), 256, 0, stream, cfg, qb, inst,
+                       tiles, ntiles);
+  } else {
+    hipLaunchKernelGGL(batch_halo_dim_kernel<2>, dim3(grid), 256, 0, stream, cfg, qb, inst,
+                       tiles, ntiles);
+  }
   return hipGetLastError();
 }
 
@@ -289,9 +380,9 @@ hipError_t launch_halo_prim_finish(const HaloArgs& a, hipStream_t stream) {
 hipError_t launch_max_speed(const SpeedArgs& a, unsigned* max_bits, float cfl,
                             float hmin, float* dt, hipStream_t stream) {
   hipLaunchKernelGGL(speed_init_kernel, 1, 1, 0, stream, max_bits);
-  const long long total = (long long)a.nx * a.ny * a.nz;
-  long long blocks = (total + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  // One wave per interior row, at most 16384 blocks of 4 waves.
+  long long blocks = ((long long)a.nx * a.nz + 3) / 4;
+  if (blocks > 16384) blocks = 16384;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(max_speed_kernel, (unsigned)blocks, 256, 0, stream, a, max_bits);
   if (dt != nullptr) {

@@ -29,6 +29,14 @@ struct HaloArgs {
   BcFaceDev bc[kMaxFields][3][2];
 };
 
+// Threads of the dim-d fill of one grid (extents N along x, y, z): one per
+// halo cell (field, side, layer, plane cell).
+inline long long halo_dim_threads(int nf, int h, const long long N[3], int d) {
+  const int d_a = d == 0 ? 2 : (d == 1 ? 2 : 0);
+  const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
+  return (long long)nf * 2 * h * N[d_a] * N[d_b];
+}
+
 // Applies the BCs along dim d (0 x, 1 y, 2 z) to all fields; must be launched
 // for d = 0, 1, 2 in order, then halo_prim_finish if prim_mask != 0.
 hipError_t launch_halo_dim(const HaloArgs& a, int d, hipStream_t stream);

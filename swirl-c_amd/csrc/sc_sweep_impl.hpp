@@ -38,7 +38,10 @@ struct SweepTuning {
   int seg_z = 32;
   int by_z = 4;
   int by_y = 4;     // waves per block, y sweep
-  int xcd_remap = 0;  // 1: XCD-aware block -> tile order (single-grid sweeps)
+  int xcd_remap = 1;  // 1: XCD-aware block -> tile order (single-grid sweeps)
+  int adj_seg = 16;   // adjoint marching passes: owned cells per thread
+  int adj_by = 4;     // adjoint marching passes: waves per block
+  int adj_by_y = 4;   // adjoint y pass: waves per block
 };
 
 // Dirichlet values on the domain's boundary faces of one sweep direction
@@ -330,16 +333,15 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   auto kk = [&](int c) { return DIR == 0 ? o : c; };
   auto ii = [&](int c) { return DIR == 0 ? c : o; };
 
-  // Every cell of the window is converted to primitives by the same code (the
-  // loop body below, which starts 2*RAD cells early to fill the window), so a
-  // cell's primitives do not depend on where a segment starts: the compiler
-  // may contract multiply-adds differently in separately compiled copies, and
-  // results are then independent of the launch geometry bit for bit.
+  // Window of primitives, cells c-RAD .. c+RAD at iteration c; the first
+  // WN-1 cells are converted here (cons_to_prim rounds the same in every
+  // copy, so results do not depend on where a segment starts).
   float W[WN][NV];
 #pragma unroll
-  for (int w = 0; w < WN; ++w) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) W[w][v] = 0.0f;
+  for (int w = 0; w < WN - 1; ++w) {
+    const int c = c_begin - 1 - RAD + w;
+    load_prim<NS, PD>(a, base + (long long)c * sm, in_core(c, a.h, n_march),
+                      pe_at(a, kk(c), ii(c), j), W[w]);
   }
   float negPrev[NV];
   float Fprev[NF];
@@ -352,13 +354,12 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 
   // Raw conserved values of the cell entering the window, loaded one
   // iteration ahead so the loads overlap the arithmetic of the current cell.
-  const int c_first = c_begin - 1 - 2 * RAD;  // enters cell c_begin - 1 - RAD
   float qn[NF];
-  load_raw<NF>(a, base + (long long)(c_first + RAD) * sm, qn);
+  load_raw<NF>(a, base + (long long)(c_begin - 1 + RAD) * sm, qn);
   float rpre[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) rpre[f] = 0.0f;
-  for (int c = c_first; c <= c_end; ++c) {
+  for (int c = c_begin - 1; c <= c_end; ++c) {
     float rcur[NF];
     if constexpr (PHASE == 1) {
 #pragma unroll
@@ -379,50 +380,48 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
       cons_to_prim<NS>(qc, pe_at(a, kk(cn), ii(cn), j), W[WN - 1]);
       set_pdyn<NS, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n_march), W[WN - 1]);
     }
-    if (c >= c_begin - 1) {
-      float negP[NV], posM[NV];
+    float negP[NV], posM[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        float s[WN];
+    for (int v = 0; v < NV; ++v) {
+      float s[WN];
 #pragma unroll
-        for (int w = 0; w < WN; ++w) s[w] = W[w][v];
-        recon<SCHEME>(s, negP[v], posM[v]);
-      }
-      float tCur[2][2];
-      if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
-      if (c >= c_begin) {
-        float F[NF];
-        const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
-        const BoundaryBc* bb =
-            (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
-        face_flux<DIR, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, kk(c), ii(c), j),
-                                           W[RAD - 1], W[RAD], tPrev, tCur, a.vc, F);
-        if (c > c_begin) {
-          const long long idx = base + (long long)(c - 1) * sm;
-          if constexpr (PHASE == 0) {
-            // The gravity source takes d_rho = rho - rho_ref with a reference state.
-            const float rho = PD ? W[RAD - 1][kRho] - a.rho_ref[idx] : W[RAD - 1][kRho];
+      for (int w = 0; w < WN; ++w) s[w] = W[w][v];
+      recon<SCHEME>(s, negP[v], posM[v]);
+    }
+    float tCur[2][2];
+    if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
+    if (c >= c_begin) {
+      float F[NF];
+      const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
+      const BoundaryBc* bb =
+          (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
+      face_flux<DIR, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, kk(c), ii(c), j),
+                                         W[RAD - 1], W[RAD], tPrev, tCur, a.vc, F);
+      if (c > c_begin) {
+        const long long idx = base + (long long)(c - 1) * sm;
+        if constexpr (PHASE == 0) {
+          // The gravity source takes d_rho = rho - rho_ref with a reference state.
+          const float rho = PD ? W[RAD - 1][kRho] - a.rho_ref[idx] : W[RAD - 1][kRho];
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
-              a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
-            }
-          } else {
-#pragma unroll
-            for (int f = 0; f < NF; ++f) a.r.p[f][idx] = rcur[f] - (F[f] - Fprev[f]) * a.inv_hdir;
+          for (int f = 0; f < NF; ++f) {
+            a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
           }
+        } else {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) a.r.p[f][idx] = rcur[f] - (F[f] - Fprev[f]) * a.inv_hdir;
         }
-#pragma unroll
-        for (int f = 0; f < NF; ++f) Fprev[f] = F[f];
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) negPrev[v] = negP[v];
-      if constexpr (VISC) {
-        tPrev[0][0] = tCur[0][0];
-        tPrev[0][1] = tCur[0][1];
-        tPrev[1][0] = tCur[1][0];
-        tPrev[1][1] = tCur[1][1];
-      }
-    }  // c >= c_begin - 1
+      for (int f = 0; f < NF; ++f) Fprev[f] = F[f];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) negPrev[v] = negP[v];
+    if constexpr (VISC) {
+      tPrev[0][0] = tCur[0][0];
+      tPrev[0][1] = tCur[0][1];
+      tPrev[1][0] = tCur[1][0];
+      tPrev[1][1] = tCur[1][1];
+    }
 #pragma unroll
     for (int w = 0; w < WN - 1; ++w) {
 #pragma unroll

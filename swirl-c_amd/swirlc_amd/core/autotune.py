@@ -2,14 +2,17 @@
 
 The reference leaves scheduling to XLA; here each sweep kernel's launch
 geometry (cells per thread and waves per block of the marching x / z sweeps,
-waves per block of the y sweep, and the XCD-aware block order) is chosen per grid by timing the candidates on the actual state
-with HIP events, one direction at a time (a direction's time does not depend
-on the other directions' geometry), then the block order over all three.
-Results are bit-for-bit independent of the choice, so tuning never changes
-the numbers, only the time.
+waves per block of the y sweep, and the XCD-aware block order) is chosen per
+grid by timing the candidates on the actual state with HIP events, one
+direction at a time (a direction's time does not depend on the other
+directions' geometry), then the block order over all three.  Forward results
+are bit-for-bit independent of the choice, so tuning never changes the
+numbers, only the time; the adjoint passes sum each cell's terms in the same
+order under every geometry (equal to a few ulps).
 
   tune_context(nctx, q, out, dt_dev)   single grid (sc_sweep_only timings)
   tune_batch(batch)                    batched ensemble (full-stage timings)
+  tune_adjoint(nctx, q, g, lq)         adjoint passes (sc_rhs_vjp timings)
 """
 
 import itertools
@@ -20,6 +23,8 @@ from swirlc_amd import _native as N
 
 MARCH_SPACE = tuple(itertools.product((16, 32, 64), (2, 4)))  # seg, by
 Y_SPACE = (1, 2, 4)  # waves per block
+ADJ_MARCH_SPACE = tuple(itertools.product((8, 16, 32), (1, 2, 4)))  # adj_seg, adj_by
+ADJ_Y_SPACE = (1, 2, 4)  # adj_by_y
 
 
 MARGIN = 0.02  # a candidate must beat the current geometry by 2 % to replace it
@@ -96,7 +101,7 @@ def tune_context(nctx, q, out, dt_dev, u0=None, reps=5, march_space=MARCH_SPACE,
     for d in (0, 2, 1):
       nctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, dt_dev, u0, q, out)
 
-  for remap in (0, 1):
+  for remap in (base['xcd_remap'], 1 - base['xcd_remap']):
     t = dict(best, xcd_remap=remap)
     nctx.set_tuning(t)
     timings.append(('all', t, _timed(all_three, reps)))
@@ -131,4 +136,32 @@ def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=Y_SPACE):
     timings.append((d, t, ms))
   best = _pick(timings, base)
   batch.set_tuning(best)
+  return best, timings
+
+
+def tune_adjoint(nctx, q, g, lq, reps=3, march_space=ADJ_MARCH_SPACE, y_space=ADJ_Y_SPACE):
+  """Times the RHS VJP of state q (cotangent g, result in lq) for every
+  adjoint geometry: the marching x / z passes share (adj_seg, adj_by), the
+  y pass has adj_by_y.  Keeps the fastest of each if it beats the current
+  geometry by MARGIN.  Returns (tuning, [(knob, tuning, ms)])."""
+  base = nctx.get_tuning()
+  nctx.enable_adjoint()
+  timings = []
+  for seg, by in march_space:
+    t = dict(base, adj_seg=seg, adj_by=by)
+    nctx.set_tuning(t)
+    timings.append(('march', t, _timed(lambda: nctx.rhs_vjp(q, g, lq), reps)))
+  for by in y_space:
+    t = dict(base, adj_by_y=by)
+    nctx.set_tuning(t)
+    timings.append(('y', t, _timed(lambda: nctx.rhs_vjp(q, g, lq), reps)))
+  best = dict(base)
+  for knob, ks in (('march', ('adj_seg', 'adj_by')), ('y', ('adj_by_y',))):
+    rows = [(ms, t) for kk, t, ms in timings if kk == knob]
+    base_ms = [ms for ms, t in rows if all(t[k] == base[k] for k in ks)]
+    ms, t = min(rows, key=lambda r: r[0])
+    if not base_ms or ms < (1.0 - MARGIN) * base_ms[0]:
+      for k in ks:
+        best[k] = t[k]
+  nctx.set_tuning(best)
   return best, timings

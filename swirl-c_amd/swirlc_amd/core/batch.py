@@ -111,8 +111,9 @@ class BatchSolver:
     return t.as_dict()
 
   def set_tuning(self, tuning):
-    """Sets the launch geometry (rebuilds the tile lists; drops a captured step)."""
-    t = N.ScTuning.from_dict(tuning)
+    """Sets the given fields of the launch geometry (others keep their values;
+    rebuilds the tile lists; drops a captured step)."""
+    t = N.ScTuning.from_dict(dict(self.get_tuning(), **tuning))
     N.check(N.lib().sc_batch_set_tuning(self._b, ctypes.byref(t)), 'sc_batch_set_tuning')
     self._graph = None
 

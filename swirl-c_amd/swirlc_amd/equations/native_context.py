@@ -377,7 +377,8 @@ class NativeContext:
     return t.as_dict()
 
   def set_tuning(self, tuning):
-    t = N.ScTuning.from_dict(tuning)
+    """Sets the given fields of the launch geometry (others keep their values)."""
+    t = N.ScTuning.from_dict(dict(self.get_tuning(), **tuning))
     N.check(N.lib().sc_ctx_set_tuning(self._ctx, ctypes.byref(t)), 'sc_ctx_set_tuning')
 
   # ---- Adjoint -------------------------------------------------------------

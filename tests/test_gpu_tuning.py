@@ -1,5 +1,6 @@
 """Launch-geometry tuning changes timings only: every tuning gives bit-identical
-results (single grid and batched), and the autotuner returns a valid one."""
+results (single grid and batched), the adjoint passes agree across their
+geometries, and the autotuners return valid ones."""
 
 import copy
 
@@ -41,7 +42,7 @@ def test_tuning_is_bitwise_neutral_single_grid():
   for t in TUNINGS:
     print(f'[tuning] single grid {t}', flush=True)
     ctx.set_tuning(t)
-    assert ctx.get_tuning() == t
+    assert {k: ctx.get_tuning()[k] for k in t} == t
     got = _stage(ctx, q, dt)
     for a, b in zip(got, ref):
       assert torch.equal(a, b)
@@ -50,6 +51,10 @@ def test_tuning_is_bitwise_neutral_single_grid():
     ctx.set_tuning(dict(base, by_y=8))
   with pytest.raises(N.NativeError):
     ctx.set_tuning(dict(base, seg_x=2))
+  with pytest.raises(N.NativeError):
+    ctx.set_tuning(dict(base, adj_by=5))
+  with pytest.raises(N.NativeError):
+    ctx.set_tuning(dict(base, adj_seg=3))
 
 
 def test_autotuner_picks_a_valid_geometry():
@@ -91,3 +96,42 @@ def test_tuning_is_bitwise_neutral_batched():
   got = run()
   for a, b in zip(got, ref):
     assert torch.equal(a, b)
+
+
+ADJ_TUNINGS = [
+    dict(adj_seg=4, adj_by=1, adj_by_y=2),
+    dict(adj_seg=7, adj_by=3, adj_by_y=3),
+    dict(adj_seg=64, adj_by=2, adj_by_y=1),
+]
+
+
+def test_adjoint_geometry_changes_no_result():
+  """The RHS VJP under several adjoint launch geometries matches the default
+  one to a few ulps (the same per-cell sums in the same order; only the
+  compiler's multiply-add contraction may differ between inlined copies)."""
+  cfg_d, st = configs.kh(n=40)
+  ctx = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg_d))).native_context
+  q = [torch.from_numpy(st[k]).cuda() for k in ctx.names]
+  g = [torch.randn(t.shape, generator=torch.Generator().manual_seed(i)).cuda()
+       for i, t in enumerate(q)]
+
+  def vjp():
+    lq = [torch.zeros_like(t) for t in q]
+    ctx.rhs_vjp(q, g, lq)
+    torch.cuda.synchronize()
+    return lq
+
+  base = ctx.get_tuning()
+  ref = vjp()
+  for t in ADJ_TUNINGS:
+    ctx.set_tuning(t)
+    got = vjp()
+    err = max(float((a - b).abs().max() / b.abs().max()) for a, b in zip(got, ref))
+    print(f'[tuning] adjoint {t}: max rel diff {err:.1e}', flush=True)
+    assert err <= 1e-6
+  ctx.set_tuning(base)
+  lq = [torch.zeros_like(t) for t in q]
+  best, timings = autotune.tune_adjoint(ctx, q, g, lq, reps=1)
+  print(f'[tuning] adjoint autotuned {best}', flush=True)
+  assert ctx.get_tuning() == best
+  assert len(timings) == len(autotune.ADJ_MARCH_SPACE) + len(autotune.ADJ_Y_SPACE)
