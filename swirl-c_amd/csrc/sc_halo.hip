@@ -380,9 +380,11 @@ hipError_t launch_halo_prim_finish(const HaloArgs& a, hipStream_t stream) {
 hipError_t launch_max_speed(const SpeedArgs& a, unsigned* max_bits, float cfl,
                             float hmin, float* dt, hipStream_t stream) {
   hipLaunchKernelGGL(speed_init_kernel, 1, 1, 0, stream, max_bits);
-  // One wave per interior row, at most 16384 blocks of 4 waves.
+  // Up to one wave per interior row, at most 1024 blocks of 4 waves (4 waves
+  // per SIMD; each block ends in one atomic on the shared maximum, so more
+  // blocks only queue atomics).
   long long blocks = ((long long)a.nx * a.nz + 3) / 4;
-  if (blocks > 16384) blocks = 16384;
+  if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(max_speed_kernel, (unsigned)blocks, 256, 0, stream, a, max_bits);
   if (dt != nullptr) {

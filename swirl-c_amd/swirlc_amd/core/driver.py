@@ -6,8 +6,8 @@ contract visible through the reference's configuration is kept here on one
 GPU:
 
   * the initial state comes from `customized_init_fn(replica_id, coordinates,
-    cfg)`, or is all zeros (core/initializer.py:41-75, meant to be replaced by
-    a loaded state);
+    cfg)`, or from `core.initializer.default_initializer` (all zeros,
+    core/initializer.py:41-71, meant to be replaced by a loaded state);
   * the fields named in cfg.states_from_file are then loaded from the save
     file of step cfg.start_step;
   * cfg.num_cycles cycles of cfg.num_steps steps run from cfg.start_step, and
@@ -25,6 +25,7 @@ import os
 import numpy as np
 import torch
 
+from swirlc_amd.core import initializer
 from swirlc_amd.core.solver import Solver
 
 
@@ -66,8 +67,7 @@ def init_state(customized_init_fn, cfg, device='cuda'):
   shape = (cfg.core_nz + 2 * cfg.halo_width, cfg.core_nx + 2 * cfg.halo_width,
            cfg.core_ny + 2 * cfg.halo_width)
   if customized_init_fn is None:
-    return {n: torch.zeros(shape, dtype=torch.float32, device=device)
-            for n in cfg.conservative_variable_names}
+    customized_init_fn = initializer.default_initializer
   states = customized_init_fn(0, (0, 0, 0), cfg)
   out = {}
   for n in cfg.conservative_variable_names:
