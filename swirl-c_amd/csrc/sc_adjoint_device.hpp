@@ -95,13 +95,14 @@ __device__ __forceinline__ void recon_adj<kWeno5>(const float* a, float lfp, flo
   const float lap2 = lNp * (-(wp0 * (p01 + p12) + wp1 * p12));
   const float lqn0 = lNn * an0, lqn1 = lNn * an1, lqn2 = lNn * an2;
   const float lqp0 = lNp * ap0, lqp1 = lNp * ap1, lqp2 = lNp * ap2;
-  // alpha = d * i, i = (eps + beta)^-2: d i / d beta = -2 i / (eps + beta).
+  // alpha = d * i, i = (eps + beta)^-2: d i / d beta = -2 i / (eps + beta),
+  // with 1 / (eps + beta) = (eps + beta) * i (no further reciprocal).
   const float li0 = 0.3f * lan0 + 0.1f * lap0;
   const float li1 = 0.6f * (lan1 + lap1);
   const float li2 = 0.1f * lan2 + 0.3f * lap2;
-  const float lb0 = -2.0f * li0 * i0 * frcp(e0);
-  const float lb1 = -2.0f * li1 * i1 * frcp(e1);
-  const float lb2 = -2.0f * li2 * i2 * frcp(e2);
+  const float lb0 = -2.0f * li0 * i0 * (e0 * i0);
+  const float lb1 = -2.0f * li1 * i1 * (e1 * i1);
+  const float lb2 = -2.0f * li2 * i2 * (e2 * i2);
   // beta = c t^2 + 0.25 s^2 (smoothness-indicator derivatives).
   const float lt0 = 2.0f * c1312 * t0 * lb0, ls0 = 0.5f * s0 * lb0;
   const float lt1 = 2.0f * c1312 * t1 * lb1, ls1 = 0.5f * s1 * lb1;
@@ -147,7 +148,7 @@ __device__ __forceinline__ void recon_adj<kWeno3>(const float* a, float lfp, flo
   const float lap0 = lNp * (ap1 * rp) * p01, lap1 = -lNp * (ap0 * rp) * p01;
   const float lqn0 = lNn * an0, lqn1 = lNn * an1, lqp0 = lNp * ap0, lqp1 = lNp * ap1;
   const float li0 = d0 * lan0 + d1 * lap0, li1 = d1 * lan1 + d0 * lap1;
-  const float lb0 = -2.0f * li0 * i0 * frcp(e0), lb1 = -2.0f * li1 * i1 * frcp(e1);
+  const float lb0 = -2.0f * li0 * i0 * (e0 * i0), lb1 = -2.0f * li1 * i1 * (e1 * i1);
   const float lt0 = 2.0f * t0 * lb0, lt1 = 2.0f * t1 * lb1;
   la[2] += lt0;
   la[1] += -lt0 + lt1;
@@ -353,14 +354,24 @@ __device__ __forceinline__ void hllc_adj(const float* L, const float* R, float p
   } else {
     br = 3;
   }
-  float lsl = 0.0f, lsr = 0.0f, lsm = 0.0f;
-  if (br == 0 || br == 1) {
-    physical_flux_adj<D>(L, lF, lL);
-    if (br == 1) hllc_star_adj<D>(L, sl, sm, lF, lL, lsl, lsm);
-  } else {
-    physical_flux_adj<D>(R, lF, lR);
-    if (br == 2) hllc_star_adj<D>(R, sr, sm, lF, lR, lsr, lsm);
+  // One side's flux (L for branches 0, 1, R for 2, 3), selected per lane as
+  // in the forward, so the lanes of a wave share one adjoint evaluation.
+  const bool left = br <= 1;
+  float S[7], lS[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    S[v] = left ? L[v] : R[v];
+    lS[v] = 0.0f;
   }
+  float lsk = 0.0f, lsm = 0.0f;
+  physical_flux_adj<D>(S, lF, lS);
+  if (br == 1 || br == 2) hllc_star_adj<D>(S, left ? sl : sr, sm, lF, lS, lsk, lsm);
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    lL[v] += left ? lS[v] : 0.0f;
+    lR[v] += left ? 0.0f : lS[v];
+  }
+  float lsl = left ? lsk : 0.0f, lsr = left ? 0.0f : lsk;
   if (!snapped && den != 0.0f && lsm != 0.0f) {
     const float ir = frcp(den);
     const float lnum = lsm * ir;

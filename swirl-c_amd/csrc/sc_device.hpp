@@ -336,23 +336,24 @@ __device__ __forceinline__ int hllc_flux(const float* L0, const float* R0, float
   } else {
     br = 3;
   }
+  // The flux is built from one side's state (L for branches 0, 1, R for 2,
+  // 3), selected per lane, so lanes of a wave on different sides share one
+  // evaluation of the physical flux and of the star correction.
+  const bool left = br <= 1;
+  float S0[NV], Ss[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    S0[v] = left ? L0[v] : R0[v];
+    Ss[v] = left ? L[v] : R[v];
+  }
+  const float sk = left ? sl : sr;
   float f[NF];
-  if (br == 0 || br == 1) {
-    physical_flux<D, NS, PD>(L0, f);
-    if (br == 1) {
-      hllc_star<D, NS>(L, sl, sm, f, F);
-    } else {
-#pragma unroll
-      for (int k = 0; k < NF; ++k) F[k] = f[k];
-    }
+  physical_flux<D, NS, PD>(S0, f);
+  if (br == 1 || br == 2) {
+    hllc_star<D, NS>(Ss, sk, sm, f, F);
   } else {
-    physical_flux<D, NS, PD>(R0, f);
-    if (br == 2) {
-      hllc_star<D, NS>(R, sr, sm, f, F);
-    } else {
 #pragma unroll
-      for (int k = 0; k < NF; ++k) F[k] = f[k];
-    }
+    for (int k = 0; k < NF; ++k) F[k] = f[k];
   }
   return br;
 }
