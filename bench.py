@@ -91,6 +91,9 @@ def parse():
   ap.add_argument('--adjoint-steps', type=int, default=200)
   ap.add_argument('--adjoint-every', type=int, default=20)
   ap.add_argument('--no-adjoint', action='store_true')
+  ap.add_argument('--no-large', action='store_true',
+                  help='skip the 512 x 512 x 256 buoyant-bubble leg')
+  ap.add_argument('--large-steps', type=int, default=10)
   ap.add_argument('--no-tune', action='store_true',
                   help='keep the default launch geometry (no autotuning)')
   ap.add_argument('--pmc-probe', action='store_true',
@@ -281,6 +284,36 @@ def bench_adjoint(n, steps, every):
           'tuning': {k: adj_tuning[k] for k in ('adj_seg', 'adj_by', 'adj_by_y')}}
 
 
+def bench_large(steps):
+  """The buoyant bubble at its full 512 x 512 x 256 size (MUSCL + HLL,
+  viscous, gravity, Dirichlet z walls, CFL 0.5): RK3 steps timed with HIP
+  events after two warm-up steps."""
+  import torch
+  from swirlc_amd import configs
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.core.solver import Solver
+  cfg_d, st = configs.bubble(512, 512, 256)
+  sim = Solver(SwirlCParameters(cfg_d), {k: torch.from_numpy(v).cuda() for k, v in st.items()},
+               cfl=0.5, use_graph=True)
+  del st
+  sim.step(2)
+  torch.cuda.synchronize()
+  ev0 = torch.cuda.Event(enable_timing=True)
+  ev1 = torch.cuda.Event(enable_timing=True)
+  ev0.record()
+  sim.step(steps)
+  ev1.record()
+  ev1.synchronize()
+  ms = ev0.elapsed_time(ev1) / steps
+  cells = 512 * 512 * 256
+  finite = all(bool(torch.isfinite(v).all()) for v in sim.state.values())
+  del sim
+  torch.cuda.empty_cache()
+  return {'workload': 'buoyant_bubble_512x512x256_muscl_hll_viscous_gravity_ssprk3_cfl0.5',
+          'steps': steps, 'ms_per_step': round(ms, 3),
+          'Mcell_updates_per_s': round(cells / ms / 1e3, 1), 'finite': finite}
+
+
 def cpu_baseline(n, steps):
   """The fp32 numpy oracle on an n^3 sample of the same workload (1 thread)."""
   import numpy as np
@@ -375,6 +408,14 @@ def main():
     batched = bench_batch(args.batch_instances, args.batch_steps, args.batch_loop_steps)
     print(f'[bench] batched: {batched}', file=sys.stderr, flush=True)
 
+  large = None
+  if not args.no_large:
+    print(f'[bench] buoyant bubble 512x512x256, {args.large_steps} steps', file=sys.stderr,
+          flush=True)
+    torch.cuda.empty_cache()
+    large = bench_large(args.large_steps)
+    print(f'[bench] large: {large}', file=sys.stderr, flush=True)
+
   adjoint = None
   if not args.no_adjoint:
     print(f'[bench] adjoint: {args.adjoint_n}^3, {args.adjoint_steps} steps', file=sys.stderr,
@@ -400,6 +441,7 @@ def main():
       'step_hbm_frac_700B': round(step_frac, 4),
       'cpu_baseline': cpu,
       'batched': batched,
+      'large': large,
       'adjoint': adjoint,
       'finite': finite,
   }
