@@ -120,30 +120,31 @@ __device__ __forceinline__ void recon(const float* a, float& fp, float& fm);
 template <>
 __device__ __forceinline__ void recon<kWeno5>(const float* a, float& fp, float& fm) {
   const float vm2 = a[0], vm1 = a[1], v0 = a[2], vp1 = a[3], vp2 = a[4];
-  const float c1312 = (float)(13.0 / 12.0);
-  float t, s;
-  t = v0 - 2.0f * vp1 + vp2;
-  s = 3.0f * v0 - 4.0f * vp1 + vp2;
-  const float b0 = c1312 * (t * t) + 0.25f * (s * s);
-  t = vm1 - 2.0f * v0 + vp1;
-  s = vm1 - vp1;
-  const float b1 = c1312 * (t * t) + 0.25f * (s * s);
-  t = vm2 - 2.0f * vm1 + v0;
-  s = vm2 - 4.0f * vm1 + 3.0f * v0;
-  const float b2 = c1312 * (t * t) + 0.25f * (s * s);
-  // Candidate stencils (coefficients c[k][r] of the prototype).
-  const float c13 = (float)(1.0 / 3.0), c56 = (float)(5.0 / 6.0);
-  const float c16 = (float)(-1.0 / 6.0), c76 = (float)(-7.0 / 6.0);
-  const float c116 = (float)(11.0 / 6.0);
-  const float qn0 = (c13 * v0 + c56 * vp1) + c16 * vp2;
-  const float qn1 = (c16 * vm1 + c56 * v0) + c13 * vp1;
-  const float qn2 = (c13 * vm2 + c76 * vm1) + c116 * v0;
-  const float qp0 = (c116 * v0 + c76 * vp1) + c13 * vp2;
-  const float qp1 = (c13 * vm1 + c56 * v0) + c16 * vp1;
-  const float qp2 = (c16 * vm2 + c56 * vm1) + c13 * v0;
+  const float c1312 = (float)(13.0 / 12.0), c16 = (float)(1.0 / 6.0);
+  // First and second differences of the stencil.
+  const float dm2 = vm1 - vm2, dm1 = v0 - vm1, d0 = vp1 - v0, d1 = vp2 - vp1;
+  const float t0 = d1 - d0, t1 = d0 - dm1, t2 = dm1 - dm2;
+  // Smoothness indicators beta_r = 13/12 t_r^2 + 1/4 s_r^2 (s_r / 2 below).
+  const float s0 = 0.5f * d1 - 1.5f * d0;
+  const float s1 = -0.5f * (d0 + dm1);
+  const float s2 = 1.5f * dm1 - 0.5f * dm2;
+  const float b0 = c1312 * (t0 * t0) + s0 * s0;
+  const float b1 = c1312 * (t1 * t1) + s1 * s1;
+  const float b2 = c1312 * (t2 * t2) + s2 * s2;
+  // Candidate values of the prototype's c[k][r] stencils from the
+  // differences (q = v0 +- d / 2 + t * {1/3, -1/6}; a stencil of equal values
+  // gives that value exactly), then the standard weighted average, whose
+  // rounding keeps a result near zero as small, and of the same sign, as the
+  // reference's (the dynamic pressure next to its zero halo cells feeds a
+  // square root).
+  const float c13 = (float)(1.0 / 3.0);
+  const float hn0 = v0 + 0.5f * d0, hn1 = v0 + 0.5f * dm1;
+  const float hp0 = v0 - 0.5f * d0, hp1 = v0 - 0.5f * dm1;
+  const float qn0 = hn0 - c16 * t0, qn1 = hn0 - c16 * t1, qn2 = hn1 + c13 * t2;
+  const float qp0 = hp0 + c13 * t0, qp1 = hp1 - c16 * t1, qp2 = hp1 - c16 * t2;
   // alpha_r = d_r / (eps + beta_r)^2: one reciprocal per stencil serves the
-  // weights of both faces; omega_r = alpha_r / sum(alpha).
-  float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
+  // weights of both faces; f = sum(alpha_r q_r) / sum(alpha_r).
+  const float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
   const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1), i2 = frcp(e2 * e2);
   const float an0 = 0.3f * i0, an1 = 0.6f * i1, an2 = 0.1f * i2;
   const float ap0 = 0.1f * i0, ap1 = 0.6f * i1, ap2 = 0.3f * i2;
