@@ -724,7 +724,10 @@ sc_status upload(const std::vector<T>& v, T** dst) {
                    "hipMemcpy batch table");
 }
 
-int grid_for(int ntiles) { return ntiles < 2048 ? (ntiles > 0 ? ntiles : 1) : 2048; }
+// One block per tile: blocks pick up their tile descriptor in parallel (a
+// grid-stride walk by fewer blocks serialises the descriptor loads, which
+// dominated the halo fills' many small tiles).
+int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 
 // Sweep tile lists of a batch for its current tuning: x sweep (y chunks of
 // 64, z groups, x segments); y sweep (y chunks, x groups, z); z sweep (y

@@ -3,10 +3,10 @@
 //
 // Instance i's field f occupies [off_i, off_i + N_i) of arena f (N_i its
 // halo-padded cell count); all instances share the physics configuration.
-// A tile is one block of the plain launch geometry of one instance; the
-// kernels walk the tile list with a grid-stride loop, so one launch of
-// ~(CUs x resident blocks) blocks covers every instance (no per-instance
-// launches, no idle tail for small grids).  Per-instance dt comes from a
+// A tile is one block of the plain launch geometry of one instance; one
+// launch covers every tile of every instance (one block per tile; the
+// kernels' tile loop also accepts fewer blocks), so nothing is launched per
+// instance and small grids leave no idle tail.  Per-instance dt comes from a
 // device array, so a whole RK step of the ensemble can be captured in a HIP
 // graph and replayed.
 #pragma once
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) batch_y_kernel(const BatchArgs b) {
 struct BatchTiles {
   const BatchTile* t[3];
   int n[3];
-  int grid[3];  // blocks launched per direction (grid-stride over the tiles)
+  int grid[3];  // blocks launched per direction (one per tile)
 };
 
 template <int SCHEME, int FLUX, bool VISC, int NS>
