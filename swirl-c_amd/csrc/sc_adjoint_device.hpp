@@ -46,6 +46,53 @@ __device__ __forceinline__ void cons_to_prim_adj(const float* W, float pe, const
 }
 
 // ---------------------------------------------------------------------------
+// Forward replay of the face reconstruction inside the adjoint passes.  For
+// WENO-5 this evaluates the candidates and smoothness indicators directly
+// from the cell values (the same function as recon<kWeno5>, rounded
+// differently): the difference form of recon<kWeno5> shares its first and
+// second differences with recon_adj's, and the compiler then keeps them alive
+// across the Riemann adjoint between the two calls (+60 registers, one
+// occupancy level less).
+// ---------------------------------------------------------------------------
+template <int SCHEME>
+__device__ __forceinline__ void recon_replay(const float* a, float& fp, float& fm) {
+  recon<SCHEME>(a, fp, fm);
+}
+
+template <>
+__device__ __forceinline__ void recon_replay<kWeno5>(const float* a, float& fp, float& fm) {
+  const float vm2 = a[0], vm1 = a[1], v0 = a[2], vp1 = a[3], vp2 = a[4];
+  const float c1312 = (float)(13.0 / 12.0);
+  float t, s;
+  t = v0 - 2.0f * vp1 + vp2;
+  s = 3.0f * v0 - 4.0f * vp1 + vp2;
+  const float b0 = c1312 * (t * t) + 0.25f * (s * s);
+  t = vm1 - 2.0f * v0 + vp1;
+  s = vm1 - vp1;
+  const float b1 = c1312 * (t * t) + 0.25f * (s * s);
+  t = vm2 - 2.0f * vm1 + v0;
+  s = vm2 - 4.0f * vm1 + 3.0f * v0;
+  const float b2 = c1312 * (t * t) + 0.25f * (s * s);
+  const float c13 = (float)(1.0 / 3.0), c56 = (float)(5.0 / 6.0);
+  const float c16 = (float)(-1.0 / 6.0), c76 = (float)(-7.0 / 6.0);
+  const float c116 = (float)(11.0 / 6.0);
+  const float qn0 = (c13 * v0 + c56 * vp1) + c16 * vp2;
+  const float qn1 = (c16 * vm1 + c56 * v0) + c13 * vp1;
+  const float qn2 = (c13 * vm2 + c76 * vm1) + c116 * v0;
+  const float qp0 = (c116 * v0 + c76 * vp1) + c13 * vp2;
+  const float qp1 = (c13 * vm1 + c56 * v0) + c16 * vp1;
+  const float qp2 = (c16 * vm2 + c56 * vm1) + c13 * v0;
+  const float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
+  const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1), i2 = frcp(e2 * e2);
+  const float an0 = 0.3f * i0, an1 = 0.6f * i1, an2 = 0.1f * i2;
+  const float ap0 = 0.1f * i0, ap1 = 0.6f * i1, ap2 = 0.3f * i2;
+  const float rn = frcp((an0 + an1) + an2);
+  const float rp = frcp((ap0 + ap1) + ap2);
+  fp = ((an0 * qn0 + an1 * qn1) + an2 * qn2) * rn;
+  fm = ((ap0 * qp0 + ap1 * qp1) + ap2 * qp2) * rp;
+}
+
+// ---------------------------------------------------------------------------
 // Reconstruction adjoints: given the adjoints lfp (left state of face c+1/2)
 // and lfm (right state of face c-1/2), accumulate into la[] (offsets -R..R).
 // ---------------------------------------------------------------------------

@@ -135,7 +135,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
     float s[WN], dummy;
 #pragma unroll
     for (int w = 0; w < WN; ++w) s[w] = W[w][v];
-    recon<SCHEME>(s, fpPrev[v], dummy);
+    recon_replay<SCHEME>(s, fpPrev[v], dummy);
     lRprev[v] = 0.0f;
   }
   float gPrev[5];
@@ -158,7 +158,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       float s[WN];
 #pragma unroll
       for (int w = 0; w < WN; ++w) s[w] = W[w + 1][v];
-      recon<SCHEME>(s, fpC[v], fmC[v]);
+      recon_replay<SCHEME>(s, fpC[v], fmC[v]);
     }
     float gC[5];
     load_g(c, gC);
@@ -309,7 +309,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
       S[v][w] = off == 0 ? W0[v] : __shfl(W0[v], lane + off, 64);
     }
     Am[v] = S[v][RAD - 1];
-    recon<SCHEME>(S[v], fp[v], fm[v]);
+    recon_replay<SCHEME>(S[v], fp[v], fm[v]);
   }
   float gC[5];
 #pragma unroll
