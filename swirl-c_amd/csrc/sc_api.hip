@@ -174,8 +174,8 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
     return fail(SC_ERR_INVALID,
                 "numeric_flux_scheme is not implemented. Valid options are: "
                 "['HLL', 'HLLC']");
-  if (p.n_fields < 5 || p.n_fields > 6)
-    return fail(SC_ERR_UNSUPPORTED, "n_fields must be 5 or 6 (at most one transported scalar)");
+  if (p.n_fields < 5 || p.n_fields > SC_MAX_FIELDS)
+    return fail(SC_ERR_UNSUPPORTED, "n_fields must be 5, 6 or 7 (at most two transported scalars)");
   if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1)
     return fail(SC_ERR_INVALID, "grid extents must be positive");
   const int h = grid->halo;

@@ -73,8 +73,11 @@ hipError_t launch_rhs_variant(const SweepArgs& a, const float h[3], int ns,
     return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, true>(a, h, tune, stream, od)
               : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, false>(a, h, tune, stream, od);
   }
-  return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, true>(a, h, tune, stream, od)
-            : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(a, h, tune, stream, od);
+  if (ns == 1)
+    return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, true>(a, h, tune, stream, od)
+              : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(a, h, tune, stream, od);
+  return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 2, false, true>(a, h, tune, stream, od)
+            : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 2, false, false>(a, h, tune, stream, od);
 }
 
 template <int SCHEME, int NS, bool PD = false>
@@ -111,8 +114,9 @@ hipError_t launch_branch_codes_t(const SweepArgs& a, int dim, uint8_t* codes,
                                          uint8_t* codes, hipStream_t stream) {   \
     if (ns == 0 && a.p_ref != nullptr)                                           \
       return launch_branch_codes_t<SCHEME, 0, true>(a, dim, codes, stream);      \
-    return ns == 0 ? launch_branch_codes_t<SCHEME, 0>(a, dim, codes, stream)     \
-                   : launch_branch_codes_t<SCHEME, 1>(a, dim, codes, stream);    \
+    if (ns == 0) return launch_branch_codes_t<SCHEME, 0>(a, dim, codes, stream); \
+    return ns == 1 ? launch_branch_codes_t<SCHEME, 1>(a, dim, codes, stream)     \
+                   : launch_branch_codes_t<SCHEME, 2>(a, dim, codes, stream);    \
   }
 
 }  // namespace sc

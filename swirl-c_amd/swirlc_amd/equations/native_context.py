@@ -80,8 +80,8 @@ class NativeContext:
     if self.names[:5] != list(types.BASE_CONSERVATIVE):
       raise ValueError(
           f'conservative_variable_names must start with {list(types.BASE_CONSERVATIVE)}')
-    if len(self.names) > 6:
-      raise NotImplementedError('at most one transported scalar is supported')
+    if len(self.names) > 7:
+      raise NotImplementedError('at most two transported scalars are supported')
     # The reference rebuilds face conserved states from the primitives listed
     # in primitive_variable_names (compressible_navier_stokes.py:131-144), so a
     # scalar whose primitive is not listed has no face state there.

@@ -1,9 +1,10 @@
-"""Transported passive scalars (rho_y: one extra conserved field) on the GPU.
+"""Transported passive scalars (rho_y0, rho_y1: extra conserved fields) on the GPU.
 
 The reference carries scalars through reconstruction and the convective flux
 (equations/compressible_navier_stokes.py:419-427, hllc.py:182-192, hll.py)
-and its demos use one (rho_y in demo/sods_shock_tube, rho_c in
-demo/kelvin_helmholtz_instability); the diffusive flux leaves them untouched.
+for any number of them; its demos use one (rho_y in demo/sods_shock_tube,
+rho_c in demo/kelvin_helmholtz_instability) and the C ABI takes up to two;
+the diffusive flux leaves them untouched.
 Bar: per-field relative L-infinity error <= 1e-5 against the fp32 oracle,
 scalar mass conserved to roundoff on periodic grids.
 """
@@ -25,28 +26,34 @@ from helpers import grid_cfg, interior, oracle_cfg, random_state, rel_linf
 pytestmark = pytest.mark.gpu
 BC = bc_types.BoundaryCondition
 TOL = 1e-5
-NAMES = ['rho', 'rho_u', 'rho_v', 'rho_w', 'rho_e', 'rho_y0']
+BASE = ['rho', 'rho_u', 'rho_v', 'rho_w', 'rho_e']
 
 
-def _cfg(n, scheme, flux, visc, bcs):
+def _names(nscalar):
+  return BASE + [f'rho_y{s}' for s in range(nscalar)]
+
+
+def _cfg(n, scheme, flux, visc, bcs, nscalar=1):
+  names = _names(nscalar)
   cfg = grid_cfg(n, 3)
   if bcs == 'periodic':
     spec = {d: {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)} for d in 'xyz'}
-    cells = {k: spec for k in NAMES}
+    cells = {k: spec for k in names}
   else:
     cells = {}
-    for i, k in enumerate(NAMES):
+    for i, k in enumerate(names):
       cells[k] = {'x': {0: (BC.NEUMANN, 0.0), 1: (BC.NEUMANN, 0.01 * i)},
                   'y': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
-                  'z': {0: (BC.DIRICHLET, 0.5 if k == 'rho_y0' else
+                  'z': {0: (BC.DIRICHLET, 0.5 - 0.2 * (i - 5) if i >= 5 else
                                           {'rho': 1.0, 'rho_u': 0.0, 'rho_v': 0.1,
                                            'rho_w': 0.0, 'rho_e': 6.0}[k]),
                         1: (BC.NEUMANN, 0.0)}}
-    # The scalar's BC may also be given on its primitive (y0 = rho_y0 / rho).
+    # The first scalar's BC is given on its primitive (y0 = rho_y0 / rho).
     cells['y0'] = cells.pop('rho_y0')
   cfg.update({'interpolation_scheme': scheme, 'numeric_flux_scheme': flux,
-              'include_diffusion': visc, 'conservative_variable_names': NAMES,
-              'primitive_variable_names': ['rho', 'u', 'v', 'w', 'e', 'y0'],
+              'include_diffusion': visc, 'conservative_variable_names': names,
+              'primitive_variable_names': ['rho', 'u', 'v', 'w', 'e'] +
+                                          [f'y{s}' for s in range(nscalar)],
               'transport_parameters': {'nu': 2e-3, 'pr': 0.7}, 'dt': 2e-3,
               'bc': {'cell_averages': cells}})
   return cfg
@@ -61,33 +68,38 @@ COMBOS = [('WENO_5', 'HLLC', True, 'periodic'), ('WENO_5', 'HLL', False, 'period
           ('MUSCL', 'HLLC', True, 'periodic')]
 
 
+@pytest.mark.parametrize('nscalar', [1, 2])
 @pytest.mark.parametrize('scheme,flux,visc,bcs', COMBOS)
-def test_rhs_with_scalar_matches_oracle(scheme, flux, visc, bcs):
+def test_rhs_with_scalar_matches_oracle(scheme, flux, visc, bcs, nscalar):
   n, h = (13, 11, 9), 3
-  cfg = _cfg(n, scheme, flux, visc, bcs)
-  st = random_state(n, h, seed=12, nscalar=1)
+  names = _names(nscalar)
+  cfg = _cfg(n, scheme, flux, visc, bcs, nscalar)
+  st = random_state(n, h, seed=12, nscalar=nscalar)
   ref = O.rhs(st, {}, oracle_cfg(cfg))
   fn = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
   out = fn(0, _gpu(st), {})
   torch.cuda.synchronize()
-  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in NAMES}
-  print(f'[scalar] {scheme}/{flux}/visc={visc}/{bcs}: ' +
+  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
+  print(f'[scalar] {scheme}/{flux}/visc={visc}/{bcs}/ns={nscalar}: ' +
         ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
   assert max(errs.values()) <= TOL, errs
   if bcs == 'periodic':
-    r = interior(out['rho_y0'].cpu().numpy(), h).astype(np.float64)
-    assert abs(r.sum()) <= 1e-5 * np.abs(r).sum()
+    for k in names[5:]:
+      r = interior(out[k].cpu().numpy(), h).astype(np.float64)
+      assert abs(r.sum()) <= 1e-5 * np.abs(r).sum()
 
 
-def test_rk3_step_with_scalar_matches_oracle():
+@pytest.mark.parametrize('nscalar', [1, 2])
+def test_rk3_step_with_scalar_matches_oracle(nscalar):
   n, h = (13, 11, 9), 3
-  cfg = _cfg(n, 'WENO_5', 'HLLC', True, 'periodic')
-  st = random_state(n, h, seed=13, nscalar=1)
+  names = _names(nscalar)
+  cfg = _cfg(n, 'WENO_5', 'HLLC', True, 'periodic', nscalar)
+  st = random_state(n, h, seed=13, nscalar=nscalar)
   ref = O.rk3(st, {}, oracle_cfg(cfg), dt=np.float32(2e-3))
   c = SwirlCParameters(copy.deepcopy(cfg))
   fn = cns.get_rhs_fn(None, c)
   out = rk3.integrate(0, None, _gpu(st), {}, c, None, fn)
   torch.cuda.synchronize()
-  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in NAMES}
-  print('[scalar] rk3: ' + ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
+  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
+  print(f'[scalar] rk3 ns={nscalar}: ' + ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
   assert max(errs.values()) <= TOL, errs

@@ -80,6 +80,14 @@ def test_reflective_rejected():
   assert st == 2 and 'REFLECTIVE' in msg
 
 
+@pytest.mark.parametrize('nf', [4, 8])
+def test_field_count_out_of_range_rejected(nf):
+  p = _params()
+  p.n_fields = nf
+  st, msg = _create(_grid(), p)
+  assert st == 2 and 'n_fields' in msg
+
+
 def test_periodic_with_too_few_cells_rejected():
   st, msg = _create(_grid(n=2), _params(N.SC_MUSCL))
   assert st != N.SC_OK and 'PERIODIC' in msg
@@ -104,3 +112,8 @@ def test_python_config_validation_errors():
     NativeContext(SwirlCParameters({'source_functions': ['coriolis']}))
   with pytest.raises(ValueError, match='cells'):
     NativeContext(SwirlCParameters({'bc': {'cell_averages': {}}}))
+  base = ['rho', 'rho_u', 'rho_v', 'rho_w', 'rho_e']
+  with pytest.raises(NotImplementedError, match='two transported scalars'):
+    NativeContext(SwirlCParameters({
+        'conservative_variable_names': base + ['rho_a', 'rho_b', 'rho_c'],
+        'primitive_variable_names': ['rho', 'u', 'v', 'w', 'e', 'a', 'b', 'c']}))
