@@ -263,8 +263,8 @@ __global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
 // y adjoint pass (contiguous direction).  One wave per run of y cells: lane l
 // holds cell y0 - KA + l; lane l computes face l (cells l-1, l) and the
 // reconstruction adjoint of its own cell; a cell's adjoint gathers, through
-// shuffles, the contributions of the RAD lanes on either side.  Lanes
-// KA .. KA + OUTA - 1 write.
+// DPP lane shifts (every lane active at each shift), the contributions of the
+// RAD lanes on either side.  Lanes KA .. KA + OUTA - 1 write.
 // ---------------------------------------------------------------------------
 template <int SCHEME>
 struct YAdjGeom {
@@ -306,7 +306,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
 #pragma unroll
     for (int w = 0; w < WN; ++w) {
       const int off = w - RAD;
-      S[v][w] = off == 0 ? W0[v] : __shfl(W0[v], lane + off, 64);
+      S[v][w] = lane_shift(W0[v], off);
     }
     Am[v] = S[v][RAD - 1];
     recon_replay<SCHEME>(S[v], fp[v], fm[v]);
@@ -316,9 +316,9 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   for (int f = 0; f < 5; ++f) gC[f] = inner ? A.g.p[f][idx] : 0.0f;
   float L[NV], gP[5];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) L[v] = __shfl(fp[v], lane - 1, 64);
+  for (int v = 0; v < NV; ++v) L[v] = wave_shr1(fp[v]);
 #pragma unroll
-  for (int f = 0; f < 5; ++f) gP[f] = __shfl(gC[f], lane - 1, 64);
+  for (int f = 0; f < 5; ++f) gP[f] = wave_shr1(gC[f]);
   float lL[NV], lR[NV], lA[NV], lB[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) lL[v] = lR[v] = lA[v] = lB[v] = 0.0f;
@@ -327,8 +327,8 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   if constexpr (VISC) transverse_grads<1>(a, k, i, yc, tCur);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    tPrev[t][0] = __shfl(tCur[t][0], lane - 1, 64);
-    tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
+    tPrev[t][0] = wave_shr1(tCur[t][0]);
+    tPrev[t][1] = wave_shr1(tCur[t][1]);
   }
   if (face) {
     float lF[5];
@@ -342,7 +342,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   float la[NV][WN];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    const float lfp = __shfl(lL[v], lane + 1, 64);
+    const float lfp = wave_shl1(lL[v]);
 #pragma unroll
     for (int w = 0; w < WN; ++w) la[v][w] = 0.0f;
     if (rcell) recon_adj<SCHEME>(S[v], lfp, lR[v], la[v]);
@@ -356,15 +356,15 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
 #pragma unroll
     for (int w = 0; w < WN; ++w) {
       const int off = w - RAD;  // lane l - off holds the contribution at w
-      if (off != 0) s += __shfl(la[v][w], lane - off, 64);
+      if (off != 0) s += lane_shift(la[v][w], -off);
     }
-    tot[v] = s + __shfl(lA[v], lane + 1, 64);
+    tot[v] = s + wave_shl1(lA[v]);
   }
   float lt[2][2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    lt[t][0] = ltb[t][0] + __shfl(lta[t][0], lane + 1, 64);
-    lt[t][1] = ltb[t][1] + __shfl(lta[t][1], lane + 1, 64);
+    lt[t][0] = ltb[t][0] + wave_shl1(lta[t][0]);
+    lt[t][1] = ltb[t][1] + wave_shl1(lta[t][1]);
   }
   if (out_lane) {
     retire_cell(A, idx, W0, pe, tot, false, inner);

@@ -182,6 +182,9 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   if (h < radius_of(p.interpolation) + 1 || h > SC_MAX_HALO)
     return fail(SC_ERR_INVALID, "halo_width too small for the interpolation stencil"
                                 " (WENO_5 needs 3, WENO_3/MUSCL need 2) or above 4");
+  if (grid->nx >= (1 << 20) || (long long)grid->nx * (grid->ny + 2 * h) >= (1LL << 30))
+    return fail(SC_ERR_UNSUPPORTED, "grid too large: nx must be below 2^20 and "
+                                    "nx * (ny + 2 * halo_width) below 2^30");
   if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0))
     return fail(SC_ERR_INVALID, "grid spacing must be positive");
   const int gdir = gravity_direction(p.g);
@@ -730,8 +733,8 @@ sc_status upload(const std::vector<T>& v, T** dst) {
 int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 
 // Sweep tile lists of a batch for its current tuning: x sweep (y chunks of
-// 64, z groups, x segments); y sweep (y chunks, x groups, z); z sweep (y
-// chunks, x groups, z segments).  Replaces any previous lists.
+// 64, z groups, x segments); y sweep (groups of by_y windows of the z
+// plane's run of rows, z); z sweep (y chunks, x groups, z segments).  Replaces any previous lists.
 sc_status build_sweep_tiles(sc_batch* b) {
   const int rad = radius_of(b->params.interpolation);
   const int outw = 64 - 2 * rad - 2;
@@ -742,9 +745,9 @@ sc_status build_sweep_tiles(sc_batch* b) {
     for (int bz = 0; bz < (g.nx + t.seg_x - 1) / t.seg_x; ++bz)
       for (int by = 0; by < (g.nz + t.by_x - 1) / t.by_x; ++by)
         for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[0].push_back({i, bx, by, bz});
+    const int nwin = sc::y_windows(g.nx, g.ny, b->h, outw);
     for (int bz = 0; bz < g.nz; ++bz)
-      for (int by = 0; by < (g.nx + t.by_y - 1) / t.by_y; ++by)
-        for (int bx = 0; bx < (g.ny + outw - 1) / outw; ++bx) tl[1].push_back({i, bx, by, bz});
+      for (int bx = 0; bx < (nwin + t.by_y - 1) / t.by_y; ++bx) tl[1].push_back({i, bx, 0, bz});
     for (int bz = 0; bz < (g.nz + t.seg_z - 1) / t.seg_z; ++bz)
       for (int by = 0; by < (g.nx + t.by_z - 1) / t.by_z; ++by)
         for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[2].push_back({i, bx, by, bz});
@@ -833,6 +836,9 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
       return fail(SC_ERR_INVALID, "instance extents and spacings must be positive");
     if (halo < radius_of(p.interpolation) + 1 || halo > SC_MAX_HALO)
       return fail(SC_ERR_INVALID, "halo_width too small for the interpolation stencil");
+    if (g.nx >= (1 << 20) || (long long)g.nx * (g.ny + 2 * halo) >= (1LL << 30))
+      return fail(SC_ERR_UNSUPPORTED, "batched instance too large: nx must be below 2^20 and "
+                                      "nx * (ny + 2 * halo_width) below 2^30");
     const int nn[3] = {g.nx, g.ny, g.nz};
     for (int f = 0; f < 5; ++f)
       for (int d = 0; d < 3; ++d)

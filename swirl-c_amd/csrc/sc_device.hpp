@@ -56,6 +56,27 @@ struct NVar {
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Value of v held by lane (this lane + off) of the 64-lane wave.  off must
+// fold to a constant (small, e.g. a stencil offset in an unrolled loop): each
+// step is one DPP move over the whole wave (wave_shr:1 takes lane - 1,
+// wave_shl:1 lane + 1), a VALU op with no LDS round trip or wait.  Lanes whose
+// source lies outside the wave read 0.
+__device__ __forceinline__ float wave_shr1(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_shl1(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_shift(float v, int off) {
+#pragma unroll
+  for (; off < 0; ++off) v = wave_shr1(v);
+#pragma unroll
+  for (; off > 0; --off) v = wave_shl1(v);
+  return v;
+}
+
 // tf.math.divide_no_nan / reciprocal_no_nan.
 __device__ __forceinline__ float dnn(float x, float y) {
   return y == 0.0f ? 0.0f : x * frcp(y);

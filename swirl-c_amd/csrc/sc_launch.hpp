@@ -52,8 +52,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
   if (only_dir < 0 || only_dir == 1) {
     SweepArgs ay = a;
     ay.inv_hdir = 1.0f / h[1];
-    constexpr int OUTW = YGeom<SCHEME>::OUTW;
-    const dim3 grid((a.ny + OUTW - 1) / OUTW, (a.nx + tune.by_y - 1) / tune.by_y, a.nz);
+    const int nwin = y_windows(a.nx, a.ny, a.h, YGeom<SCHEME>::OUTW);
+    const dim3 grid((nwin + tune.by_y - 1) / tune.by_y, 1, a.nz);
     hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB>), grid,
                        dim3(64, tune.by_y), 0, stream, ay);
   }

@@ -17,7 +17,7 @@
 // Layout [z][x][y], y contiguous.  The x and z sweeps map 64 lanes to 64
 // consecutive y cells (coalesced) and march each thread along its pencil
 // with a register sliding window; the y sweep maps lanes along y and gets
-// the stencil neighbours through cross-lane shuffles.
+// the stencil neighbours through DPP lane shifts.
 #pragma once
 
 #include "sc_device.hpp"
@@ -439,10 +439,11 @@ sweep_march_kernel(const SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// y sweep (contiguous direction, last).  Block (64, BY): one wave per (z, x)
-// row chunk; lane l holds cell y0 - K + l and the stencil comes from
-// shuffles.  Outputs cells y0 .. y0 + OUTW - 1: R - div into R (rk_form 0) or
-// the RK stage update into out.
+// y sweep (contiguous direction, last).  Block (64, BY): one wave per window
+// of 64 consecutive positions of a z plane's run of rows (y_windows); the
+// stencil comes from DPP lane shifts (every lane of the wave is active at
+// each shift).  Lanes K .. K + OUTW - 1 at interior cells output: R - div
+// into R (rk_form 0) or the RK stage update into out.
 // ---------------------------------------------------------------------------
 template <int SCHEME>
 struct YGeom {
@@ -450,6 +451,18 @@ struct YGeom {
   static constexpr int K = RAD + 1;
   static constexpr int OUTW = 64 - 2 * RAD - 2;
 };
+
+// The interior x rows of one z plane are adjacent in memory (row stride NY),
+// so the y sweep treats them as one run of nx * NY positions and slides
+// 64-lane windows along it with stride OUTW, across row ends: each row's h >= K
+// halo cells on either side keep every output's stencil inside its own row.
+// Window w's lane K sits at position h + w * OUTW; the last interior cell of
+// the plane is position nx * NY - h - 1.  (One window per 58 positions instead
+// of ceil(ny / 58) per row: 290 lanes per 256-cell row, not 320.)
+__host__ __device__ inline int y_windows(int nx, int ny, int h, int outw) {
+  const long long run = (long long)nx * (ny + 2 * h);
+  return (int)((run - 2 * h + outw - 1) / outw);
+}
 
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
 __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti) {
@@ -460,15 +473,29 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   constexpr int NF = 5 + NS;
   const int lane = ti.tx;
-  const int i = a.h + ti.by * ti.by_dim + ti.ty;
+  const int win = ti.bx * ti.by_dim + ti.ty;
   const int k = a.h + ti.bz;
-  if (i >= a.h + a.nx || k >= a.h + a.nz) return;  // uniform per wave
-  const int y0 = a.h + ti.bx * OUTW;
-  if (y0 >= a.h + a.ny) return;
-  const int y = y0 - K + lane;
-  const int yc = min(y, a.NY - 1);
-  const long long idx = (long long)k * a.sz + (long long)i * a.sx + yc;
-  const bool out_lane = lane >= K && lane < K + OUTW && y < a.h + a.ny;
+  // uniform per wave
+  if (k >= a.h + a.nz || win >= y_windows(a.nx, a.ny, a.h, OUTW)) return;
+  const int run = a.nx * a.NY;
+  const int pos = win * OUTW + a.h - K + lane;
+  const int pc = min(pos, run - 1);
+  // pc = row * NY + yc: a float estimate of the quotient is off by at most
+  // one for row counts below 2^20 (checked at context creation).
+  int row = (int)((float)pc * frcp((float)a.NY));
+  int yc = pc - row * a.NY;
+  if (yc < 0) {
+    --row;
+    yc += a.NY;
+  }
+  if (yc >= a.NY) {
+    ++row;
+    yc -= a.NY;
+  }
+  const int i = a.h + row;
+  const long long idx = (long long)k * a.sz + (long long)a.h * a.sx + pc;
+  const bool out_lane = lane >= K && lane < K + OUTW && pos < run && yc >= a.h &&
+                        yc < a.h + a.ny;
   // The RHS accumulator and the RK operands are read first so their latency
   // overlaps the arithmetic.
   const bool rk = a.rk_form != 0;
@@ -489,30 +516,30 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
 #pragma unroll
     for (int w = 0; w < WN; ++w) {
       const int off = w - RAD;
-      s[w] = off == 0 ? W0[v] : __shfl(W0[v], lane + off, 64);
+      s[w] = lane_shift(W0[v], off);
     }
     Am[v] = s[RAD - 1];
     recon<SCHEME>(s, negP[v], posM[v]);
   }
   float negPrev[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) negPrev[v] = __shfl(negP[v], lane - 1, 64);
+  for (int v = 0; v < NV; ++v) negPrev[v] = wave_shr1(negP[v]);
   float F[NF];
   float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, tPrev[2][2];
   if constexpr (VISC) transverse_grads<1>(a, k, i, yc, tCur);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    tPrev[t][0] = __shfl(tCur[t][0], lane - 1, 64);
-    tPrev[t][1] = __shfl(tCur[t][1], lane - 1, 64);
+    tPrev[t][0] = wave_shr1(tCur[t][0]);
+    tPrev[t][1] = wave_shr1(tCur[t][1]);
   }
-  const int side = y == a.h ? 0 : (y == a.h + a.ny ? 1 : -1);
+  const int side = yc == a.h ? 0 : (yc == a.h + a.ny ? 1 : -1);
   const BoundaryBc* bb =
       (BB && ((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
   face_flux<1, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, k, i, yc), Am, W0, tPrev,
                                    tCur, a.vc, F);
   float Fn[NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) Fn[f] = __shfl(F[f], lane + 1, 64);
+  for (int f = 0; f < NF; ++f) Fn[f] = wave_shl1(F[f]);
   if (out_lane) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
