@@ -151,6 +151,26 @@ def test_rhs_multi_chunk_grid():
   assert max(errs.values()) <= TOL, errs
 
 
+@pytest.mark.parametrize('scheme,h,n,bcs', [
+    ('WENO_3', 2, (11, 7, 6), 'periodic'),
+    ('MUSCL', 2, (9, 61, 5), 'neumann'),
+    ('WENO_5', 4, (10, 9, 7), 'neumann'),
+    ('WENO_5', 3, (6, 1, 5), 'neumann'),
+    ('WENO_5', 3, (5, 130, 4), 'periodic'),
+])
+def test_rhs_halo_widths_and_row_lengths(scheme, h, n, bcs):
+  """Halo widths 2 and 4 (the y sweep's first output sits at position h of
+  the plane's run of rows) and rows much shorter or longer than one 64-lane
+  window (ny = 1: a window spans many rows; ny = 130: a row spans windows)."""
+  cfg = _cfg(n, h, scheme, 'HLLC', True, bcs)
+  st = random_state(n, h, seed=9)
+  ref = O.rhs(st, {}, oracle_cfg(cfg))
+  got = _gpu_rhs(cfg, st)
+  errs = {k: rel_linf(interior(got[k], h), interior(ref[k], h)) for k in ref}
+  _report(f'{scheme}/h={h}/n={n}', errs)
+  assert max(errs.values()) <= TOL, errs
+
+
 def test_halo_fill_matches_oracle_bitwise():
   n, h = (12, 10, 11), 3
   cfg = _cfg(n, h, 'WENO_5', 'HLLC', False, 'mixed')
