@@ -132,7 +132,9 @@ typedef struct sc_ctx sc_ctx;
  * potential-energy tables).  Stands in for get_rhs_fn
  * (equations/compressible_navier_stokes.py:46-67) and PhysicsModels
  * (physics/physics_models.py:45-92): invalid scheme names, missing gravity
- * direction and halo widths too small for the stencil fail here. */
+ * direction and halo widths too small for the stencil fail here, as do
+ * grids with nx >= 2^20 or nx * (ny + 2 * halo) >= 2^30 (SC_ERR_UNSUPPORTED:
+ * the y sweep addresses a z plane's rows with 32-bit positions). */
 sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params,
                         sc_ctx** out);
 sc_status sc_ctx_destroy(sc_ctx* ctx);
