@@ -88,6 +88,14 @@ def test_field_count_out_of_range_rejected(nf):
   assert st == 2 and 'n_fields' in msg
 
 
+@pytest.mark.parametrize('nx,ny', [(1 << 20, 8), (4096, (1 << 18) - 6)])
+def test_grid_beyond_y_sweep_indexing_rejected(nx, ny):
+  g = _grid()
+  g.nx, g.ny = nx, ny
+  st, msg = _create(g, _params())
+  assert st == 2 and 'grid too large' in msg
+
+
 def test_periodic_with_too_few_cells_rejected():
   st, msg = _create(_grid(n=2), _params(N.SC_MUSCL))
   assert st != N.SC_OK and 'PERIODIC' in msg
