@@ -260,11 +260,15 @@ __global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// y adjoint pass (contiguous direction).  One wave per run of y cells: lane l
-// holds cell y0 - KA + l; lane l computes face l (cells l-1, l) and the
-// reconstruction adjoint of its own cell; a cell's adjoint gathers, through
-// DPP lane shifts (every lane active at each shift), the contributions of the
-// RAD lanes on either side.  Lanes KA .. KA + OUTA - 1 write.
+// y adjoint pass (contiguous direction).  One wave per window of 64
+// consecutive positions of a z plane's run of rows (as the forward y sweep):
+// lane l computes face l (cells l-1, l) and the reconstruction adjoint of its
+// own cell; a cell's adjoint gathers, through DPP lane shifts (every lane
+// active at each shift), the contributions of the RAD lanes on either side.
+// Lanes KA .. KA + OUTA - 1 write the cells with y in [h-1-RAD, h+ny+1+RAD)
+// (every cell a y-face stencil reaches).  Only face and reconstruction lanes
+// (y in [h-1, h+ny]) contribute, and all they read lies in their own row
+// (RAD <= h-1), so windows may run across row ends.
 // ---------------------------------------------------------------------------
 template <int SCHEME>
 struct YAdjGeom {
@@ -272,6 +276,14 @@ struct YAdjGeom {
   static constexpr int KA = 2 * RAD + 1;
   static constexpr int OUTA = 62 - 4 * RAD;
 };
+
+// Windows of the adjoint y pass over a plane's nx * NY positions: window w's
+// lane KA sits at position h-1-RAD + w * OUTA; the last output position is
+// (nx-1) * NY + h+ny+RAD.
+__host__ __device__ inline int y_adj_windows(int nx, int ny, int h, int rad, int outa) {
+  const long long span = (long long)(nx - 1) * (ny + 2 * h) + ny + 2 * rad + 2;
+  return (int)((span + outa - 1) / outa);
+}
 
 template <int SCHEME, int FLUX, bool VISC>
 __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) {
@@ -282,20 +294,33 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   constexpr int NV = 7;
   const SweepArgs& a = A.s;
   const int lane = ti.tx;
-  const int i = a.h + ti.by * ti.by_dim + ti.ty;
+  const int win = ti.bx * ti.by_dim + ti.ty;
   const int k = a.h + ti.bz;
-  if (i >= a.h + a.nx || k >= a.h + a.nz) return;  // uniform per wave
-  const int ext_hi = a.h + a.ny + 1 + RAD;
-  const int y0 = a.h - 1 - RAD + ti.bx * OUTA;
-  if (y0 >= ext_hi) return;
-  const int y = y0 - KA + lane;
-  const int yc = min(max(y, 0), a.NY - 1);
-  const long long row = (long long)k * a.sz + (long long)i * a.sx;
-  const long long idx = row + yc;
-  const bool out_lane = lane >= KA && lane < KA + OUTA && y < ext_hi;
-  const bool face = y >= a.h && y <= a.h + a.ny;
-  const bool rcell = y >= a.h - 1 && y <= a.h + a.ny;
-  const bool inner = y >= a.h && y < a.h + a.ny;
+  // uniform per wave
+  if (k >= a.h + a.nz || win >= y_adj_windows(a.nx, a.ny, a.h, RAD, OUTA)) return;
+  const int run = a.nx * a.NY;
+  const int pos = win * OUTA + (a.h - 1 - RAD) - KA + lane;
+  const bool valid = pos >= 0 && pos < run;
+  const int pc = min(max(pos, 0), run - 1);
+  // pc = r * NY + yc (float quotient estimate, off by at most one for row
+  // counts below 2^20, checked at context creation).
+  int r = (int)((float)pc * frcp((float)a.NY));
+  int yc = pc - r * a.NY;
+  if (yc < 0) {
+    --r;
+    yc += a.NY;
+  }
+  if (yc >= a.NY) {
+    ++r;
+    yc -= a.NY;
+  }
+  const int i = a.h + r;
+  const long long idx = (long long)k * a.sz + (long long)a.h * a.sx + pc;
+  const bool out_lane = valid && lane >= KA && lane < KA + OUTA && yc >= a.h - 1 - RAD &&
+                        yc < a.h + a.ny + 1 + RAD;
+  const bool face = valid && yc >= a.h && yc <= a.h + a.ny;
+  const bool rcell = valid && yc >= a.h - 1 && yc <= a.h + a.ny;
+  const bool inner = valid && yc >= a.h && yc < a.h + a.ny;
   const float pe = pe_at(a, k, i, yc);
 
   float W0[NV];
@@ -404,9 +429,8 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
   {
     AdjArgs ay = A;
     ay.s.inv_hdir = 1.0f / h[1];
-    constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
-    const int ext = a.ny + 2 * RAD + 2;
-    const dim3 grid((ext + OUTA - 1) / OUTA, (a.nx + by_y - 1) / by_y, a.nz);
+    const int nwin = y_adj_windows(a.nx, a.ny, a.h, RAD, YAdjGeom<SCHEME>::OUTA);
+    const dim3 grid((nwin + by_y - 1) / by_y, 1, a.nz);
     hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC>), grid, blk_y, 0, stream, ay);
   }
   {
