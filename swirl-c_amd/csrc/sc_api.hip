@@ -36,6 +36,7 @@ struct sc_ctx {
   float* adj_lq[5];
   float* adj_lG[9];
   sc::HaloArgs halo;  // BC tables; field pointers filled per call
+  int halo_all_periodic;  // one-launch periodic fill applies
   sc::SweepArgs sweep;
   sc::SweepTuning tune;
 };
@@ -298,6 +299,11 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
         for (int i = 0; i < SC_MAX_HALO; ++i) o.layer_values[i] = b.layer_values[i];
         o.layer_planes = b.layer_planes;
       }
+  c->halo_all_periodic = ha.prim_mask == 0 && sc::halo_periodic_threads(ha) < (1LL << 31);
+  for (int f = 0; f < c->nf; ++f)
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < 2; ++s)
+        if (p.bc[f][d][s].type != SC_BC_PERIODIC) c->halo_all_periodic = 0;
 
   // Sweep constants.
   sc::SweepArgs& a = c->sweep;
@@ -359,6 +365,7 @@ sc_status sc_fill_halos(sc_ctx* c, float* const* q, void* stream) {
     if (q[f] == nullptr) return fail(SC_ERR_INVALID, "sc_fill_halos: null field pointer");
   hipStream_t s = (hipStream_t)stream;
   fill_halo_fields(c, q);
+  if (c->halo_all_periodic) return hip_check(sc::launch_halo_periodic(c->halo, s), "halo kernel");
   for (int d = 0; d < 3; ++d) {
     sc_status st = hip_check(sc::launch_halo_dim(c->halo, d, s), "halo kernel");
     if (st != SC_OK) return st;

@@ -142,6 +142,53 @@ __global__ void __launch_bounds__(256) halo_dim_kernel(const HaloArgs a) {
   halo_cell_fill<D>(a, g, hc);
 }
 
+// Every field PERIODIC on all six faces, no primitive-variable BC: the x, y, z
+// fill in sequence leaves each halo cell equal to the interior cell at its
+// index wrapped along all three dims (edges and corners included), so one
+// launch copies that directly.  Threads of one field (blockIdx.y): first the
+// full y rows of the z halo planes and of the x halo rows of interior planes,
+// then the 2h y-halo cells of each interior row.  Sources are interior cells
+// and destinations halo cells, so no thread reads another's output.
+__global__ void __launch_bounds__(256)
+halo_periodic_kernel(const HaloArgs a, int full_cells, int total) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= total) return;
+  const int h = a.h, h2 = 2 * a.h;
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int NX = a.N[0], NY = a.N[1];
+  int k, i, j;
+  if (t < full_cells) {
+    const int row = t / NY;
+    j = t - row * NY;
+    const int zrows = h2 * NX;
+    if (row < zrows) {
+      const int kk = row / NX;
+      i = row - kk * NX;
+      k = kk < h ? kk : nz + kk;
+    } else {
+      const int r = row - zrows;
+      const int kk = r / h2;
+      const int sl = r - kk * h2;
+      k = h + kk;
+      i = sl < h ? sl : nx + sl;
+    }
+  } else {
+    const int r = t - full_cells;
+    const int row = r / h2;
+    const int sl = r - row * h2;
+    const int kk = row / nx;
+    k = h + kk;
+    i = h + (row - kk * nx);
+    j = sl < h ? sl : ny + sl;
+  }
+  auto wrap = [h](int c, int n) { return c < h ? c + n : (c >= h + n ? c - n : c); };
+  const long long dst = (long long)k * a.st[2] + (long long)i * a.st[0] + j;
+  const long long src =
+      (long long)wrap(k, nz) * a.st[2] + (long long)wrap(i, nx) * a.st[0] + wrap(j, ny);
+  float* v = a.q[blockIdx.y];
+  v[dst] = v[src];
+}
+
 // Batched: tile (instance, blk) covers threads blk*256 .. of the instance's
 // nf * 2h * A * B halo cells (32-bit: the tile builder checks the count),
 // decoded with ib fastest (D = 0, 2) or (side, layer) fastest (D = 1).
@@ -332,6 +379,19 @@ hipError_t launch_halo_dim(const HaloArgs& a, int d, hipStream_t stream) {
       hipLaunchKernelGGL(halo_dim_kernel<2>, grid, 256, 0, stream, a);
     }
   }
+  return hipGetLastError();
+}
+
+long long halo_periodic_threads(const HaloArgs& a) {
+  const long long full = ((long long)2 * a.h * a.N[0] + (long long)a.n[2] * 2 * a.h) * a.N[1];
+  return full + (long long)a.n[2] * a.n[0] * 2 * a.h;
+}
+
+hipError_t launch_halo_periodic(const HaloArgs& a, hipStream_t stream) {
+  const int full = (int)(((long long)2 * a.h * a.N[0] + (long long)a.n[2] * 2 * a.h) * a.N[1]);
+  const int total = (int)halo_periodic_threads(a);
+  hipLaunchKernelGGL(halo_periodic_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)a.nf),
+                     256, 0, stream, a, full, total);
   return hipGetLastError();
 }
 

@@ -42,6 +42,12 @@ inline long long halo_dim_threads(int nf, int h, const long long N[3], int d) {
 hipError_t launch_halo_dim(const HaloArgs& a, int d, hipStream_t stream);
 hipError_t launch_halo_prim_finish(const HaloArgs& a, hipStream_t stream);
 
+// One-launch fill for grids whose fields are all PERIODIC on every face with
+// no primitive-variable BC (same result as the three dim passes); its thread
+// count must stay below 2^31.
+long long halo_periodic_threads(const HaloArgs& a);
+hipError_t launch_halo_periodic(const HaloArgs& a, hipStream_t stream);
+
 // Batched halo fill: `cfg` carries the shared BC tables (its field pointers
 // and extents are unused); instance geometry comes from `inst` (nx, ny, nz,
 // arena offset) and the tile list (instance, block) of each dim.
