@@ -741,7 +741,8 @@ int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 
 // Sweep tile lists of a batch for its current tuning: x sweep (y chunks of
 // 64, z groups, x segments); y sweep (groups of by_y windows of the z
-// plane's run of rows, z); z sweep (y chunks, x groups, z segments).  Replaces any previous lists.
+// plane's run of rows, z); z sweep (y chunks, x groups, z segments).
+// Replaces any previous lists.
 sc_status build_sweep_tiles(sc_batch* b) {
   const int rad = radius_of(b->params.interpolation);
   const int outw = 64 - 2 * rad - 2;

@@ -390,8 +390,9 @@ long long halo_periodic_threads(const HaloArgs& a) {
 hipError_t launch_halo_periodic(const HaloArgs& a, hipStream_t stream) {
   const int full = (int)(((long long)2 * a.h * a.N[0] + (long long)a.n[2] * 2 * a.h) * a.N[1]);
   const int total = (int)halo_periodic_threads(a);
-  hipLaunchKernelGGL(halo_periodic_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)a.nf),
-                     256, 0, stream, a, full, total);
+  hipLaunchKernelGGL(halo_periodic_kernel,
+                     dim3((unsigned)(((long long)total + 255) / 256), (unsigned)a.nf), 256, 0,
+                     stream, a, full, total);
   return hipGetLastError();
 }
 
