@@ -1,8 +1,10 @@
 """The benchmark configurations at their full sizes, checked through
 size-independent properties (the oracle cannot run them in seconds):
 
-  * Kelvin-Helmholtz 256^3, periodic: mass, momentum and energy conserved to
-    roundoff over 10 RK3 steps;
+  * Kelvin-Helmholtz 256^3, periodic, the workload's 100 RK3 steps at CFL
+    0.5: mass, momentum and energy conserved to fp32 roundoff of the stage
+    updates (fp32(1/3) + fp32(2/3) = 1 + 2.98e-8 scales the totals by that
+    factor per step, 3e-6 over 100 steps, as in the reference's fp32 RK3);
   * buoyant bubble 512 x 512 x 256 (MUSCL + HLL, viscous, gravity, Dirichlet
     z walls), 50 steps: finite, positive density, mass lost only through the
     walls, the warm bubble rises (upward momentum through it) while the
@@ -39,17 +41,20 @@ def test_kelvin_helmholtz_256_conserves_mass_momentum_energy():
                cfl=0.5, use_graph=True)
   del st
   t0, a0 = _totals(sim, h), _abs_totals(sim, h)
-  sim.step(10)
+  ev0 = float((_interior(sim.state['rho_v'], h).double() ** 2).sum())
+  sim.step(100)
   torch.cuda.synchronize()
+  ev1 = float((_interior(sim.state['rho_v'], h).double() ** 2).sum())
   t1 = _totals(sim, h)
   # Momenta against the total momentum magnitude (w starts at zero).
   mom = sum(a0[k] for k in ('rho_u', 'rho_v', 'rho_w'))
   drift = {k: abs(t1[k] - t0[k]) / (mom if k in ('rho_u', 'rho_v', 'rho_w') else a0[k])
            for k in t0}
-  print(f'[full] KH 256^3 x 10 steps, drift / sum|q|: '
-        + ' '.join(f'{k}={v:.1e}' for k, v in drift.items()), flush=True)
+  print(f'[full] KH 256^3 x 100 steps, drift / sum|q|: '
+        + ' '.join(f'{k}={v:.1e}' for k, v in drift.items())
+        + f'; sum (rho v)^2 grew {ev1 / ev0:.3f}x', flush=True)
   assert all(bool(torch.isfinite(v).all()) for v in sim.state.values())
-  assert max(drift.values()) < 1e-6, drift
+  assert max(drift.values()) < 5e-6, drift
 
 
 def test_buoyant_bubble_512x512x256_rises_and_conserves_mass():
