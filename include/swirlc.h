@@ -176,6 +176,22 @@ sc_status sc_ctx_set_reference_state(sc_ctx* ctx, const float* p_ref, const floa
 sc_status sc_max_wave_speed(sc_ctx* ctx, const float* const* q, float cfl,
                             float* out_dev, float* dt_dev, void* stream);
 
+/* Face states of one field: (neg, pos) = left / right states of face i-1/2
+ * stored at index i along dim (0 x, 1 y, 2 z) of the [z][x][y] device array
+ * with extents[3] = {Nx, Ny, Nz} (halos included).  scheme SC_WENO_5 /
+ * SC_WENO_3 is swirl_lm's interpolation.weno(v, dim, k=3 / 2), SC_MUSCL its
+ * flux_limiter(v, dim, MUSCL), as called by _face_interpolation
+ * (equations/compressible_navier_stokes.py:536-615); entries whose stencil
+ * leaves the array keep the cell value (WENO) or use a zero slope (MUSCL).
+ * v, neg and pos must be distinct. */
+sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme,
+                         float* neg, float* pos, void* stream);
+
+/* numerics/interpolation.py:21-53 linear_interpolation: out[i] =
+ * 0.5 * (v[i-1] + v[i]) on face i-1/2 along dim, v[-1] taken as 0. */
+sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,
+                           void* stream);
+
 /* Debug/parity: HLLC branch code of every face normal to dim (0 x, 1 y, 2 z),
  * codes[i] for the face i-1/2 stored at cell i of the full array, interior
  * faces only (hllc.py:227-243: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R).  q's halos

@@ -9,6 +9,7 @@
 #include "../../include/swirlc.h"
 #include "sc_adjoint_impl.hpp"
 #include "sc_halo.hpp"
+#include "sc_interp.hpp"
 #include "sc_launch.hpp"
 
 struct sc_ctx {
@@ -489,6 +490,38 @@ sc_status sc_max_wave_speed(sc_ctx* c, const float* const* q, float cfl, float* 
   return hip_check(sc::launch_max_speed(a, (unsigned*)out_dev, cfl, hmin, dt_dev,
                                         (hipStream_t)stream),
                    "max speed kernel");
+}
+
+sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme, float* neg,
+                         float* pos, void* stream) {
+  if (v == nullptr || extents == nullptr || neg == nullptr || pos == nullptr)
+    return fail(SC_ERR_INVALID, "sc_face_interp: null argument");
+  if (neg == v || pos == v || neg == pos)
+    return fail(SC_ERR_INVALID, "sc_face_interp: v, neg and pos must be distinct arrays");
+  if (dim < 0 || dim > 2)
+    return fail(SC_ERR_INVALID, "sc_face_interp: dim must be 0 (x), 1 (y) or 2 (z)");
+  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
+    return fail(SC_ERR_INVALID, "sc_face_interp: extents must be positive");
+  if (scheme != SC_WENO_3 && scheme != SC_WENO_5 && scheme != SC_MUSCL)
+    return fail(SC_ERR_UNSUPPORTED,
+                "interpolation_scheme is not implemented. Valid options are: "
+                "['WENO_3', 'WENO_5', 'MUSCL']");
+  const int s = scheme == SC_WENO_5 ? sc::kWeno5 : (scheme == SC_WENO_3 ? sc::kWeno3 : sc::kMuscl);
+  return hip_check(sc::launch_face_interp(v, extents, dim, s, neg, pos, (hipStream_t)stream),
+                   "face interpolation kernel");
+}
+
+sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,
+                           void* stream) {
+  if (v == nullptr || extents == nullptr || out == nullptr)
+    return fail(SC_ERR_INVALID, "sc_linear_interp: null argument");
+  if (out == v) return fail(SC_ERR_INVALID, "sc_linear_interp: out must not alias v");
+  if (dim < 0 || dim > 2)
+    return fail(SC_ERR_INVALID, "sc_linear_interp: dim must be 0 (x), 1 (y) or 2 (z)");
+  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
+    return fail(SC_ERR_INVALID, "sc_linear_interp: extents must be positive");
+  return hip_check(sc::launch_linear_interp(v, extents, dim, out, (hipStream_t)stream),
+                   "linear interpolation kernel");
 }
 
 sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_t* codes,

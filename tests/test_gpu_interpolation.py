@@ -1,0 +1,82 @@
+"""The stand-alone interpolation and cell-average BC seams on the GPU against
+the oracle (numerics/interpolation.py:21-53, the swirl_lm weno /
+flux_limiter contract, boundary/boundary.py:464-539).
+
+Bar: WENO within 1e-5 relative L-infinity of the fp32 oracle (the kernels
+build the candidate stencils from differences, so rounding differs); MUSCL,
+linear interpolation and the halo fill bit for bit.  Shapes include lines
+shorter than the WENO stencil and of one or two cells.
+"""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import swirlc_oracle as O
+from swirlc_amd.boundary import boundary
+from swirlc_amd.core.parameter import SwirlCParameters
+from swirlc_amd.numerics import interpolation
+
+from helpers import oracle_cfg, random_state, rel_linf
+import test_gpu_rhs_parity as P
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(9, 12, 11), (3, 7, 64), (1, 2, 5), (4, 1, 70)]
+
+
+def _field(shape, seed):
+  rng = np.random.default_rng(seed)
+  v = 1.0 + 0.3 * rng.standard_normal(shape)
+  v[:, :, shape[2] // 2:] += 0.5  # a jump along y
+  return v.astype(np.float32)
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+@pytest.mark.parametrize('dim', ['x', 'y', 'z'])
+@pytest.mark.parametrize('k', [2, 3])
+def test_weno_matches_oracle(shape, dim, k):
+  v = _field(shape, 1)
+  neg, pos = interpolation.weno(torch.from_numpy(v).cuda(), dim, k)
+  rn, rp = O.weno(v, dim, k)
+  torch.cuda.synchronize()
+  en = rel_linf(neg.cpu().numpy(), rn)
+  ep = rel_linf(pos.cpu().numpy(), rp)
+  assert max(en, ep) <= 1e-5, (en, ep)
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+@pytest.mark.parametrize('dim', ['x', 'y', 'z'])
+def test_muscl_and_linear_interpolation_match_oracle_bitwise(shape, dim):
+  v = _field(shape, 2)
+  t = torch.from_numpy(v).cuda()
+  neg, pos = interpolation.flux_limiter(t, dim, 'MUSCL')
+  lin = interpolation.linear_interpolation(t, dim)
+  rn, rp = O.muscl(v, dim)
+  rl = O.linear_interpolation(v, dim)
+  torch.cuda.synchronize()
+  np.testing.assert_array_equal(neg.cpu().numpy(), rn)
+  np.testing.assert_array_equal(pos.cpu().numpy(), rp)
+  np.testing.assert_array_equal(lin.cpu().numpy(), rl)
+
+
+def test_update_conservative_cell_averages_matches_oracle_bitwise():
+  n, h = (12, 10, 11), 3
+  cfg = P._cfg(n, h, 'WENO_5', 'HLLC', False, 'mixed')
+  st = random_state(n, h, seed=4)
+  ref = O.update_conservative_cell_averages(st, oracle_cfg(cfg))
+  g = {k: torch.from_numpy(v.copy()).cuda() for k, v in st.items()}
+  out = boundary.update_conservative_cell_averages(0, None, g,
+                                                   SwirlCParameters(copy.deepcopy(cfg)))
+  torch.cuda.synchronize()
+  for k in ref:
+    np.testing.assert_array_equal(out[k].cpu().numpy(), ref[k], err_msg=k)
+    np.testing.assert_array_equal(g[k].cpu().numpy(), st[k], err_msg=k)  # input unchanged
+
+
+def test_interpolation_rejects_non_contiguous_input():
+  t = torch.zeros(6, 8, 10, device='cuda').transpose(1, 2)
+  with pytest.raises(ValueError, match='contiguous'):
+    interpolation.weno(t, 'x')

@@ -108,6 +108,34 @@ def test_null_arguments_rejected():
   assert L.sc_rk_stage(None, 1, 0.0, 0.0, None, None, None, None, None) == 1
 
 
+def test_interpolation_entry_points_reject_bad_arguments():
+  """Argument checks of sc_face_interp / sc_linear_interp run before any GPU use."""
+  L = N.lib()
+  ext = (ctypes.c_int * 3)(4, 4, 4)
+  a, b, c = ctypes.c_void_p(16), ctypes.c_void_p(32), ctypes.c_void_p(48)
+  assert L.sc_face_interp(None, ext, 0, N.SC_WENO_5, b, c, None) == 1
+  assert L.sc_face_interp(a, ext, 3, N.SC_WENO_5, b, c, None) == 1
+  assert L.sc_face_interp(a, ext, 0, N.SC_WENO_5, a, c, None) == 1
+  assert L.sc_face_interp(a, ext, 0, 7, b, c, None) == 2
+  assert 'interpolation_scheme' in L.sc_last_error().decode()
+  assert L.sc_face_interp(a, (ctypes.c_int * 3)(4, 0, 4), 0, N.SC_MUSCL, b, c, None) == 1
+  assert L.sc_linear_interp(a, ext, 1, a, None) == 1
+  assert L.sc_linear_interp(a, ext, -1, b, None) == 1
+
+
+def test_interpolation_host_checks():
+  import torch
+  from swirlc_amd.numerics import interpolation
+  with pytest.raises(ValueError, match='valid dimension'):
+    interpolation.weno(torch.zeros(3, 3, 3), 'w')
+  with pytest.raises(ValueError, match='GPU tensor'):
+    interpolation.linear_interpolation(torch.zeros(3, 3, 3), 'x')
+  with pytest.raises(NotImplementedError):
+    interpolation.weno(torch.zeros(3, 3, 3), 'x', k=4)
+  with pytest.raises(NotImplementedError, match='MUSCL'):
+    interpolation.flux_limiter(torch.zeros(3, 3, 3), 'x', 'VAN_LEER')
+
+
 def test_python_config_validation_errors():
   """The Python host raises the reference's exception types before any GPU use."""
   from swirlc_amd.core.parameter import SwirlCParameters
