@@ -192,6 +192,20 @@ sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme
 sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,
                            void* stream);
 
+/* Numerical flux across face i-1/2 (stored at i) normal to dim from given
+ * face states, with the context's flux scheme: hll.hll_convective_flux
+ * (numerics/riemann_solver/hll.py:36-175) or hllc.hllc_convective_flux
+ * (hllc.py:33-257), the physical flux being _compute_physical_flux_1d
+ * (equations/compressible_navier_stokes.py:338-427).  cons_*: the n_fields
+ * conservative face states; prim_*: rho, u, v, w, e, p, h, then the
+ * scalars' primitives; pdyn_*: helper variable p_dyn on the faces or both
+ * NULL.  Full-size device arrays of the context's grid; flux: n_fields
+ * outputs.  The gravitational potential is the context's (cfg.g). */
+sc_status sc_riemann_flux(sc_ctx* ctx, int dim, const float* const* cons_neg,
+                          const float* const* cons_pos, const float* const* prim_neg,
+                          const float* const* prim_pos, const float* pdyn_neg,
+                          const float* pdyn_pos, float* const* flux, void* stream);
+
 /* Debug/parity: HLLC branch code of every face normal to dim (0 x, 1 y, 2 z),
  * codes[i] for the face i-1/2 stored at cell i of the full array, interior
  * faces only (hllc.py:227-243: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R).  q's halos

@@ -10,6 +10,7 @@
 #include "sc_adjoint_impl.hpp"
 #include "sc_halo.hpp"
 #include "sc_interp.hpp"
+#include "sc_riemann.hpp"
 #include "sc_launch.hpp"
 
 struct sc_ctx {
@@ -522,6 +523,45 @@ sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* o
     return fail(SC_ERR_INVALID, "sc_linear_interp: extents must be positive");
   return hip_check(sc::launch_linear_interp(v, extents, dim, out, (hipStream_t)stream),
                    "linear interpolation kernel");
+}
+
+sc_status sc_riemann_flux(sc_ctx* c, int dim, const float* const* cons_neg,
+                         const float* const* cons_pos, const float* const* prim_neg,
+                         const float* const* prim_pos, const float* pdyn_neg,
+                         const float* pdyn_pos, float* const* flux, void* stream) {
+  if (c == nullptr || cons_neg == nullptr || cons_pos == nullptr || prim_neg == nullptr ||
+      prim_pos == nullptr || flux == nullptr)
+    return fail(SC_ERR_INVALID, "sc_riemann_flux: null argument");
+  if (dim < 0 || dim > 2)
+    return fail(SC_ERR_INVALID, "sc_riemann_flux: dim must be 0 (x), 1 (y) or 2 (z)");
+  if ((pdyn_neg == nullptr) != (pdyn_pos == nullptr))
+    return fail(SC_ERR_INVALID, "sc_riemann_flux: give p_dyn on both sides or neither");
+  sc::RiemannArgs a{};
+  const int nv = 7 + (c->nf - 5);
+  for (int f = 0; f < c->nf; ++f) {
+    if (cons_neg[f] == nullptr || cons_pos[f] == nullptr || flux[f] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_riemann_flux: null field pointer");
+    a.cl[f] = cons_neg[f];
+    a.cr[f] = cons_pos[f];
+    a.out[f] = flux[f];
+  }
+  for (int v = 0; v < nv; ++v) {
+    if (prim_neg[v] == nullptr || prim_pos[v] == nullptr)
+      return fail(SC_ERR_INVALID, "sc_riemann_flux: null primitive pointer");
+    a.wl[v] = prim_neg[v];
+    a.wr[v] = prim_pos[v];
+  }
+  a.pdl = pdyn_neg;
+  a.pdr = pdyn_pos;
+  for (int d = 0; d < 3; ++d) a.N[d] = c->N[d];
+  a.total = c->cells_full;
+  a.has_pe = c->has_pe;
+  a.pe_x = c->pe_x;
+  a.pe_y = c->pe_y;
+  a.pe_z = c->pe_z;
+  const int flux_kind = c->params.numeric_flux == SC_HLLC ? sc::kHllc : sc::kHll;
+  return hip_check(sc::launch_riemann_flux(a, dim, flux_kind, c->nf - 5, (hipStream_t)stream),
+                   "riemann flux kernel");
 }
 
 sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_t* codes,

@@ -29,7 +29,8 @@ EXPORTED = (
     'sc_batch_create', 'sc_batch_destroy', 'sc_batch_fill_halos', 'sc_batch_rk_stage',
     'sc_batch_max_wave_speed', 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
     'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning', 'sc_batch_set_tuning',
-    'sc_ctx_set_reference_state', 'sc_face_interp', 'sc_linear_interp', 'sc_last_error',
+    'sc_ctx_set_reference_state', 'sc_face_interp', 'sc_linear_interp', 'sc_riemann_flux',
+    'sc_last_error',
     'sc_version',
 )
 
@@ -139,6 +140,7 @@ def lib():
     ip3 = ctypes.POINTER(ctypes.c_int)
     L.sc_face_interp.argtypes = [vp, ip3, i, i, vp, vp, vp]
     L.sc_linear_interp.argtypes = [vp, ip3, i, vp, vp]
+    L.sc_riemann_flux.argtypes = [vp, i, fpp, fpp, fpp, fpp, vp, vp, fpp, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_batch_create.argtypes = [ctypes.POINTER(ScParams), i, i, ctypes.POINTER(ScInstance),
@@ -164,7 +166,7 @@ def lib():
                  'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
                  'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning',
                  'sc_batch_set_tuning', 'sc_ctx_set_reference_state', 'sc_face_interp',
-                 'sc_linear_interp'):
+                 'sc_linear_interp', 'sc_riemann_flux'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

@@ -1,0 +1,67 @@
+"""Shared driver of the stand-alone HLL / HLLC functions (sc_riemann_flux)."""
+
+import copy
+import ctypes
+
+import torch
+
+from swirlc_amd import _native as N
+from swirlc_amd.common import types
+from swirlc_amd.equations.native_context import NativeContext
+
+_DIMS = {'x': 0, 'y': 1, 'z': 2}
+_BASE_PRIM = ('rho', 'u', 'v', 'w', 'e', 'p', 'h')
+
+
+def _ptrs(tensors, n):
+  arr = (ctypes.c_void_p * n)()
+  for k, t in enumerate(tensors):
+    N.require_gpu_tensor(t, 'face state')
+    arr[k] = t.data_ptr()
+  return arr
+
+
+def _p_dyn(helper, dim):
+  """p_dyn of the faces normal to dim, from a helper-variable dict keyed by
+  dimension (the reference's layout) or a flat one; None when absent."""
+  if not helper:
+    return None
+  h = helper.get(dim, helper) if isinstance(helper, dict) else helper
+  return h.get('p_dyn') if isinstance(h, dict) else None
+
+
+def numerical_flux(scheme, conservative_neg, conservative_pos, primitive_neg, primitive_pos,
+                   helper_var_neg, helper_var_pos, dim, cfg):
+  """Flux dict across face i-1/2 (stored at i) normal to dim, scheme 'HLL' / 'HLLC'."""
+  if dim not in _DIMS:
+    raise ValueError(f'"{dim}" is not a valid dimension. Available options are: "x", "y", "z".')
+  c = copy.copy(cfg)
+  c.numeric_flux_scheme = scheme
+  ctx = NativeContext(c)
+  names = ctx.names
+  prim_names = list(_BASE_PRIM) + [types.conservative_to_primitive_name(n) for n in names[5:]]
+  for d, what in ((conservative_neg, 'conservative_neg'), (conservative_pos, 'conservative_pos')):
+    missing = [n for n in names if n not in d]
+    if missing:
+      raise KeyError(f'{what} lacks {missing}')
+  for d, what in ((primitive_neg, 'primitive_neg'), (primitive_pos, 'primitive_pos')):
+    missing = [n for n in prim_names if n not in d]
+    if missing:
+      raise KeyError(f'{what} lacks {missing}')
+  pdn, pdp = _p_dyn(helper_var_neg, dim), _p_dyn(helper_var_pos, dim)
+  if (pdn is None) != (pdp is None):
+    raise ValueError('p_dyn must be given on both sides or on neither')
+  for t in (pdn, pdp):
+    if t is not None:
+      N.require_gpu_tensor(t, 'p_dyn')
+  out = {n: torch.empty_like(conservative_neg[n]) for n in names}
+  N.check(N.lib().sc_riemann_flux(
+      ctx._ctx, _DIMS[dim],
+      _ptrs([conservative_neg[n] for n in names], N.SC_MAX_FIELDS),
+      _ptrs([conservative_pos[n] for n in names], N.SC_MAX_FIELDS),
+      _ptrs([primitive_neg[n] for n in prim_names], N.SC_MAX_FIELDS + 2),
+      _ptrs([primitive_pos[n] for n in prim_names], N.SC_MAX_FIELDS + 2),
+      ctypes.c_void_p(pdn.data_ptr()) if pdn is not None else None,
+      ctypes.c_void_p(pdp.data_ptr()) if pdp is not None else None,
+      _ptrs([out[n] for n in names], N.SC_MAX_FIELDS), N.stream_handle()), 'sc_riemann_flux')
+  return out
