@@ -206,6 +206,21 @@ sc_status sc_riemann_flux(sc_ctx* ctx, int dim, const float* const* cons_neg,
                           const float* const* prim_pos, const float* pdyn_neg,
                           const float* pdyn_pos, float* const* flux, void* stream);
 
+/* physics/diffusion.py:33-67 shear_stress with fluid.py:31-111 strain rate
+ * and TransportSimple (simple.py:44-79, constant kinematic nu):
+ * tau[3 i + j] = 2 rho_face[j] nu S_ij on the j faces (face i-1/2 stored at
+ * i).  u, v, w: cell velocities; rho_face[j]: density on the j faces; h:
+ * dx, dy, dz; full [z][x][y] device arrays of extents {Nx, Ny, Nz}. */
+sc_status sc_shear_stress(const float* u, const float* v, const float* w,
+                          const float* const* rho_face, const int* extents, const double* h,
+                          double nu, float* const* tau, void* stream);
+
+/* physics/diffusion.py:76-109 single_component_heat_flux with
+ * TransportSimple (simple.py:81-108): flux[d] = -kappa_f dT/dx_d on the d
+ * faces, T = p / (rho R), kappa = (1 / pr) cp nu rho linearly interpolated. */
+sc_status sc_heat_flux(const float* rho, const float* p, const int* extents, const double* h,
+                       double nu, double pr, float* const* flux, void* stream);
+
 /* Debug/parity: HLLC branch code of every face normal to dim (0 x, 1 y, 2 z),
  * codes[i] for the face i-1/2 stored at cell i of the full array, interior
  * faces only (hllc.py:227-243: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R).  q's halos

@@ -9,6 +9,7 @@
 #include "../../include/swirlc.h"
 #include "sc_adjoint_impl.hpp"
 #include "sc_halo.hpp"
+#include "sc_diffusion.hpp"
 #include "sc_interp.hpp"
 #include "sc_riemann.hpp"
 #include "sc_launch.hpp"
@@ -562,6 +563,63 @@ sc_status sc_riemann_flux(sc_ctx* c, int dim, const float* const* cons_neg,
   const int flux_kind = c->params.numeric_flux == SC_HLLC ? sc::kHllc : sc::kHll;
   return hip_check(sc::launch_riemann_flux(a, dim, flux_kind, c->nf - 5, (hipStream_t)stream),
                    "riemann flux kernel");
+}
+
+namespace {
+sc_status check_extents_spacing(const int* extents, const double* h, const char* what) {
+  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
+    return fail(SC_ERR_INVALID, std::string(what) + ": extents must be positive");
+  if (!(h[0] > 0 && h[1] > 0 && h[2] > 0))
+    return fail(SC_ERR_INVALID, std::string(what) + ": grid spacing must be positive");
+  return SC_OK;
+}
+}  // namespace
+
+sc_status sc_shear_stress(const float* u, const float* v, const float* w,
+                          const float* const* rho_face, const int* extents, const double* h,
+                          double nu, float* const* tau, void* stream) {
+  if (u == nullptr || v == nullptr || w == nullptr || rho_face == nullptr ||
+      extents == nullptr || h == nullptr || tau == nullptr)
+    return fail(SC_ERR_INVALID, "sc_shear_stress: null argument");
+  sc_status st = check_extents_spacing(extents, h, "sc_shear_stress");
+  if (st != SC_OK) return st;
+  sc::ShearArgs a{};
+  a.u = u;
+  a.v = v;
+  a.w = w;
+  for (int d = 0; d < 3; ++d) {
+    if (rho_face[d] == nullptr) return fail(SC_ERR_INVALID, "sc_shear_stress: null face density");
+    a.rho_f[d] = rho_face[d];
+    a.N[d] = extents[d];
+    a.h[d] = (float)h[d];
+  }
+  for (int t = 0; t < 9; ++t) {
+    if (tau[t] == nullptr) return fail(SC_ERR_INVALID, "sc_shear_stress: null output");
+    a.tau[t] = tau[t];
+  }
+  a.nu = (float)nu;
+  return hip_check(sc::launch_shear_stress(a, (hipStream_t)stream), "shear stress kernel");
+}
+
+sc_status sc_heat_flux(const float* rho, const float* p, const int* extents, const double* h,
+                       double nu, double pr, float* const* flux, void* stream) {
+  if (rho == nullptr || p == nullptr || extents == nullptr || h == nullptr || flux == nullptr)
+    return fail(SC_ERR_INVALID, "sc_heat_flux: null argument");
+  sc_status st = check_extents_spacing(extents, h, "sc_heat_flux");
+  if (st != SC_OK) return st;
+  if (!(pr > 0)) return fail(SC_ERR_INVALID, "sc_heat_flux: pr must be positive");
+  sc::HeatArgs a{};
+  a.rho = rho;
+  a.p = p;
+  for (int d = 0; d < 3; ++d) {
+    if (flux[d] == nullptr) return fail(SC_ERR_INVALID, "sc_heat_flux: null output");
+    a.flux[d] = flux[d];
+    a.N[d] = extents[d];
+    a.h[d] = (float)h[d];
+  }
+  a.nu = (float)nu;
+  a.inv_pr = (float)(1.0 / pr);
+  return hip_check(sc::launch_heat_flux(a, (hipStream_t)stream), "heat flux kernel");
 }
 
 sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_t* codes,

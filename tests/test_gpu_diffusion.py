@@ -1,0 +1,58 @@
+"""The stand-alone diffusive terms on the GPU against the oracle
+(physics/diffusion.py:33-109, fluid.py:31-111, simple.py:44-108).
+
+The kernels round each operation in the oracle's order (contraction off,
+IEEE division), so the bar is bit-for-bit equality over the whole array,
+halo and edge cells included (neighbours outside the array count as 0 on
+both sides).
+"""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import swirlc_oracle as O
+from swirlc_amd.core.parameter import SwirlCParameters
+from swirlc_amd.physics import diffusion
+
+from helpers import grid_cfg, oracle_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def _prims(shape, seed):
+  rng = np.random.default_rng(seed)
+  f = lambda s, o: (o + s * rng.standard_normal(shape)).astype(np.float32)
+  return {'rho': f(0.2, 1.2), 'u': f(0.3, 0.1), 'v': f(0.3, -0.2), 'w': f(0.3, 0.05),
+          'p': f(0.3, 2.5)}
+
+
+def _gpu(a):
+  return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize('n', [(8, 9, 10), (5, 3, 12)])
+def test_shear_stress_and_heat_flux_match_oracle_bitwise(n):
+  cfg = grid_cfg(n, 3, (1.0, 0.7, 1.3))
+  cfg['transport_parameters'] = {'nu': 2e-3, 'pr': 0.7}
+  ocfg = oracle_cfg(cfg)
+  shape = tuple(x + 6 for x in (n[2], n[0], n[1]))
+  prim = _prims(shape, 3)
+  prim_f = {d: {'rho': O.linear_interpolation(prim['rho'], d)} for d in O.DIMS}
+  ref_tau = O.shear_stress(prim, prim_f, ocfg)
+  ref_q = O.heat_flux(prim, ocfg)
+  c = SwirlCParameters(copy.deepcopy(cfg))
+  g = {k: _gpu(v) for k, v in prim.items()}
+  tau = diffusion.shear_stress(g, {'rho': _gpu(prim_f['x']['rho'])},
+                               {'rho': _gpu(prim_f['y']['rho'])},
+                               {'rho': _gpu(prim_f['z']['rho'])}, c)
+  q = diffusion.single_component_heat_flux(g, c)
+  torch.cuda.synchronize()
+  for i in range(3):
+    for j in range(3):
+      np.testing.assert_array_equal(tau[i][j].cpu().numpy(), ref_tau[i][j],
+                                    err_msg=f'tau[{i}][{j}]')
+  for d in O.DIMS:
+    np.testing.assert_array_equal(q[d].cpu().numpy(), ref_q[d], err_msg=f'q[{d}]')
