@@ -221,6 +221,13 @@ sc_status sc_shear_stress(const float* u, const float* v, const float* w,
 sc_status sc_heat_flux(const float* rho, const float* p, const int* extents, const double* h,
                        double nu, double pr, float* const* flux, void* stream);
 
+/* boundary/boundary.py:34-86 exchange_halos for one field of one replica:
+ * fills the halo layers of u ([nz+2h][nx+2h][ny+2h], extents = interior
+ * {nx, ny, nz}, h = halo) in place from bc[2 * dim + side] (dim 0 x, 1 y,
+ * 2 z; side 0 low, 1 high), x then y then z as swirl_lm's exchange. */
+sc_status sc_exchange_halos(float* u, const int* extents, int halo, const sc_bc_face* bc,
+                            void* stream);
+
 /* Debug/parity: HLLC branch code of every face normal to dim (0 x, 1 y, 2 z),
  * codes[i] for the face i-1/2 stored at cell i of the full array, interior
  * faces only (hllc.py:227-243: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R).  q's halos

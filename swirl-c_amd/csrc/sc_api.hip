@@ -622,6 +622,54 @@ sc_status sc_heat_flux(const float* rho, const float* p, const int* extents, con
   return hip_check(sc::launch_heat_flux(a, (hipStream_t)stream), "heat flux kernel");
 }
 
+sc_status sc_exchange_halos(float* u, const int* extents, int halo, const sc_bc_face* bc,
+                           void* stream) {
+  if (u == nullptr || extents == nullptr || bc == nullptr)
+    return fail(SC_ERR_INVALID, "sc_exchange_halos: null argument");
+  if (halo < 1 || halo > SC_MAX_HALO)
+    return fail(SC_ERR_INVALID, "sc_exchange_halos: halo width must be 1..4");
+  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
+    return fail(SC_ERR_INVALID, "sc_exchange_halos: interior extents must be positive");
+  sc::HaloArgs ha{};
+  ha.q[0] = u;
+  ha.nf = 1;
+  ha.h = halo;
+  for (int d = 0; d < 3; ++d) {
+    ha.n[d] = extents[d];
+    ha.N[d] = extents[d] + 2 * halo;
+  }
+  ha.st[0] = ha.N[1];
+  ha.st[1] = 1;
+  ha.st[2] = (long long)ha.N[0] * ha.N[1];
+  bool all_periodic = true;
+  for (int d = 0; d < 3; ++d)
+    for (int sd = 0; sd < 2; ++sd) {
+      const sc_bc_face& b = bc[2 * d + sd];
+      if (b.type < SC_BC_PERIODIC || b.type > SC_BC_NEUMANN)
+        return fail(SC_ERR_UNSUPPORTED, "sc_exchange_halos: unsupported BC type");
+      if (b.type == SC_BC_PERIODIC && extents[d] < halo)
+        return fail(SC_ERR_UNSUPPORTED,
+                    "PERIODIC BC needs at least halo_width interior cells");
+      if (b.n_layer_values != 0 && b.n_layer_values != halo)
+        return fail(SC_ERR_INVALID, "per-layer DIRICHLET values must have halo_width entries");
+      sc::BcFaceDev& o = ha.bc[0][d][sd];
+      o.type = b.type;
+      o.value = b.value;
+      o.n_layer_values = b.n_layer_values;
+      for (int i = 0; i < SC_MAX_HALO; ++i) o.layer_values[i] = b.layer_values[i];
+      o.layer_planes = b.layer_planes;
+      if (b.type != SC_BC_PERIODIC) all_periodic = false;
+    }
+  hipStream_t st = (hipStream_t)stream;
+  if (all_periodic && sc::halo_periodic_threads(ha) < (1LL << 31))
+    return hip_check(sc::launch_halo_periodic(ha, st), "halo kernel");
+  for (int d = 0; d < 3; ++d) {
+    sc_status r = hip_check(sc::launch_halo_dim(ha, d, st), "halo kernel");
+    if (r != SC_OK) return r;
+  }
+  return SC_OK;
+}
+
 sc_status sc_hllc_branch_codes(sc_ctx* c, const float* const* q, int dim, uint8_t* codes,
                                void* stream) {
   if (c == nullptr || q == nullptr || codes == nullptr)

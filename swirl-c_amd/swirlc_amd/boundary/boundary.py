@@ -1,5 +1,7 @@
-"""Cell-average boundary update on the GPU (boundary/boundary.py).
+"""Halo updates on the GPU (boundary/boundary.py).
 
+  exchange_halos(replica_id, replicas, u, bc_dict, width)
+      boundary/boundary.py:34-86
   update_conservative_cell_averages(replica_id, replicas, conservative, cfg)
       boundary/boundary.py:464-539
 
@@ -13,9 +15,53 @@ one replica.  Each call builds a native context for cfg (device allocations
 included); code that fills halos every step keeps one NativeContext instead.
 """
 
+import ctypes
+
 import torch
 
+from swirlc_amd import _native as N
 from swirlc_amd.equations.native_context import NativeContext
+
+
+class _OneField:
+  """The BC translation of NativeContext for one field of a given shape."""
+  _plane_shape = NativeContext._plane_shape
+  _layer_plane = NativeContext._layer_plane
+  _fill_dim_bc = NativeContext._fill_dim_bc
+
+  def __init__(self, shape, hw, device):
+    self.shape = tuple(shape)
+    self.hw = hw
+    self.device = device
+    self._keep = []
+
+
+def exchange_halos(replica_id, replicas, u, bc_dict, width):
+  """New tensor: u with its `width` halo layers filled from `bc_dict`
+  ({'x' | 'y' | 'z': {0 | 1: (BoundaryCondition, value)}}: PERIODIC,
+  DIRICHLET with a scalar, per-layer scalars or planes, NEUMANN), x then y
+  then z as swirl_lm's halo exchange.  One replica."""
+  del replica_id, replicas
+  N.require_gpu_tensor(u, 'u')
+  if u.dim() != 3:
+    raise ValueError(f'expected a 3-D field [nz, nx, ny], got shape {tuple(u.shape)}')
+  hw = int(width)
+  nz, nx, ny = (int(x) - 2 * hw for x in u.shape)
+  if min(nx, ny, nz) < 1:
+    raise ValueError('the field has no interior cells for this halo width')
+  t = _OneField(u.shape, hw, u.device)
+  faces = (N.ScBcFace * 6)()
+  for d, dim in enumerate(('x', 'y', 'z')):
+    if dim not in bc_dict:
+      raise ValueError(f'bc_dict lacks dimension {dim}')
+    t._fill_dim_bc((faces[2 * d], faces[2 * d + 1]), bc_dict[dim], d)
+  out = u.clone()
+  N.check(N.lib().sc_exchange_halos(ctypes.c_void_p(out.data_ptr()),
+                                    (ctypes.c_int * 3)(nx, ny, nz), hw, faces,
+                                    N.stream_handle()), 'sc_exchange_halos')
+  if t._keep:
+    torch.cuda.current_stream().synchronize()  # layer planes stay alive until used
+  return out
 
 
 def update_conservative_cell_averages(replica_id, replicas, conservative, cfg):

@@ -1,6 +1,6 @@
-"""The stand-alone interpolation and cell-average BC seams on the GPU against
-the oracle (numerics/interpolation.py:21-53, the swirl_lm weno /
-flux_limiter contract, boundary/boundary.py:464-539).
+"""The stand-alone interpolation and halo seams on the GPU against the oracle
+(numerics/interpolation.py:21-53, the swirl_lm weno / flux_limiter
+contract, boundary/boundary.py:34-86 and 464-539).
 
 Bar: WENO within 1e-5 relative L-infinity of the fp32 oracle (the kernels
 build the candidate stencils from differences, so rounding differs); MUSCL,
@@ -15,6 +15,7 @@ import pytest
 import torch
 
 from oracle import swirlc_oracle as O
+from swirlc_amd.boundary import bc_types
 from swirlc_amd.boundary import boundary
 from swirlc_amd.core.parameter import SwirlCParameters
 from swirlc_amd.numerics import interpolation
@@ -23,6 +24,7 @@ from helpers import oracle_cfg, random_state, rel_linf
 import test_gpu_rhs_parity as P
 
 pytestmark = pytest.mark.gpu
+BC = bc_types.BoundaryCondition
 
 SHAPES = [(9, 12, 11), (3, 7, 64), (1, 2, 5), (4, 1, 70)]
 
@@ -80,3 +82,32 @@ def test_interpolation_rejects_non_contiguous_input():
   t = torch.zeros(6, 8, 10, device='cuda').transpose(1, 2)
   with pytest.raises(ValueError, match='contiguous'):
     interpolation.weno(t, 'x')
+
+
+def _bc_cases(n, h):
+  nx, ny, nz = n
+  plane_z = [np.full((nx + 2 * h, ny + 2 * h), 0.5 + 0.1 * k, np.float32) for k in range(h)]
+  return {
+      'mixed': {'x': {0: (BC.DIRICHLET, 1.5), 1: (BC.NEUMANN, 0.02)},
+                'y': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
+                'z': {0: (BC.DIRICHLET, plane_z), 1: (BC.NEUMANN, -0.01)}},
+      'layers': {'x': {0: (BC.NEUMANN, 0.0), 1: (BC.DIRICHLET, [0.1 * (k + 1) for k in range(h)])},
+                 'y': {0: (BC.DIRICHLET, 0.0), 1: (BC.NEUMANN, 0.1)},
+                 'z': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)}},
+      'periodic': {d: {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)} for d in 'xyz'},
+  }
+
+
+@pytest.mark.parametrize('n,h', [((7, 6, 5), 3), ((4, 9, 3), 2), ((5, 4, 6), 4)])
+@pytest.mark.parametrize('case', ['mixed', 'layers', 'periodic'])
+def test_exchange_halos_matches_oracle_bitwise(n, h, case):
+  nx, ny, nz = n
+  bc = _bc_cases(n, h)[case]
+  u = np.random.default_rng(8).standard_normal((nz + 2 * h, nx + 2 * h, ny + 2 * h))
+  u = u.astype(np.float32)
+  ref = O.exchange_halos(u, bc, h)
+  t = torch.from_numpy(u.copy()).cuda()
+  got = boundary.exchange_halos(0, None, t, bc, h)
+  torch.cuda.synchronize()
+  np.testing.assert_array_equal(got.cpu().numpy(), ref)
+  np.testing.assert_array_equal(t.cpu().numpy(), u)  # input unchanged
