@@ -187,6 +187,12 @@ sc_status sc_max_wave_speed(sc_ctx* ctx, const float* const* q, float cfl,
 sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme,
                          float* neg, float* pos, void* stream);
 
+/* Dirichlet overwrite of one boundary plane, coordinate[dim] == index, of a
+ * [z][x][y] device array of extents {Nx, Ny, Nz} with value: the plane
+ * write of update_fluxes / update_flux_1d (boundary/boundary.py:89-183). */
+sc_status sc_fill_plane(float* f, const int* extents, int dim, int index, float value,
+                        void* stream);
+
 /* numerics/interpolation.py:21-53 linear_interpolation: out[i] =
  * 0.5 * (v[i-1] + v[i]) on face i-1/2 along dim, v[-1] taken as 0. */
 sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,

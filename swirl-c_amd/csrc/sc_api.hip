@@ -513,6 +513,19 @@ sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme
                    "face interpolation kernel");
 }
 
+sc_status sc_fill_plane(float* f, const int* extents, int dim, int index, float value,
+                        void* stream) {
+  if (f == nullptr || extents == nullptr) return fail(SC_ERR_INVALID, "sc_fill_plane: null argument");
+  if (dim < 0 || dim > 2)
+    return fail(SC_ERR_INVALID, "sc_fill_plane: dim must be 0 (x), 1 (y) or 2 (z)");
+  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
+    return fail(SC_ERR_INVALID, "sc_fill_plane: extents must be positive");
+  if (index < 0 || index >= extents[dim])
+    return fail(SC_ERR_INVALID, "sc_fill_plane: plane index outside the array");
+  return hip_check(sc::launch_fill_plane(f, extents, dim, index, value, (hipStream_t)stream),
+                   "plane fill kernel");
+}
+
 sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,
                            void* stream) {
   if (v == nullptr || extents == nullptr || out == nullptr)

@@ -74,7 +74,36 @@ linear_interp_kernel(const float* __restrict__ v, LineGeom g, float* __restrict_
   out[idx] = 0.5f * (prev + v[idx]);
 }
 
+// f = value on the plane coordinate[dim] == index (one thread per plane cell).
+__global__ void __launch_bounds__(256)
+fill_plane_kernel(float* f, int nx, int ny, int nz, int dim, int index, float value) {
+  const int na = dim == 0 ? nz : (dim == 1 ? nz : nx);  // slow plane axis
+  const int nb = dim == 1 ? nx : ny;                     // fast plane axis
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)na * nb) return;
+  const int ia = (int)(t / nb), ib = (int)(t % nb);
+  int x, y, z;
+  if (dim == 0) {
+    x = index; z = ia; y = ib;
+  } else if (dim == 1) {
+    y = index; z = ia; x = ib;
+  } else {
+    z = index; x = ia; y = ib;
+  }
+  f[((long long)z * nx + x) * ny + y] = value;
+}
+
 }  // namespace
+
+hipError_t launch_fill_plane(float* f, const int N[3], int dim, int index, float value,
+                             hipStream_t stream) {
+  const int na = dim == 2 ? N[0] : N[2];
+  const int nb = dim == 1 ? N[0] : N[1];
+  const long long n = (long long)na * nb;
+  hipLaunchKernelGGL(fill_plane_kernel, dim3((unsigned)((n + 255) / 256)), 256, 0, stream, f,
+                     N[0], N[1], N[2], dim, index, value);
+  return hipGetLastError();
+}
 
 hipError_t launch_face_interp(const float* v, const int N[3], int dim, int scheme, float* neg,
                               float* pos, hipStream_t stream) {

@@ -12,6 +12,9 @@ namespace sc {
 // kWeno5, kWeno3 or kMuscl.  Arguments are validated by the caller.
 hipError_t launch_face_interp(const float* v, const int N[3], int dim, int scheme, float* neg,
                               float* pos, hipStream_t stream);
+// f = value on the plane coordinate[dim] == index of the [z][x][y] array.
+hipError_t launch_fill_plane(float* f, const int N[3], int dim, int index, float value,
+                             hipStream_t stream);
 // out[i] = 0.5 * (v[i-1] + v[i]) along dim, v[-1] = 0.
 hipError_t launch_linear_interp(const float* v, const int N[3], int dim, float* out,
                                 hipStream_t stream);

@@ -2,6 +2,8 @@
 
   exchange_halos(replica_id, replicas, u, bc_dict, width)
       boundary/boundary.py:34-86
+  update_fluxes(replica_id, replicas, fluxes, cfg, flux_type)
+      boundary/boundary.py:89-183
   update_conservative_cell_averages(replica_id, replicas, conservative, cfg)
       boundary/boundary.py:464-539
 
@@ -83,4 +85,57 @@ def update_conservative_cell_averages(replica_id, replicas, conservative, cfg):
     if f > 0 and (ctx.prim_bc_mask >> f) & 1:
       q = out[name][core]
       q.copy_(rho * torch.where(rho == 0, torch.zeros_like(q), q / rho))
+  return out
+
+
+def _flux_spec(cfg, flux_type):
+  bc = cfg.bc or {}
+  table = bc.get('intercell_fluxes') or {}
+  for key in (flux_type, getattr(flux_type, 'name', None), getattr(flux_type, 'value', None)):
+    if key is not None:
+      for k in (key, str(key).lower()):
+        if k in table:
+          return table[k]
+  return None
+
+
+def update_fluxes(replica_id, replicas, fluxes, cfg, flux_type):
+  """Fluxes {dim: {var: tensor}} with the DIRICHLET values of
+  cfg.bc['intercell_fluxes'][flux_type] written on the boundary flux planes
+  (index halo_width on the low face, halo_width + core_n on the high face);
+  unchanged when no such entry exists.  Only conservative variables may
+  carry a flux BC (AssertionError, as in the reference); a None type is
+  skipped; values must be scalars.  Modified fields are new tensors."""
+  del replica_id, replicas
+  spec = _flux_spec(cfg, flux_type)
+  if spec is None:
+    return fluxes
+  names = list(cfg.conservative_variable_names)
+  bad = set(spec) - set(names)
+  assert not bad, (f'{bad} cannot be specified by a flux boundary condition. Only '
+                   f'conservative variables {set(names)} can be specified with a flux '
+                   'boundary condition.')
+  hw = int(cfg.halo_width)
+  core_n = {'x': cfg.core_nx, 'y': cfg.core_ny, 'z': cfg.core_nz}
+  out = {dim: dict(v) for dim, v in fluxes.items()}
+  for d, dim in enumerate(('x', 'y', 'z')):
+    for side in (0, 1):
+      for var, faces in spec.items():
+        entry = faces[dim][side]
+        kind = getattr(entry[0], 'name', entry[0])
+        if kind is None:
+          continue
+        assert kind == 'DIRICHLET', f'Unsupported interface (face or flux) BC type {kind}.'
+        val = entry[1]
+        if not isinstance(val, (int, float)):
+          raise NotImplementedError('flux boundary values must be scalars')
+        f = out[dim][var].clone()
+        N.require_gpu_tensor(f, 'flux')
+        nz, nx, ny = f.shape
+        index = hw if side == 0 else hw + int(core_n[dim])
+        N.check(N.lib().sc_fill_plane(ctypes.c_void_p(f.data_ptr()),
+                                      (ctypes.c_int * 3)(nx, ny, nz), d, index,
+                                      ctypes.c_float(float(val)), N.stream_handle()),
+                'sc_fill_plane')
+        out[dim][var] = f
   return out

@@ -111,3 +111,23 @@ def test_exchange_halos_matches_oracle_bitwise(n, h, case):
   torch.cuda.synchronize()
   np.testing.assert_array_equal(got.cpu().numpy(), ref)
   np.testing.assert_array_equal(t.cpu().numpy(), u)  # input unchanged
+
+
+@pytest.mark.parametrize('flux_type', ['convective', 'diffusive', 'total', 'none'])
+def test_update_fluxes_matches_oracle_bitwise(flux_type):
+  import test_gpu_boundary_values as BV
+  n, h = (12, 10, 9), 3
+  cfg = BV._cfg(n, 'WENO_5', 'HLLC', True, fluxes=BV._flux_spec())
+  ocfg = oracle_cfg(cfg)
+  rng = np.random.default_rng(11)
+  shape = (n[2] + 2 * h, n[0] + 2 * h, n[1] + 2 * h)
+  names = ocfg['conservative_variable_names']
+  fl = {d: {v: rng.standard_normal(shape).astype(np.float32) for v in names} for d in O.DIMS}
+  ref = O.update_fluxes({d: dict(v) for d, v in fl.items()}, ocfg, flux_type)
+  g = {d: {v: torch.from_numpy(a.copy()).cuda() for v, a in fv.items()} for d, fv in fl.items()}
+  out = boundary.update_fluxes(0, None, g, SwirlCParameters(copy.deepcopy(cfg)), flux_type)
+  torch.cuda.synchronize()
+  for d in O.DIMS:
+    for v in names:
+      np.testing.assert_array_equal(out[d][v].cpu().numpy(), ref[d][v], err_msg=f'{d}/{v}')
+      np.testing.assert_array_equal(g[d][v].cpu().numpy(), fl[d][v])  # input unchanged
