@@ -136,6 +136,32 @@ def test_interpolation_host_checks():
     interpolation.flux_limiter(torch.zeros(3, 3, 3), 'x', 'VAN_LEER')
 
 
+def test_boundary_and_riemann_seams_host_checks():
+  """The stand-alone seams validate their arguments before any GPU use."""
+  import torch
+  from swirlc_amd.boundary import bc_types, boundary
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.numerics.riemann_solver import riemann_solver
+  BC = bc_types.BoundaryCondition
+  with pytest.raises(ValueError, match='GPU tensor'):
+    boundary.exchange_halos(0, None, torch.zeros(8, 8, 8), {}, 2)
+  spec = {'u': {d: {0: (BC.DIRICHLET, 0.0), 1: (None, None)} for d in 'xyz'}}
+  cfg = SwirlCParameters({'bc': {'intercell_fluxes': {'convective': spec}}})
+  with pytest.raises(AssertionError, match='flux boundary condition'):
+    boundary.update_fluxes(0, None, {}, cfg, 'convective')
+  assert boundary.update_fluxes(0, None, {'x': {}}, cfg, 'total') == {'x': {}}
+  with pytest.raises(ValueError, match='numeric_flux_scheme'):
+    riemann_solver.select_numeric_flux_fn(SwirlCParameters({'numeric_flux_scheme': 'ROE'}))
+  L = N.lib()
+  ext = (ctypes.c_int * 3)(4, 4, 4)
+  assert L.sc_fill_plane(ctypes.c_void_p(16), ext, 1, 4, ctypes.c_float(1.0), None) == 1
+  assert L.sc_exchange_halos(None, ext, 2, None, None) == 1
+  assert L.sc_riemann_flux(None, 0, None, None, None, None, None, None, None, None) == 1
+  hh = (ctypes.c_double * 3)(0.1, 0.0, 0.1)
+  assert L.sc_heat_flux(ctypes.c_void_p(16), ctypes.c_void_p(32), ext, hh, 1e-3, 0.7,
+                        (ctypes.c_void_p * 3)(48, 64, 80), None) == 1
+
+
 def test_python_config_validation_errors():
   """The Python host raises the reference's exception types before any GPU use."""
   from swirlc_amd.core.parameter import SwirlCParameters
