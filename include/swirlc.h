@@ -193,6 +193,13 @@ sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme
 sc_status sc_fill_plane(float* f, const int* extents, int dim, int index, float value,
                         void* stream);
 
+/* Whole-array steps of the face BC update _face_boundary_update_1d
+ * (boundary/boundary.py:230-337) in IEEE fp32 over n elements:
+ * op 0 out = divide_no_nan(a, b), op 1 out = a * b, op 2 out = (1 / b) * a
+ * (conservative_to_primitive, common/utils.py:145-183). */
+sc_status sc_face_bc_elementwise(int op, const float* a, const float* b, float* out,
+                                 long long n, void* stream);
+
 /* numerics/interpolation.py:21-53 linear_interpolation: out[i] =
  * 0.5 * (v[i-1] + v[i]) on face i-1/2 along dim, v[-1] taken as 0. */
 sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,

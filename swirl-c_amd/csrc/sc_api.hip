@@ -513,6 +513,16 @@ sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme
                    "face interpolation kernel");
 }
 
+sc_status sc_face_bc_elementwise(int op, const float* a, const float* b, float* out,
+                                 long long n, void* stream) {
+  if (a == nullptr || b == nullptr || out == nullptr)
+    return fail(SC_ERR_INVALID, "sc_face_bc_elementwise: null argument");
+  if (op < 0 || op > 2) return fail(SC_ERR_INVALID, "sc_face_bc_elementwise: op must be 0, 1 or 2");
+  if (n < 1) return fail(SC_ERR_INVALID, "sc_face_bc_elementwise: n must be positive");
+  return hip_check(sc::launch_elementwise(op, a, b, out, n, (hipStream_t)stream),
+                   "elementwise kernel");
+}
+
 sc_status sc_fill_plane(float* f, const int* extents, int dim, int index, float value,
                         void* stream) {
   if (f == nullptr || extents == nullptr) return fail(SC_ERR_INVALID, "sc_fill_plane: null argument");

@@ -93,7 +93,33 @@ fill_plane_kernel(float* f, int nx, int ny, int nz, int dim, int index, float va
   f[((long long)z * nx + x) * ny + y] = value;
 }
 
+// Whole-array steps of the face BC update (boundary.py:230-337), IEEE fp32:
+// op 0 out = b == 0 ? 0 : a / b (divide_no_nan), 1 out = a * b,
+// 2 out = (1 / b) * a (conservative_to_primitive's reciprocal, then product).
+__global__ void __launch_bounds__(256)
+elementwise_kernel(int op, const float* a, const float* b, float* out, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float x = a[t], y = b[t];
+  float r;
+  if (op == 0) {
+    r = y == 0.0f ? 0.0f : x / y;
+  } else if (op == 1) {
+    r = x * y;
+  } else {
+    r = (1.0f / y) * x;
+  }
+  out[t] = r;
+}
+
 }  // namespace
+
+hipError_t launch_elementwise(int op, const float* a, const float* b, float* out, long long n,
+                              hipStream_t stream) {
+  hipLaunchKernelGGL(elementwise_kernel, dim3((unsigned)((n + 255) / 256)), 256, 0, stream, op,
+                     a, b, out, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_plane(float* f, const int N[3], int dim, int index, float value,
                              hipStream_t stream) {

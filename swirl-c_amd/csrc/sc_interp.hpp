@@ -12,6 +12,9 @@ namespace sc {
 // kWeno5, kWeno3 or kMuscl.  Arguments are validated by the caller.
 hipError_t launch_face_interp(const float* v, const int N[3], int dim, int scheme, float* neg,
                               float* pos, hipStream_t stream);
+// out = a / b or 0 where b == 0 (op 0), a * b (op 1), (1 / b) * a (op 2).
+hipError_t launch_elementwise(int op, const float* a, const float* b, float* out, long long n,
+                              hipStream_t stream);
 // f = value on the plane coordinate[dim] == index of the [z][x][y] array.
 hipError_t launch_fill_plane(float* f, const int N[3], int dim, int index, float value,
                              hipStream_t stream);

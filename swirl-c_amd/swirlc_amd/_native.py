@@ -30,7 +30,8 @@ EXPORTED = (
     'sc_batch_max_wave_speed', 'sc_ctx_enable_adjoint', 'sc_rhs_vjp', 'sc_rk_stage_vjp',
     'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning', 'sc_batch_set_tuning',
     'sc_ctx_set_reference_state', 'sc_face_interp', 'sc_linear_interp', 'sc_riemann_flux',
-    'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane', 'sc_last_error',
+    'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
+    'sc_face_bc_elementwise', 'sc_last_error',
     'sc_version',
 )
 
@@ -146,6 +147,7 @@ def lib():
     L.sc_heat_flux.argtypes = [vp, vp, ip3, dp3, ctypes.c_double, ctypes.c_double, fpp, vp]
     L.sc_exchange_halos.argtypes = [vp, ip3, i, ctypes.POINTER(ScBcFace), vp]
     L.sc_fill_plane.argtypes = [vp, ip3, i, i, f, vp]
+    L.sc_face_bc_elementwise.argtypes = [i, vp, vp, vp, ctypes.c_longlong, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_batch_create.argtypes = [ctypes.POINTER(ScParams), i, i, ctypes.POINTER(ScInstance),
@@ -172,7 +174,7 @@ def lib():
                  'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning',
                  'sc_batch_set_tuning', 'sc_ctx_set_reference_state', 'sc_face_interp',
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
-                 'sc_exchange_halos', 'sc_fill_plane'):
+                 'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

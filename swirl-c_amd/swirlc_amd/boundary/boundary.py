@@ -4,6 +4,9 @@
       boundary/boundary.py:34-86
   update_fluxes(replica_id, replicas, fluxes, cfg, flux_type)
       boundary/boundary.py:89-183
+  update_faces(replica_id, replicas, conservative_neg, conservative_pos,
+               primitive_neg, primitive_pos, cfg)
+      boundary/boundary.py:230-461
   update_conservative_cell_averages(replica_id, replicas, conservative, cfg)
       boundary/boundary.py:464-539
 
@@ -22,6 +25,7 @@ import ctypes
 import torch
 
 from swirlc_amd import _native as N
+from swirlc_amd.common import types
 from swirlc_amd.equations.native_context import NativeContext
 
 
@@ -139,3 +143,87 @@ def update_fluxes(replica_id, replicas, fluxes, cfg, flux_type):
                 'sc_fill_plane')
         out[dim][var] = f
   return out
+
+
+def _ew(op, a, b):
+  out = torch.empty_like(a)
+  N.check(N.lib().sc_face_bc_elementwise(op, ctypes.c_void_p(a.data_ptr()),
+                                         ctypes.c_void_p(b.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), a.numel(),
+                                         N.stream_handle()), 'sc_face_bc_elementwise')
+  return out
+
+
+def _plane(f, d, index, val):
+  f = f.clone()
+  nz, nx, ny = f.shape
+  N.check(N.lib().sc_fill_plane(ctypes.c_void_p(f.data_ptr()), (ctypes.c_int * 3)(nx, ny, nz),
+                                d, index, ctypes.c_float(float(val)), N.stream_handle()),
+          'sc_fill_plane')
+  return f
+
+
+def _face_update_1d(cons, prim, spec, d, index):
+  """_face_boundary_update_1d (boundary.py:230-337) on one face plane."""
+  if not spec:
+    return cons, prim
+  cons = dict(cons)
+  prim = dict(prim)
+  if 'rho' in spec:
+    cons['rho'] = _plane(cons['rho'], d, index, spec['rho'])
+    # The reference rebuilds the whole primitive dict here (p and h are gone).
+    prim = {'rho': cons['rho']}
+    for k, q in cons.items():
+      if k != 'rho' and k.startswith('rho_') and len(k) > 4:
+        prim[types.conservative_to_primitive_name(k)] = _ew(2, q, cons['rho'])
+  for var, val in spec.items():
+    if var == 'rho':
+      continue
+    if var in cons:
+      cons[var] = _plane(cons[var], d, index, val)
+      prim[types.conservative_to_primitive_name(var)] = _ew(0, cons[var], cons['rho'])
+    else:
+      if var not in prim:
+        raise ValueError(f'"{var}" is not a valid primitive or conservative'
+                         ' variable name for the face boundary condition.')
+      prim[var] = _plane(prim[var], d, index, val)
+      cons[types.primitive_to_conservative_name(var)] = _ew(1, cons['rho'], prim[var])
+  return cons, prim
+
+
+def update_faces(replica_id, replicas, conservative_neg, conservative_pos, primitive_neg,
+                 primitive_pos, cfg):
+  """The four face-state dicts ({dim: {var: tensor}}) with the DIRICHLET
+  values of cfg.bc['cell_faces'] written on the boundary face planes (index
+  halo_width low, halo_width + core_n high), each value followed by the
+  reference's update of its partner variable over the whole array
+  (primitive = divide_no_nan(conservative, rho), or conservative = rho *
+  primitive); a density value rebuilds the primitive dict from the
+  conservative one, as the reference does.  Values must be scalars; inputs
+  are not modified."""
+  del replica_id, replicas
+  table = (cfg.bc or {}).get('cell_faces')
+  if not table:
+    return conservative_neg, conservative_pos, primitive_neg, primitive_pos
+  hw = int(cfg.halo_width)
+  core_n = {'x': cfg.core_nx, 'y': cfg.core_ny, 'z': cfg.core_nz}
+  cn = dict(conservative_neg)
+  cp = dict(conservative_pos)
+  pn = dict(primitive_neg)
+  pp = dict(primitive_pos)
+  for d, dim in enumerate(('x', 'y', 'z')):
+    for side in (0, 1):
+      spec = {}
+      for var, faces in table.items():
+        entry = faces[dim][side]
+        kind = getattr(entry[0], 'name', entry[0])
+        if kind is None:
+          continue
+        assert kind == 'DIRICHLET', f'Unsupported interface (face or flux) BC type {kind}.'
+        if not isinstance(entry[1], (int, float)):
+          raise NotImplementedError('face boundary values must be scalars')
+        spec[var] = entry[1]
+      index = hw if side == 0 else hw + int(core_n[dim])
+      cn[dim], pn[dim] = _face_update_1d(cn[dim], pn[dim], spec, d, index)
+      cp[dim], pp[dim] = _face_update_1d(cp[dim], pp[dim], spec, d, index)
+  return cn, cp, pn, pp
