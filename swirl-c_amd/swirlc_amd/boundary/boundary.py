@@ -7,6 +7,8 @@
   update_faces(replica_id, replicas, conservative_neg, conservative_pos,
                primitive_neg, primitive_pos, cfg)
       boundary/boundary.py:230-461
+  update_boundary(replica_id, replicas, states, cfg)
+      boundary/boundary.py:606-628
   update_conservative_cell_averages(replica_id, replicas, conservative, cfg)
       boundary/boundary.py:464-539
 
@@ -227,3 +229,11 @@ def update_faces(replica_id, replicas, conservative_neg, conservative_pos, primi
       cn[dim], pn[dim] = _face_update_1d(cn[dim], pn[dim], spec, d, index)
       cp[dim], pp[dim] = _face_update_1d(cp[dim], pp[dim], spec, d, index)
   return cn, cp, pn, pp
+
+
+def update_boundary(replica_id, replicas, states, cfg):
+  """{key: exchange_halos(u, cfg.bc[key], cfg.halo_width)} for every state
+  (boundary.py:606-628): each variable's BC dict sits under its own name in
+  cfg.bc."""
+  return {key: exchange_halos(replica_id, replicas, val, cfg.bc[key], cfg.halo_width)
+          for key, val in states.items()}

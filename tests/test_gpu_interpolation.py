@@ -170,3 +170,18 @@ def test_update_faces_matches_oracle_bitwise(case):
       for k in r[dim]:
         np.testing.assert_array_equal(g[dim][k].cpu().numpy(), r[dim][k],
                                       err_msg=f'{what}/{dim}/{k}')
+
+
+def test_update_boundary_matches_oracle_bitwise():
+  n, h = (7, 6, 5), 3
+  cases = _bc_cases(n, h)
+  nx, ny, nz = n
+  rng = np.random.default_rng(9)
+  states = {k: rng.standard_normal((nz + 2 * h, nx + 2 * h, ny + 2 * h)).astype(np.float32)
+            for k in ('a', 'b')}
+  cfg = SwirlCParameters({'halo_width': h, 'bc': {'a': cases['mixed'], 'b': cases['layers']}})
+  got = boundary.update_boundary(0, None, {k: torch.from_numpy(v.copy()).cuda()
+                                           for k, v in states.items()}, cfg)
+  torch.cuda.synchronize()
+  for k, bc in (('a', cases['mixed']), ('b', cases['layers'])):
+    np.testing.assert_array_equal(got[k].cpu().numpy(), O.exchange_halos(states[k], bc, h))
