@@ -277,7 +277,7 @@ def bench_adjoint(n, steps, every):
   finite = all(bool(torch.isfinite(t.grad).all()) for t in x)
   gnorm = float(sum((t.grad.double() ** 2).sum() for t in x) ** 0.5)
   return {'workload': f'kelvin_helmholtz_{n}^3_viscous_ssprk3_{steps}steps_ckpt{every}',
-          'dt': float(dt[0]), 'loss': float(J), 'grad_norm': gnorm, 'finite': finite,
+          'dt': float(dt[0]), 'loss': float(J.detach()), 'grad_norm': gnorm, 'finite': finite,
           'forward_s': round(t_fwd, 4), 'adjoint_s': round(t_bwd, 4),
           'adjoint_over_forward': round(t_bwd / t_fwd, 3),
           'forward_Mcell_updates_per_s': round(n ** 3 * steps / t_fwd / 1e6, 1),
