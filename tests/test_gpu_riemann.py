@@ -5,8 +5,8 @@ Face states come from the oracle's own pipeline (halo fill, primitives,
 face interpolation, primitive -> conservative faces; p_dyn from a reference
 state where given), so both sides see identical inputs; the bar is 1e-5
 relative L-infinity over the interior faces of every direction, with
-gravity (the potential enters the Roe sound speed), a transported scalar
-and the dynamic pressure covered.
+gravity (the potential enters the Roe sound speed), one and two transported
+scalars and the dynamic pressure covered.
 """
 
 import copy
@@ -78,9 +78,10 @@ def test_hllc_with_dynamic_pressure_matches_oracle():
   _check(cfg, st, helper, 'HLLC/p_dyn')
 
 
+@pytest.mark.parametrize('nscalar', [1, 2])
 @pytest.mark.parametrize('flux', ['HLL', 'HLLC'])
-def test_flux_with_scalar_matches_oracle(flux):
+def test_flux_with_scalars_matches_oracle(flux, nscalar):
   n = (9, 10, 8)
-  cfg = SC._cfg(n, 'MUSCL', flux, False, 'periodic', 1)
-  st = random_state(n, 3, seed=5, nscalar=1)
-  _check(cfg, st, {}, f'{flux}/scalar')
+  cfg = SC._cfg(n, 'MUSCL', flux, False, 'periodic', nscalar)
+  st = random_state(n, 3, seed=5, nscalar=nscalar)
+  _check(cfg, st, {}, f'{flux}/{nscalar} scalars')
