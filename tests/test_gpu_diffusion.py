@@ -4,7 +4,7 @@
 The kernels round each operation in the oracle's order (contraction off,
 IEEE division), so the bar is bit-for-bit equality over the whole array,
 halo and edge cells included (neighbours outside the array count as 0 on
-both sides).
+both sides); the thin case puts every cell within one cell of an edge.
 """
 
 import copy
@@ -33,12 +33,12 @@ def _gpu(a):
   return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize('n', [(8, 9, 10), (5, 3, 12)])
-def test_shear_stress_and_heat_flux_match_oracle_bitwise(n):
-  cfg = grid_cfg(n, 3, (1.0, 0.7, 1.3))
+@pytest.mark.parametrize('n,h', [((8, 9, 10), 3), ((5, 3, 12), 3), ((2, 1, 3), 1)])
+def test_shear_stress_and_heat_flux_match_oracle_bitwise(n, h):
+  cfg = grid_cfg(n, h, (1.0, 0.7, 1.3))
   cfg['transport_parameters'] = {'nu': 2e-3, 'pr': 0.7}
   ocfg = oracle_cfg(cfg)
-  shape = tuple(x + 6 for x in (n[2], n[0], n[1]))
+  shape = tuple(x + 2 * h for x in (n[2], n[0], n[1]))
   prim = _prims(shape, 3)
   prim_f = {d: {'rho': O.linear_interpolation(prim['rho'], d)} for d in O.DIMS}
   ref_tau = O.shear_stress(prim, prim_f, ocfg)
