@@ -98,7 +98,7 @@ def _bc_cases(n, h):
   }
 
 
-@pytest.mark.parametrize('n,h', [((7, 6, 5), 3), ((4, 9, 3), 2), ((5, 4, 6), 4)])
+@pytest.mark.parametrize('n,h', [((7, 6, 5), 3), ((4, 9, 3), 2), ((5, 4, 6), 4), ((3, 1, 2), 1)])
 @pytest.mark.parametrize('case', ['mixed', 'layers', 'periodic'])
 def test_exchange_halos_matches_oracle_bitwise(n, h, case):
   nx, ny, nz = n
