@@ -49,7 +49,10 @@ typedef int sc_status;
 /* numerics/riemann_solver/riemann_solver.py:24-27 (NumericFluxScheme). */
 #define SC_HLL 0
 #define SC_HLLC 1
-/* boundary/bc_types.py:22-27 (BoundaryCondition). */
+/* boundary/bc_types.py:22-27 (BoundaryCondition).  SC_BC_NONE marks a
+ * dimension with no BC (a bc dict without that dim, boundary.py:58-60): its
+ * halo cells are left as they are; both faces of the dim must carry it. */
+#define SC_BC_NONE (-1)
 #define SC_BC_PERIODIC 0
 #define SC_BC_DIRICHLET 1
 #define SC_BC_NEUMANN 2

@@ -203,8 +203,12 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
         const sc_bc_face& b = p.bc[f][d][s];
         if (b.type == SC_BC_REFLECTIVE)
           return fail(SC_ERR_UNSUPPORTED, "Unsupported BC type BoundaryCondition.REFLECTIVE.");
-        if (b.type < 0 || b.type > SC_BC_NEUMANN)
+        if (b.type < SC_BC_NONE || b.type > SC_BC_NEUMANN)
           return fail(SC_ERR_INVALID, "unknown BC type");
+        if ((b.type == SC_BC_NONE) != (p.bc[f][d][1 - s].type == SC_BC_NONE))
+          return fail(SC_ERR_INVALID, "SC_BC_NONE must be given on both faces of a dim");
+        if (b.type == SC_BC_NONE && ((p.prim_bc_mask >> f) & 1))
+          return fail(SC_ERR_UNSUPPORTED, "a BC given on a primitive must cover every dim");
         if (b.type == SC_BC_PERIODIC && n[d] < h)
           return fail(SC_ERR_UNSUPPORTED, "PERIODIC BC needs at least halo_width interior cells");
         if (b.n_layer_values != 0 && b.n_layer_values != h)
@@ -668,8 +672,10 @@ sc_status sc_exchange_halos(float* u, const int* extents, int halo, const sc_bc_
   for (int d = 0; d < 3; ++d)
     for (int sd = 0; sd < 2; ++sd) {
       const sc_bc_face& b = bc[2 * d + sd];
-      if (b.type < SC_BC_PERIODIC || b.type > SC_BC_NEUMANN)
+      if (b.type < SC_BC_NONE || b.type > SC_BC_NEUMANN || b.type == SC_BC_REFLECTIVE)
         return fail(SC_ERR_UNSUPPORTED, "sc_exchange_halos: unsupported BC type");
+      if ((b.type == SC_BC_NONE) != (bc[2 * d + 1 - sd].type == SC_BC_NONE))
+        return fail(SC_ERR_INVALID, "SC_BC_NONE must be given on both faces of a dim");
       if (b.type == SC_BC_PERIODIC && extents[d] < halo)
         return fail(SC_ERR_UNSUPPORTED,
                     "PERIODIC BC needs at least halo_width interior cells");

@@ -70,6 +70,7 @@ __device__ __forceinline__ void halo_cell_fill(const HaloArgs& a, const HaloGeom
   };
   const BcFaceDev& lo = a.bc[f][D][0];
   const BcFaceDev& hi = a.bc[f][D][1];
+  if (lo.type == kBcNone) return;  // no BC along D: halo cells left alone
   // Layer i of the low side is cell i, of the high side cell N - h + i;
   // Neumann runs outward from the boundary (low side: cell h - 1 - i).
   int dst;

@@ -8,7 +8,7 @@
 namespace sc {
 
 constexpr int kMaxHalo = 4;
-constexpr int kBcPeriodic = 0, kBcDirichlet = 1, kBcNeumann = 2;
+constexpr int kBcNone = -1, kBcPeriodic = 0, kBcDirichlet = 1, kBcNeumann = 2;
 
 struct BcFaceDev {
   int type;

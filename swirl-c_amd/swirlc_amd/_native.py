@@ -20,6 +20,7 @@ SC_MAX_HALO = 4
 SC_WENO_3, SC_WENO_5, SC_MUSCL = 0, 1, 2
 SC_HLL, SC_HLLC = 0, 1
 SC_BC_PERIODIC, SC_BC_DIRICHLET, SC_BC_NEUMANN, SC_BC_REFLECTIVE = 0, 1, 2, 3
+SC_BC_NONE = -1  # no BC along a dim (its halo cells are left alone)
 SC_RK_EULER, SC_RK_BLEND, SC_RK_AVG = 1, 2, 3
 
 # Every entry point include/swirlc.h declares.

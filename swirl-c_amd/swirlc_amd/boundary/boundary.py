@@ -18,8 +18,8 @@ BCs given on the primitive variable, boundary.py:516-531), in the x, y, z
 order of swirl_lm's halo exchange, with libswirlc_amd's halo kernels
 (sc_fill_halos).  As in the reference the input dict is left unchanged and a
 new dict is returned.  Fields are float32 GPU tensors [nz+2h, nx+2h, ny+2h];
-one replica.  Each call builds a native context for cfg (device allocations
-included); code that fills halos every step keeps one NativeContext instead.
+one replica.  The native context of a configuration is made once and reused
+by later calls with an equal configuration.
 """
 
 import ctypes
@@ -28,7 +28,7 @@ import torch
 
 from swirlc_amd import _native as N
 from swirlc_amd.common import types
-from swirlc_amd.equations.native_context import NativeContext
+from swirlc_amd.equations.native_context import NativeContext, cached_context
 
 
 class _OneField:
@@ -36,12 +36,14 @@ class _OneField:
   _plane_shape = NativeContext._plane_shape
   _layer_plane = NativeContext._layer_plane
   _fill_dim_bc = NativeContext._fill_dim_bc
+  upload_planes = NativeContext.upload_planes
 
   def __init__(self, shape, hw, device):
     self.shape = tuple(shape)
     self.hw = hw
     self.device = device
     self._keep = []
+    self._pending_planes = []
 
 
 def exchange_halos(replica_id, replicas, u, bc_dict, width):
@@ -60,9 +62,12 @@ def exchange_halos(replica_id, replicas, u, bc_dict, width):
   t = _OneField(u.shape, hw, u.device)
   faces = (N.ScBcFace * 6)()
   for d, dim in enumerate(('x', 'y', 'z')):
-    if dim not in bc_dict:
-      raise ValueError(f'bc_dict lacks dimension {dim}')
+    if bc_dict.get(dim) is None:
+      # boundary.py:58-60: a dim without a BC is not exchanged.
+      faces[2 * d].type = faces[2 * d + 1].type = N.SC_BC_NONE
+      continue
     t._fill_dim_bc((faces[2 * d], faces[2 * d + 1]), bc_dict[dim], d)
+  t.upload_planes()
   out = u.clone()
   N.check(N.lib().sc_exchange_halos(ctypes.c_void_p(out.data_ptr()),
                                     (ctypes.c_int * 3)(nx, ny, nz), hw, faces,
@@ -81,7 +86,7 @@ def update_conservative_cell_averages(replica_id, replicas, conservative, cfg):
   leaves the interior alone (the sweeps apply that rounding as they load).
   """
   del replica_id, replicas
-  ctx = NativeContext(cfg)
+  ctx = cached_context(cfg)
   out = {k: v.clone() for k, v in conservative.items()}
   ctx.fill_halos(ctx.fields(out))
   h = int(cfg.halo_width)
@@ -148,6 +153,11 @@ def update_fluxes(replica_id, replicas, fluxes, cfg, flux_type):
 
 
 def _ew(op, a, b):
+  N.require_gpu_tensor(a, 'face state')
+  N.require_gpu_tensor(b, 'face state')
+  if a.shape != b.shape or a.device != b.device:
+    raise ValueError(f'face states of shapes {tuple(a.shape)} and {tuple(b.shape)} '
+                     f'(devices {a.device}, {b.device}) do not match')
   out = torch.empty_like(a)
   N.check(N.lib().sc_face_bc_elementwise(op, ctypes.c_void_p(a.data_ptr()),
                                          ctypes.c_void_p(b.data_ptr()),
@@ -157,6 +167,9 @@ def _ew(op, a, b):
 
 
 def _plane(f, d, index, val):
+  N.require_gpu_tensor(f, 'face state')
+  if f.dim() != 3:
+    raise ValueError(f'expected a 3-D face state [nz, nx, ny], got shape {tuple(f.shape)}')
   f = f.clone()
   nz, nx, ny = f.shape
   N.check(N.lib().sc_fill_plane(ctypes.c_void_p(f.data_ptr()), (ctypes.c_int * 3)(nx, ny, nz),

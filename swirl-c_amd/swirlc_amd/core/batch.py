@@ -45,6 +45,8 @@ class BatchSolver:
     probe = NativeContext(probe_cfg, device=device, create=False)
     if probe.nf != 5:
       raise NotImplementedError('batched ensembles support the 5 base fields only')
+    if probe._pending_planes:
+      raise NotImplementedError('batched ensembles do not support per-layer DIRICHLET planes')
     self.names = probe.names
     h = self.h = probe.hw
     self.n = len(instances)

@@ -6,7 +6,9 @@ face-interpolation dispatch) and owns one `sc_ctx` of libswirlc_amd.  Its
 methods launch the HIP kernels on the current torch stream.
 """
 
+import collections
 import ctypes
+import hashlib
 import numbers
 
 import numpy as np
@@ -51,10 +53,14 @@ def validate_models(cfg):
 class NativeContext:
   """One native RHS/RK context for a fixed grid and configuration."""
 
+  # Native contexts made by this process (sc_ctx_create calls that succeeded).
+  created = 0
+
   def __init__(self, cfg, device=None, create=True):
     """Validates cfg and builds the native parameters; `create=False` stops
     before the native context is made (the parameters are reused by a batch)."""
     self._ctx = None
+    self._pending_planes = []  # (sc_bc_face, host planes) uploaded at creation
     validate_models(cfg)
     scheme = _get(cfg, 'interpolation_scheme')
     if scheme not in _INTERP:
@@ -142,21 +148,62 @@ class NativeContext:
             f' No boundary condition was found for {name} or'
             f' {types.conservative_to_primitive_name(name)}.')
       for d, dim in enumerate(types.DIMS):
-        if dim not in spec:
-          raise ValueError(f'boundary condition of {name} lacks dim {dim}')
+        if dim not in spec or spec[dim] is None:
+          # boundary.py:58-60: no BC along dim, its halo cells are left alone.
+          if (mask >> f) & 1:
+            raise NotImplementedError(
+                f'a BC given on the primitive of {name} must cover every dimension')
+          for s in (0, 1):
+            params.bc[f][d][s].type = N.SC_BC_NONE
+          continue
         self._fill_dim_bc(params.bc[f][d], spec[dim], d)
     params.prim_bc_mask = mask
     self.prim_bc_mask = mask
     self._fill_boundary_values(params, bc)
 
-    if not create:
-      return
-    L = N.lib()
+    if create:
+      self.create()
+
+  def create(self):
+    """Makes the native context (sc_ctx_create) from the translated config."""
+    self.upload_planes()
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):
-      N.check(L.sc_ctx_create(ctypes.byref(grid), ctypes.byref(params),
-                              ctypes.byref(handle)), 'sc_ctx_create')
+      N.check(N.lib().sc_ctx_create(ctypes.byref(self._grid), ctypes.byref(self._params),
+                                    ctypes.byref(handle)), 'sc_ctx_create')
     self._ctx = handle
+    NativeContext.created += 1
+
+  def upload_planes(self):
+    """Copies the per-layer DIRICHLET planes to the device and points the BC
+    table at them (kept alive by the context)."""
+    for face, planes in self._pending_planes:
+      dev = torch.from_numpy(planes).to(self.device).contiguous()
+      self._keep.append(dev)
+      face.layer_planes = dev.data_ptr()
+    self._pending_planes = []
+
+  def cache_key(self):
+    """Bytes that identify the native context this translation makes: the
+    grid and parameter structs (pointer fields excluded), the coordinates
+    and the per-layer planes, and the device."""
+    digest = hashlib.sha256()
+    grid = N.ScGrid.from_buffer_copy(self._grid)
+    grid.x_full = grid.y_full = grid.z_full = None
+    digest.update(bytes(grid))
+    params = N.ScParams.from_buffer_copy(self._params)
+    for f in range(N.SC_MAX_FIELDS):
+      for d in range(3):
+        for s in range(2):
+          params.bc[f][d][s].layer_planes = None
+    digest.update(bytes(params))
+    for arr in self._keep:
+      if isinstance(arr, np.ndarray):
+        digest.update(arr.tobytes())
+    for _, planes in self._pending_planes:
+      digest.update(planes.tobytes())
+    digest.update(str(self.device).encode())
+    return digest.digest()
 
   # -- BC translation ------------------------------------------------------
 
@@ -280,10 +327,8 @@ class NativeContext:
           for i, v in enumerate(val):
             out.layer_values[i] = float(v)
         else:
-          planes = np.stack([self._layer_plane(v, d) for v in val])
-          dev = torch.from_numpy(planes).to(self.device).contiguous()
-          self._keep.append(dev)
-          out.layer_planes = dev.data_ptr()
+          planes = np.ascontiguousarray(np.stack([self._layer_plane(v, d) for v in val]))
+          self._pending_planes.append((out, planes))
       else:
         raise NotImplementedError(f'unsupported DIRICHLET value of type {type(val)}')
 
@@ -405,3 +450,33 @@ class NativeContext:
         ctypes.c_void_p(dt_dev.data_ptr()), N.ptr_array(uk), N.ptr_array(lam_out),
         N.ptr_array(lam_uk), N.ptr_array(lam_u0) if lam_u0 is not None else None,
         N.stream_handle(stream)), 'sc_rk_stage_vjp')
+
+
+_CACHE_SIZE = 8
+_cache = collections.OrderedDict()
+
+
+def cached_context(cfg, device=None):
+  """A NativeContext for cfg, reused across calls with an equal configuration
+  (grid, schemes, BC tables and values, device): the stand-alone seams call
+  this so repeated calls make no device allocations.  The least recently used
+  of more than 8 contexts is released."""
+  ctx = NativeContext(cfg, device=device, create=False)
+  if ctx.device.type == 'cuda' and ctx.device.index is None:
+    ctx.device = torch.device('cuda', torch.cuda.current_device())
+  key = ctx.cache_key()
+  hit = _cache.get(key)
+  if hit is not None:
+    _cache.move_to_end(key)
+    return hit
+  ctx.create()
+  _cache[key] = ctx
+  while len(_cache) > _CACHE_SIZE:
+    _cache.popitem(last=False)[1].close()
+  return ctx
+
+
+def clear_context_cache():
+  """Releases every cached context."""
+  while _cache:
+    _cache.popitem(last=False)[1].close()

@@ -39,13 +39,24 @@ def _ptrs(tensors):
   return arr
 
 
+def _same_grid(named):
+  """The kernels read every field over the first one's extents."""
+  (name0, t0), rest = named[0], named[1:]
+  for name, t in named:
+    N.require_gpu_tensor(t, name)
+  for name, t in rest:
+    if t.shape != t0.shape or t.device != t0.device:
+      raise ValueError(f'{name} has shape {tuple(t.shape)} on {t.device}, expected '
+                       f'{tuple(t0.shape)} on {t0.device} (as {name0})')
+
+
 def shear_stress(states, states_fx, states_fy, states_fz, cfg, physics_models=None):
   """tau[i][j] = 2 rho_j nu S_ij on the j faces (3 x 3 nested list of tensors)."""
   del physics_models
   u, v, w = states['u'], states['v'], states['w']
   rho_f = [states_fx['rho'], states_fy['rho'], states_fz['rho']]
-  for t in (u, v, w, *rho_f):
-    N.require_gpu_tensor(t, 'field')
+  _same_grid([('u', u), ('v', v), ('w', w), ('rho on the x faces', rho_f[0]),
+              ('rho on the y faces', rho_f[1]), ('rho on the z faces', rho_f[2])])
   ext, h = _geom(u, cfg)
   tau = [torch.empty_like(u) for _ in range(9)]
   N.check(N.lib().sc_shear_stress(
@@ -59,8 +70,7 @@ def single_component_heat_flux(primitive, cfg, physics_models=None):
   """{'x', 'y', 'z'}: -kappa_f dT/dx on the faces of each direction."""
   del physics_models
   rho, p = primitive['rho'], primitive['p']
-  for t in (rho, p):
-    N.require_gpu_tensor(t, 'field')
+  _same_grid([('rho', rho), ('p', p)])
   ext, h = _geom(rho, cfg)
   tp = cfg.transport_parameters
   flux = [torch.empty_like(rho) for _ in range(3)]

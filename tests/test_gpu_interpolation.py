@@ -185,3 +185,49 @@ def test_update_boundary_matches_oracle_bitwise():
   torch.cuda.synchronize()
   for k, bc in (('a', cases['mixed']), ('b', cases['layers'])):
     np.testing.assert_array_equal(got[k].cpu().numpy(), O.exchange_halos(states[k], bc, h))
+
+
+@pytest.mark.parametrize('missing', [('y',), ('x', 'z'), ('x', 'y', 'z')])
+def test_exchange_halos_without_some_dims_matches_oracle_bitwise(missing):
+  """A bc dict without a dimension leaves that dimension's halo cells as they
+  are (boundary.py:58-60: its BC stays None)."""
+  n, h = (6, 5, 7), 3
+  nx, ny, nz = n
+  bc = {d: v for d, v in _bc_cases(n, h)['mixed'].items() if d not in missing}
+  u = np.random.default_rng(9).standard_normal((nz + 2 * h, nx + 2 * h, ny + 2 * h))
+  u = u.astype(np.float32)
+  ref = O.exchange_halos(u, bc, h)
+  got = boundary.exchange_halos(0, None, torch.from_numpy(u.copy()).cuda(), bc, h)
+  torch.cuda.synchronize()
+  np.testing.assert_array_equal(got.cpu().numpy(), ref)
+
+
+def test_rhs_with_a_field_lacking_a_dim_bc_matches_oracle():
+  """A conservative field whose BC dict lacks y keeps its y halo cells."""
+  n, h = (9, 8, 7), 3
+  cfg = P._cfg(n, h, 'WENO_5', 'HLLC', True, 'neumann')
+  del cfg['bc']['cell_averages']['rho_v']['y']
+  st = random_state(n, h, seed=13)
+  ref = O.rhs(st, {}, oracle_cfg(cfg))
+  got = P._gpu_rhs(cfg, st)
+  core = (slice(h, -h),) * 3
+  errs = {k: rel_linf(got[k][core], ref[k][core]) for k in ref}
+  assert max(errs.values()) <= P.TOL, errs
+
+
+def test_cell_average_update_reuses_its_context():
+  from swirlc_amd.equations import native_context
+  n, h = (6, 5, 4), 3
+  cfg = P._cfg(n, h, 'WENO_5', 'HLLC', False, 'mixed')
+  st = random_state(n, h, seed=14)
+  g = {k: torch.from_numpy(v).cuda() for k, v in st.items()}
+  boundary.update_conservative_cell_averages(0, None, g, SwirlCParameters(copy.deepcopy(cfg)))
+  before = native_context.NativeContext.created
+  for _ in range(3):
+    out = boundary.update_conservative_cell_averages(0, None, g,
+                                                     SwirlCParameters(copy.deepcopy(cfg)))
+  torch.cuda.synchronize()
+  assert native_context.NativeContext.created == before
+  ref = O.update_conservative_cell_averages(st, oracle_cfg(cfg))
+  for k in ref:
+    np.testing.assert_array_equal(out[k].cpu().numpy(), ref[k], err_msg=k)

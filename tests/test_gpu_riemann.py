@@ -85,3 +85,59 @@ def test_flux_with_scalars_matches_oracle(flux, nscalar):
   cfg = SC._cfg(n, 'MUSCL', flux, False, 'periodic', nscalar)
   st = random_state(n, 3, seed=5, nscalar=nscalar)
   _check(cfg, st, {}, f'{flux}/{nscalar} scalars')
+
+
+def test_hll_with_dynamic_pressure_matches_oracle():
+  cfg, _, st, helper = RS._setup((10, 9, 11), 'WENO_5', 'HLL', False, seed=9, frac=0.7)
+  _check(cfg, st, helper, 'HLL/p_dyn')
+
+
+@pytest.mark.parametrize('flux', ['HLL', 'HLLC'])
+@pytest.mark.parametrize('side', ['neg', 'pos'])
+def test_p_dyn_on_one_side_matches_oracle(flux, side):
+  """hllc.py:109-120 and the physical flux substitute p_dyn per side: a side
+  whose helper dict lacks it keeps its own face pressure."""
+  cfg, _, st, helper = RS._setup((9, 8, 10), 'WENO_5', flux, False, seed=11, frac=0.7)
+  ocfg = oracle_cfg(cfg)
+  h = cfg['halo_width']
+  cn, cp, pn, pp, hn, hp, pe = _faces(st, helper, ocfg)
+  names = ocfg['conservative_variable_names']
+  ref_fn = O.numeric_flux_fn(flux)
+  fn = riemann_solver.select_numeric_flux_fn(SwirlCParameters(copy.deepcopy(cfg)))
+  for d in O.DIMS:
+    hn_d, hp_d = (hn[d], {}) if side == 'neg' else ({}, hp[d])
+    ref = ref_fn(cn[d], cp[d], pn[d], pp[d], hn_d, hp_d, d, names, pe)
+    got = fn(0, None, _gpu(cn[d]), _gpu(cp[d]), _gpu(pn[d]), _gpu(pp[d]),
+             {d: _gpu(hn_d)}, {d: _gpu(hp_d)}, None, d,
+             SwirlCParameters(copy.deepcopy(cfg)), None)
+    torch.cuda.synchronize()
+    errs = {k: rel_linf(interior(got[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
+    assert max(errs.values()) <= TOL, (d, errs)
+
+
+def test_riemann_seam_checks_shapes_and_reuses_its_context():
+  """Face states must have the grid's shape (the kernel reads and writes the
+  full extent of every pointer); repeated calls with one configuration make
+  one native context."""
+  from swirlc_amd.equations import native_context
+  native_context.clear_context_cache()
+  cfg, _, st, helper = RS._setup((9, 8, 10), 'WENO_5', 'HLLC', False, seed=12, frac=0.0)
+  ocfg = oracle_cfg(cfg)
+  cn, cp, pn, pp, hn, hp, pe = _faces(st, helper, ocfg)
+  fn = riemann_solver.select_numeric_flux_fn(SwirlCParameters(copy.deepcopy(cfg)))
+  args = [_gpu(cn['x']), _gpu(cp['x']), _gpu(pn['x']), _gpu(pp['x'])]
+  before = native_context.NativeContext.created
+  for _ in range(3):
+    fn(0, None, *args, {}, {}, None, 'x', SwirlCParameters(copy.deepcopy(cfg)), None)
+  assert native_context.NativeContext.created == before + 1
+  small = dict(args[2])
+  small['p'] = small['p'][:, :-1].contiguous()
+  with pytest.raises(ValueError, match='shape'):
+    fn(0, None, args[0], args[1], small, args[3], {}, {}, None, 'x',
+       SwirlCParameters(copy.deepcopy(cfg)), None)
+  cons = dict(args[0])
+  cons['rho'] = cons['rho'].cpu()
+  with pytest.raises(ValueError, match='GPU tensor'):
+    fn(0, None, cons, args[1], args[2], args[3], {}, {}, None, 'x',
+       SwirlCParameters(copy.deepcopy(cfg)), None)
+  torch.cuda.synchronize()
