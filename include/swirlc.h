@@ -162,6 +162,21 @@ sc_status sc_rk_stage(sc_ctx* ctx, int form, float c0, float c1,
                       const float* dt_dev, const float* const* u0,
                       float* const* uk, float* const* out, void* stream);
 
+/* No counterpart in the reference: sc_rk_stage for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Identical results (same kernels); the
+ * RHS accumulator is a file-scope 64 MiB device array, one per GPU, held
+ * until the process ends, instead of the context's scratch fields.  Limit:
+ * n_fields * ((nx+2h)(ny+2h)(nz+2h) rounded up to 64) <= 16777216 floats
+ * (e.g. 5 fields of a 128^3 grid with halo 3), else SC_ERR_UNSUPPORTED.  The
+ * context's device must be current.  Allocates nothing; needs no zeroing
+ * (the x sweep writes every interior accumulator cell before it is read).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two threads) race. */
+sc_status sc_rk_stage_noalloc(sc_ctx* ctx, int form, float c0, float c1,
+                              const float* dt_dev, const float* const* u0,
+                              float* const* uk, float* const* out, void* stream);
+
 /* Hydrostatic reference state (helper_var 'p_ref' and 'rho_ref',
  * equations/compressible_navier_stokes.py:273-335): full-size device fields
  * used by the following sc_rhs / sc_rk_stage / sc_sweep_only /

@@ -4,6 +4,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/swirlc.h"
@@ -44,9 +45,22 @@ struct sc_ctx {
   sc::SweepTuning tune;
 };
 
+// RHS accumulator of sc_rk_stage_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host it is only a stand-in for the symbol): the
+// device address of each GPU's copy is looked up once per device with
+// hipGetSymbolAddress (at context creation, with that device current) and
+// kept in g_scratch_ptr.
+constexpr long long kScratchFloats = 16LL << 20;  // 64 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
+
 namespace {
 
 thread_local std::string g_last_error;
+
+constexpr int kMaxDevices = 64;
+float* g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
 
 sc_status fail(sc_status code, const std::string& msg) {
   g_last_error = msg;
@@ -234,6 +248,19 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   c->params = p;
   c->nf = p.n_fields;
   hipGetDevice(&c->device);
+  {
+    // Device address of this GPU's no-allocation scratch array (once per device).
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices && g_scratch_ptr[c->device] == nullptr) {
+      float* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) {
+        delete c;
+        return hip_check(e, "hipGetSymbolAddress(scratch)");
+      }
+      g_scratch_ptr[c->device] = ptr;
+    }
+  }
   for (int d = 0; d < 3; ++d) c->N[d] = n[d] + 2 * h;
   c->cells_full = (long long)c->N[0] * c->N[1] * c->N[2];
   c->h[0] = (float)grid->dx;
@@ -416,9 +443,44 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
   return run_sweeps(c, a, (hipStream_t)stream);
 }
 
+static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
+                               const float* dt_dev, const float* const* u0,
+                               float* const* uk, float* const* out, float* const* racc,
+                               void* stream);
+
 sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                       const float* const* u0, float* const* uk, float* const* out,
                       void* stream) {
+  if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, stream);
+}
+
+sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
+                              const float* const* u0, float* const* uk, float* const* out,
+                              void* stream) {
+  if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage_noalloc: null argument");
+  // Each field's accumulator starts on a 256-byte boundary of the array.
+  const long long stride = (c->cells_full + 63) / 64 * 64;
+  if (stride * c->nf > kScratchFloats)
+    return fail(SC_ERR_UNSUPPORTED,
+                "sc_rk_stage_noalloc: grid too large for the 64 MiB scratch array (needs"
+                " n_fields * (nx+2h)(ny+2h)(nz+2h), each rounded up to 64, <= 16777216"
+                " floats); use sc_rk_stage");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != c->device)
+    return fail(SC_ERR_INVALID, "sc_rk_stage_noalloc: the context's device must be current");
+  float* base = g_scratch_ptr[c->device];
+  if (base == nullptr)
+    return fail(SC_ERR_HIP, "sc_rk_stage_noalloc: scratch array address not resolved");
+  float* racc[SC_MAX_FIELDS] = {};
+  for (int f = 0; f < c->nf; ++f) racc[f] = base + f * stride;
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, stream);
+}
+
+static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
+                               const float* dt_dev, const float* const* u0,
+                               float* const* uk, float* const* out, float* const* racc,
+                               void* stream) {
   if (c == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
     return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
   if (form == SC_RK_AVG)
@@ -443,7 +505,7 @@ sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_d
   for (int f = 0; f < c->nf; ++f) {
     a.q.p[f] = uk[f];
     a.u0.p[f] = form == SC_RK_EULER ? nullptr : u0[f];
-    a.r.p[f] = c->rhs_scratch[f];
+    a.r.p[f] = racc[f];
     a.out.p[f] = out[f];
   }
   a.rk_form = form;

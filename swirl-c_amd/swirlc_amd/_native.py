@@ -33,7 +33,7 @@ EXPORTED = (
     'sc_ctx_set_reference_state', 'sc_face_interp', 'sc_linear_interp', 'sc_riemann_flux',
     'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
     'sc_face_bc_elementwise', 'sc_last_error',
-    'sc_version',
+    'sc_version', 'sc_rk_stage_noalloc',
 )
 
 
@@ -138,6 +138,7 @@ def lib():
     L.sc_fill_halos.argtypes = [vp, fpp, vp]
     L.sc_rhs.argtypes = [vp, fpp, fpp, vp]
     L.sc_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_rk_stage_noalloc.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
     ip3 = ctypes.POINTER(ctypes.c_int)
     L.sc_face_interp.argtypes = [vp, ip3, i, i, vp, vp, vp]
@@ -175,7 +176,8 @@ def lib():
                  'sc_ctx_get_tuning', 'sc_ctx_set_tuning', 'sc_batch_get_tuning',
                  'sc_batch_set_tuning', 'sc_ctx_set_reference_state', 'sc_face_interp',
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
-                 'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise'):
+                 'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise',
+                 'sc_rk_stage_noalloc'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib
