@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) batch_march_kernel(const BatchArgs b) {
     const SweepArgs a = tile_args(b, tl.inst, DIR);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE>(a, ti);
+    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, false, false, true>(a, ti);
   }
 }
 
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) batch_y_kernel(const BatchArgs b) {
     const SweepArgs a = tile_args(b, tl.inst, 1);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_y_body<SCHEME, FLUX, VISC, NS>(a, ti);
+    sweep_y_body<SCHEME, FLUX, VISC, NS, false, false, true>(a, ti);
   }
 }
 

@@ -141,58 +141,59 @@ __device__ __forceinline__ void recon(const float* a, float& fp, float& fm);
 template <>
 __device__ __forceinline__ void recon<kWeno5>(const float* a, float& fp, float& fm) {
   const float vm2 = a[0], vm1 = a[1], v0 = a[2], vp1 = a[3], vp2 = a[4];
-  const float c1312 = (float)(13.0 / 12.0), c16 = (float)(1.0 / 6.0);
+  const float c16 = (float)(1.0 / 6.0), c13 = (float)(1.0 / 3.0);
   // First and second differences of the stencil.
   const float dm2 = vm1 - vm2, dm1 = v0 - vm1, d0 = vp1 - v0, d1 = vp2 - vp1;
   const float t0 = d1 - d0, t1 = d0 - dm1, t2 = dm1 - dm2;
-  // Smoothness indicators beta_r = 13/12 t_r^2 + 1/4 s_r^2 (s_r / 2 below).
-  const float s0 = 0.5f * d1 - 1.5f * d0;
-  const float s1 = -0.5f * (d0 + dm1);
-  const float s2 = 1.5f * dm1 - 0.5f * dm2;
-  const float b0 = c1312 * (t0 * t0) + s0 * s0;
-  const float b1 = c1312 * (t1 * t1) + s1 * s1;
-  const float b2 = c1312 * (t2 * t2) + s2 * s2;
+  // Smoothness indicators beta_r = 13/12 t_r^2 + 1/4 s_r^2 with s_r the
+  // one-sided slopes (d1 - 3 d0, d0 + dm1, 3 dm1 - dm2 are 2 s_r).  The
+  // nonlinear weights d_r / (eps + beta_r)^2, normalised, do not change when
+  // every eps + beta_r is scaled by one factor, so the kernels form
+  // E_r = 4 (eps + beta_r) = 13/3 t_r^2 + (2 s_r)^2 + 4 eps directly.
+  const float u0 = __builtin_fmaf(-3.0f, d0, d1);
+  const float u1 = d0 + dm1;
+  const float u2 = __builtin_fmaf(3.0f, dm1, -dm2);
+  const float c133 = (float)(13.0 / 3.0), eps4 = 4.0f * kWenoEps;
+  const float E0 = __builtin_fmaf(c133 * t0, t0, __builtin_fmaf(u0, u0, eps4));
+  const float E1 = __builtin_fmaf(c133 * t1, t1, __builtin_fmaf(u1, u1, eps4));
+  const float E2 = __builtin_fmaf(c133 * t2, t2, __builtin_fmaf(u2, u2, eps4));
   // Candidate values of the prototype's c[k][r] stencils from the
   // differences (q = v0 +- d / 2 + t * {1/3, -1/6}; a stencil of equal values
-  // gives that value exactly), then the standard weighted average, whose
-  // rounding keeps a result near zero as small, and of the same sign, as the
-  // reference's (the dynamic pressure next to its zero halo cells feeds a
-  // square root).
-  const float c13 = (float)(1.0 / 3.0);
+  // gives that value exactly), then the standard weighted average with
+  // positive weights, whose rounding keeps a result near zero as small, and
+  // of the same sign, as the reference's (the dynamic pressure next to its
+  // zero halo cells feeds a square root).
   const float hn0 = v0 + 0.5f * d0, hn1 = v0 + 0.5f * dm1;
   const float hp0 = v0 - 0.5f * d0, hp1 = v0 - 0.5f * dm1;
   const float qn0 = hn0 - c16 * t0, qn1 = hn0 - c16 * t1, qn2 = hn1 + c13 * t2;
   const float qp0 = hp0 + c13 * t0, qp1 = hp1 - c16 * t1, qp2 = hp1 - c16 * t2;
-  // alpha_r = d_r / (eps + beta_r)^2: one reciprocal per stencil serves the
-  // weights of both faces; f = sum(alpha_r q_r) / sum(alpha_r).
-  const float e0 = kWenoEps + b0, e1 = kWenoEps + b1, e2 = kWenoEps + b2;
-  const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1), i2 = frcp(e2 * e2);
-  const float an0 = 0.3f * i0, an1 = 0.6f * i1, an2 = 0.1f * i2;
-  const float ap0 = 0.1f * i0, ap1 = 0.6f * i1, ap2 = 0.3f * i2;
-  const float rn = frcp((an0 + an1) + an2);
-  const float rp = frcp((ap0 + ap1) + ap2);
-  fp = ((an0 * qn0 + an1 * qn1) + an2 * qn2) * rn;
-  fm = ((ap0 * qp0 + ap1 * qp1) + ap2 * qp2) * rp;
+  // alpha_r = d_r / E_r^2 with d = {0.3, 0.6, 0.1} (face c+1/2) and
+  // {0.1, 0.6, 0.3} (face c-1/2), both scaled by 10 (again a common factor):
+  // one reciprocal per stencil serves both faces.
+  const float i0 = frcp(E0 * E0), i1 = frcp(E1 * E1), i2 = frcp(E2 * E2);
+  const float a1 = 6.0f * i1, an0 = 3.0f * i0, ap2 = 3.0f * i2;
+  const float rn = frcp((an0 + a1) + i2);
+  const float rp = frcp((i0 + a1) + ap2);
+  fp = ((an0 * qn0 + a1 * qn1) + i2 * qn2) * rn;
+  fm = ((i0 * qp0 + a1 * qp1) + ap2 * qp2) * rp;
 }
 
 template <>
 __device__ __forceinline__ void recon<kWeno3>(const float* a, float& fp, float& fm) {
   const float vm1 = a[0], v0 = a[1], vp1 = a[2];
-  float t = vp1 - v0;
-  const float b0 = t * t;
-  t = v0 - vm1;
-  const float b1 = t * t;
+  const float dp = vp1 - v0, dm = v0 - vm1;
+  // eps + beta_r with beta_0 = (v_{+1} - v_0)^2, beta_1 = (v_0 - v_{-1})^2.
+  const float e0 = __builtin_fmaf(dp, dp, kWenoEps), e1 = __builtin_fmaf(dm, dm, kWenoEps);
   const float qn0 = 0.5f * v0 + 0.5f * vp1;
   const float qn1 = -0.5f * vm1 + 1.5f * v0;
   const float qp0 = 1.5f * v0 + -0.5f * vp1;
   const float qp1 = 0.5f * vm1 + 0.5f * v0;
-  const float d0 = (float)(2.0 / 3.0), d1 = (float)(1.0 / 3.0);
-  float e0 = kWenoEps + b0, e1 = kWenoEps + b1;
+  // alpha_r = d_r / (eps + beta_r)^2 with d = {2/3, 1/3} (face c+1/2) and
+  // {1/3, 2/3} (face c-1/2), both scaled by 3 (a common factor).
   const float i0 = frcp(e0 * e0), i1 = frcp(e1 * e1);
-  const float an0 = d0 * i0, an1 = d1 * i1;
-  const float ap0 = d1 * i0, ap1 = d0 * i1;
-  fp = (an0 * qn0 + an1 * qn1) * frcp(an0 + an1);
-  fm = (ap0 * qp0 + ap1 * qp1) * frcp(ap0 + ap1);
+  const float an0 = 2.0f * i0, ap1 = 2.0f * i1;
+  fp = (an0 * qn0 + i1 * qn1) * frcp(an0 + i1);
+  fm = (i0 * qp0 + ap1 * qp1) * frcp(i0 + ap1);
 }
 
 __device__ __forceinline__ float sgnf(float x) {
@@ -226,25 +227,32 @@ __device__ __forceinline__ void recon<kMuscl>(const float* a, float& fp, float& 
 // which replaces p in the aligned momentum flux (helper_var 'p_dyn',
 // compressible_navier_stokes.py:366-372).
 template <int D, int NS, bool PD = false>
-__device__ __forceinline__ void physical_flux(const float* s, float* f) {
+__device__ __forceinline__ void physical_flux_inv(const float* s, float is, float* f) {
   const float m = s[kRho] * s[kU + D];
   const float pm = PD ? s[7 + NS] : s[kP];
   f[0] = m;
   f[1] = D == 0 ? m * s[kU] + pm : m * s[kU];
   f[2] = D == 1 ? m * s[kV] + pm : m * s[kV];
   f[3] = D == 2 ? m * s[kW] + pm : m * s[kW];
-  f[4] = m * (s[kE] + s[kP] * frcp(s[kRho]));
+  f[4] = m * (s[kE] + s[kP] * is);
 #pragma unroll
   for (int k = 0; k < NS; ++k) f[5 + k] = m * s[7 + k];
 }
 
+// is = 1 / rho of the side (the callers share it with the wave speeds).
+template <int D, int NS, bool PD = false>
+__device__ __forceinline__ void physical_flux(const float* s, float* f) {
+  physical_flux_inv<D, NS, PD>(s, frcp(s[kRho]), f);
+}
+
 // Roe / Einfeldt wave-speed estimates: hll.py:178-283, sound speeds
 // generic.py:90-159 (opt 'p_rho' per side, opt 'h' for the Roe state).
+// iL, iR: 1 / rho of the two sides.
 template <int D>
-__device__ __forceinline__ void roe_speeds(const float* L, const float* R,
-                                           float pe, float& sl, float& sr) {
-  const float cl = fsqrt(kGamma * L[kP] * frcp(L[kRho]));
-  const float cr = fsqrt(kGamma * R[kP] * frcp(R[kRho]));
+__device__ __forceinline__ void roe_speeds_inv(const float* L, const float* R, float iL,
+                                               float iR, float pe, float& sl, float& sr) {
+  const float cl = fsqrt(kGamma * L[kP] * iL);
+  const float cr = fsqrt(kGamma * R[kP] * iR);
   const float ql = fsqrt(L[kRho]);
   const float qr = fsqrt(R[kRho]);
   const float inv = rnn(ql + qr);
@@ -259,21 +267,28 @@ __device__ __forceinline__ void roe_speeds(const float* L, const float* R,
   sr = tf_max(R[kU + D] + cr, un + c);
 }
 
+template <int D>
+__device__ __forceinline__ void roe_speeds(const float* L, const float* R,
+                                           float pe, float& sl, float& sr) {
+  roe_speeds_inv<D>(L, R, frcp(L[kRho]), frcp(R[kRho]), pe, sl, sr);
+}
+
 // HLL flux: hll.py:36-175.
 template <int D, int NS, bool PD = false>
 __device__ __forceinline__ void hll_flux(const float* L, const float* R, float pe,
                                          float* F) {
   constexpr int NF = 5 + NS;
   float sl, sr;
-  roe_speeds<D>(L, R, pe, sl, sr);
+  const float iL = frcp(L[kRho]), iR = frcp(R[kRho]);
+  roe_speeds_inv<D>(L, R, iL, iR, pe, sl, sr);
   const float ds = sr - sl;
   const float rds = ds == 0.0f ? 0.0f : frcp(ds);
   const float t1 = (tf_min(sr, 0.0f) - tf_min(sl, 0.0f)) * rds;
   const float t2 = 1.0f - t1;
   const float t3 = (sr * fabsf(sl) - sl * fabsf(sr)) * (0.5f * rds);
   float fl[NF], fr[NF], ul[NF], ur[NF];
-  physical_flux<D, NS, PD>(L, fl);
-  physical_flux<D, NS, PD>(R, fr);
+  physical_flux_inv<D, NS, PD>(L, iL, fl);
+  physical_flux_inv<D, NS, PD>(R, iR, fr);
   ul[0] = L[kRho];
   ur[0] = R[kRho];
   ul[1] = L[kRho] * L[kU];
@@ -295,7 +310,7 @@ __device__ __forceinline__ void hll_flux(const float* L, const float* R, float p
 
 // HLLC star-state flux of one side: hllc.py:134-192 (Toro eq. 10.38).
 template <int D, int NS>
-__device__ __forceinline__ void hllc_star(const float* S, float sk, float sm,
+__device__ __forceinline__ void hllc_star(const float* S, float is, float sk, float sm,
                                           const float* f, float* fs) {
   const float rho = S[kRho], vn = S[kU + D];
   const float den = sk - sm;
@@ -312,7 +327,7 @@ __device__ __forceinline__ void hllc_star(const float* S, float sk, float sm,
     }
   }
   fs[4] = f[4] + ((rho * q) * (sm - vn)) *
-                     ((S[kE] + S[kP] * frcp(rho)) + (sk - vn) * sm);
+                     ((S[kE] + S[kP] * is) + (sk - vn) * sm);
 #pragma unroll
   for (int k = 0; k < NS; ++k) fs[5 + k] = f[5 + k] + (a * S[7 + k]) * q;
 }
@@ -342,7 +357,8 @@ __device__ __forceinline__ int hllc_flux(const float* L0, const float* R0, float
     R = Rp;
   }
   float sl, sr;
-  roe_speeds<D>(L, R, pe, sl, sr);
+  const float iL = frcp(L[kRho]), iR = frcp(R[kRho]);
+  roe_speeds_inv<D>(L, R, iL, iR, pe, sl, sr);
   const float vl = L[kU + D], vr = R[kU + D];
   const float rl = L[kRho], rr = R[kRho];
   float sm = dnn(((R[kP] - L[kP]) + (rl * vl) * (sl - vl)) - (rr * vr) * (sr - vr),
@@ -369,10 +385,11 @@ __device__ __forceinline__ int hllc_flux(const float* L0, const float* R0, float
     Ss[v] = left ? L[v] : R[v];
   }
   const float sk = left ? sl : sr;
+  const float is = left ? iL : iR;
   float f[NF];
-  physical_flux<D, NS, PD>(S0, f);
+  physical_flux_inv<D, NS, PD>(S0, is, f);
   if (br == 1 || br == 2) {
-    hllc_star<D, NS>(Ss, sk, sm, f, F);
+    hllc_star<D, NS>(Ss, is, sk, sm, f, F);
   } else {
 #pragma unroll
     for (int k = 0; k < NF; ++k) F[k] = f[k];
@@ -408,11 +425,22 @@ struct ViscConst {
   float inv_two_h[3];  // 1/(2 dx), 1/(2 dy), 1/(2 dz)
 };
 
+// Per-cell factors of the heat flux: T = p / (rho R) and kappa / nu-free
+// part k = ((1/Pr) cp) (nu rho) (simple.py:104-108); a marching sweep computes
+// them once per cell and hands cell c's values to the next face.
+struct ViscCell {
+  float T, k;
+};
+__device__ __forceinline__ ViscCell visc_cell(const float* W, const ViscConst& vc) {
+  return ViscCell{W[kP] * frcp(W[kRho] * kR), vc.kfac * (vc.nu * W[kRho])};
+}
+
 template <int D, int NS>
-__device__ __forceinline__ void viscous_flux(const float* A, const float* B,
-                                             const float (*ta)[2],
-                                             const float (*tb)[2],
-                                             const ViscConst& vc, float* F) {
+__device__ __forceinline__ void viscous_flux_cells(const float* A, const float* B,
+                                                   const float (*ta)[2],
+                                                   const float (*tb)[2], ViscCell ca,
+                                                   ViscCell cb, const ViscConst& vc,
+                                                   float* F) {
   const float ih = vc.inv_h[D];
   // Backward (normal) derivatives on the face.
   const float gu = (B[kU] - A[kU]) * ih;
@@ -462,16 +490,20 @@ __device__ __forceinline__ void viscous_flux(const float* A, const float* B,
   const float vf = 0.5f * (A[kV] + B[kV]);
   const float wf = 0.5f * (A[kW] + B[kW]);
   // Heat flux -kappa_f * dT/dx, T = p/(rho R), kappa = ((1/Pr)*cp)*(nu*rho).
-  const float Ta = A[kP] * frcp(A[kRho] * kR);
-  const float Tb = B[kP] * frcp(B[kRho] * kR);
-  const float ka = vc.kfac * (vc.nu * A[kRho]);
-  const float kb = vc.kfac * (vc.nu * B[kRho]);
-  const float kf = 0.5f * (ka + kb);
-  const float q = -kf * ((Tb - Ta) * ih);
+  const float kf = 0.5f * (ca.k + cb.k);
+  const float q = -kf * ((cb.T - ca.T) * ih);
   F[1] = F[1] + (-tx);
   F[2] = F[2] + (-ty);
   F[3] = F[3] + (-tz);
   F[4] = F[4] + (q + (-((uf * tx + vf * ty) + wf * tz)));
+}
+
+template <int D, int NS>
+__device__ __forceinline__ void viscous_flux(const float* A, const float* B,
+                                             const float (*ta)[2],
+                                             const float (*tb)[2],
+                                             const ViscConst& vc, float* F) {
+  viscous_flux_cells<D, NS>(A, B, ta, tb, visc_cell(A, vc), visc_cell(B, vc), vc, F);
 }
 
 }  // namespace sc

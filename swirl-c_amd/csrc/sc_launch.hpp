@@ -23,7 +23,8 @@ hipError_t launch_branch_codes(const SweepArgs& a, int dim, int ns,
 // Generic body, instantiated once per scheme in sc_sweep_<scheme>.hip.
 // PD: reference-state (p_ref / rho_ref) build; BB: boundary face / flux
 // value build.
-template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
+          bool LEAN = false>
 hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
                                const SweepTuning& tune, hipStream_t stream,
                                int only_dir) {
@@ -35,7 +36,7 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     ax.seg_len = tune.seg_x;
     const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_x - 1) / tune.by_x,
                     (a.nx + tune.seg_x - 1) / tune.seg_x);
-    hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB>), grid,
+    hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB, LEAN>), grid,
                        dim3(64, tune.by_x), 0, stream, ax);
   }
   // z sweep (middle): R -= div_z.
@@ -45,7 +46,7 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     az.seg_len = tune.seg_z;
     const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
                     (a.nz + tune.seg_z - 1) / tune.seg_z);
-    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, PD, BB>), grid,
+    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, PD, BB, LEAN>), grid,
                        dim3(64, tune.by_z), 0, stream, az);
   }
   // y sweep (last): R -= div_y, then store R or the RK update.
@@ -54,18 +55,26 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     ay.inv_hdir = 1.0f / h[1];
     const int nwin = y_windows(a.nx, a.ny, a.h, YGeom<SCHEME>::OUTW);
     const dim3 grid((nwin + tune.by_y - 1) / tune.by_y, 1, a.nz);
-    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB>), grid,
+    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>), grid,
                        dim3(64, tune.by_y), 0, stream, ay);
   }
   return hipGetLastError();
 }
 
 // Picks the build for the configuration: boundary values (BB) or a reference
-// state (PD, scalars-free runs only) or neither.
+// state (PD, scalars-free runs only) or neither; without either and without
+// gravity or primitive-variable BCs, the LEAN build.
 template <int SCHEME, int FLUX, bool VISC>
 hipError_t launch_rhs_variant(const SweepArgs& a, const float h[3], int ns,
                               const SweepTuning& tune, hipStream_t stream, int od) {
   const bool bb = a.bbc_dims != 0;
+  if (!bb && a.p_ref == nullptr && a.prim_bc_mask == 0 && !a.has_pe && !a.grav_src) {
+    if (ns == 0)
+      return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, false, true>(a, h, tune, stream, od);
+    if (ns == 1)
+      return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, false, true>(a, h, tune, stream, od);
+    return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 2, false, false, true>(a, h, tune, stream, od);
+  }
   if (ns == 0) {
     if (a.p_ref != nullptr)
       return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, true, true>(a, h, tune, stream, od)

@@ -77,9 +77,9 @@ __global__ void __launch_bounds__(256) riemann_face_kernel(const RiemannArgs a) 
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] = fl[f];
     } else if (sl <= 0.0f && sm >= 0.0f) {
-      hllc_star<D, NS>(L, sl, sm, fl, F);
+      hllc_star<D, NS>(L, frcp(L[kRho]), sl, sm, fl, F);
     } else if (sm <= 0.0f && sr >= 0.0f) {
-      hllc_star<D, NS>(R, sr, sm, fr, F);
+      hllc_star<D, NS>(R, frcp(R[kRho]), sr, sm, fr, F);
     } else {
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] = fr[f];

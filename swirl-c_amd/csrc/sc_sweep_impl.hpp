@@ -121,7 +121,11 @@ __device__ __forceinline__ TileIdx tile_of_block(int remap) {
   return TileIdx{bx, by, bz, (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
 }
 
+// LEAN builds (no gravitational potential, no gravity source, no BC given on
+// a primitive) drop those run-time checks from the kernels.
+template <bool LEAN = false>
 __device__ __forceinline__ float pe_at(const SweepArgs& a, int k, int i, int j) {
+  if constexpr (LEAN) return 0.0f;
   return a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
 }
 
@@ -134,8 +138,9 @@ __device__ __forceinline__ void load_raw(const SweepArgs& a, long long idx, floa
   for (int f = 0; f < NF; ++f) q[f] = a.q.p[f][idx];
 }
 
-template <int NF>
+template <int NF, bool LEAN = false>
 __device__ __forceinline__ void prim_bc_round(const SweepArgs& a, bool interior, float* q) {
+  if constexpr (LEAN) return;
   if (a.prim_bc_mask && interior) {
 #pragma unroll
     for (int f = 1; f < NF; ++f)
@@ -143,11 +148,11 @@ __device__ __forceinline__ void prim_bc_round(const SweepArgs& a, bool interior,
   }
 }
 
-template <int NF>
+template <int NF, bool LEAN = false>
 __device__ __forceinline__ void load_cons(const SweepArgs& a, long long idx,
                                           bool interior, float* q) {
   load_raw<NF>(a, idx, q);
-  prim_bc_round<NF>(a, interior, q);
+  prim_bc_round<NF, LEAN>(a, interior, q);
 }
 
 // Dynamic pressure of a cell (slot 7 + NS): exchange_halos with zero
@@ -158,24 +163,25 @@ __device__ __forceinline__ void set_pdyn(const SweepArgs& a, long long idx, bool
   if constexpr (PD) w[7 + NS] = interior ? w[kP] - a.p_ref[idx] : 0.0f;
 }
 
-template <int NS, bool PD = false>
+template <int NS, bool PD = false, bool LEAN = false>
 __device__ __forceinline__ void load_prim(const SweepArgs& a, long long idx,
                                           bool interior, float pe, float* w) {
   constexpr int NF = 5 + NS;
   float q[NF];
-  load_cons<NF>(a, idx, interior, q);
+  load_cons<NF, LEAN>(a, idx, interior, q);
   cons_to_prim<NS>(q, pe, w);
   set_pdyn<NS, PD>(a, idx, interior, w);
 }
 
 // Velocity components u_c0 and u_c1 at a cell (for transverse derivatives).
+template <bool LEAN = false>
 __device__ __forceinline__ void load_vel2(const SweepArgs& a, long long idx,
                                           bool interior, int c0, int c1,
                                           float& v0, float& v1) {
   const float rho = a.q.p[0][idx];
   float m0 = a.q.p[1 + c0][idx];
   float m1 = a.q.p[1 + c1][idx];
-  if (a.prim_bc_mask && interior) {
+  if (!LEAN && a.prim_bc_mask && interior) {
     if (a.prim_bc_mask & (1 << (1 + c0))) m0 = rho * dnn_exact(m0, rho);
     if (a.prim_bc_mask & (1 << (1 + c1))) m1 = rho * dnn_exact(m1, rho);
   }
@@ -191,7 +197,7 @@ __device__ __forceinline__ bool in_core(int c, int h, int n) {
 // Central transverse derivatives at cell (k, i, j) for a face normal to D:
 // out[t][0] = d(u_t)/dx_t, out[t][1] = d(u_D)/dx_t for the transverse dims t
 // in x, y, z order (physics/fluid.py:52-63 with numerics/gradient.py:71-94).
-template <int D>
+template <int D, bool LEAN = false>
 __device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int i,
                                                  int j, float (*out)[2]) {
   int t = 0;
@@ -218,8 +224,8 @@ __device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int 
       inm = in_core(i, a.h, a.nx) && in_core(j, a.h, a.ny) && in_core(k - 1, a.h, a.nz);
     }
     float ap0, ap1, am0, am1;
-    load_vel2(a, sp, inp, dd, D, ap0, ap1);
-    load_vel2(a, sm, inm, dd, D, am0, am1);
+    load_vel2<LEAN>(a, sp, inp, dd, D, ap0, ap1);
+    load_vel2<LEAN>(a, sm, inm, dd, D, am0, am1);
     const float ith = a.vc.inv_two_h[dd];
     out[t][0] = (ap0 - am0) * ith;
     out[t][1] = (ap1 - am1) * ith;
@@ -229,7 +235,9 @@ __device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int 
 
 // Gravity source of field f at a cell (physics_models.py:94-143,
 // gravity.py:64-102): (rho * G) * g_d on the selected momenta, else 0.
+template <bool LEAN = false>
 __device__ __forceinline__ float source_of(const SweepArgs& a, int f, float rho) {
+  if constexpr (LEAN) return 0.0f;
   if (a.grav_src && f >= 1 && f <= 3 && (a.grav_mask & (1 << (f - 1)))) {
     return (rho * kG) * a.gdir[f - 1];
   }
@@ -277,6 +285,7 @@ template <int DIR, int FLUX, bool VISC, int NS, bool PD>
 __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* R, float pe,
                                           const float* A, const float* B,
                                           const float (*ta)[2], const float (*tb)[2],
+                                          ViscCell ca, ViscCell cb,
                                           const ViscConst& vc, float* F) {
   constexpr int NF = 5 + NS;
   if (b != nullptr) {
@@ -290,12 +299,12 @@ __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* 
       float Fd[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) Fd[f] = 0.0f;
-      viscous_flux<DIR, NS>(A, B, ta, tb, vc, Fd);
+      viscous_flux_cells<DIR, NS>(A, B, ta, tb, ca, cb, vc, Fd);
       apply_flux_bc<NF>(*b, 1, Fd);
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] = F[f] + Fd[f];
     } else {
-      viscous_flux<DIR, NS>(A, B, ta, tb, vc, F);
+      viscous_flux_cells<DIR, NS>(A, B, ta, tb, ca, cb, vc, F);
     }
   }
   if (b != nullptr) apply_flux_bc<NF>(*b, 2, F);
@@ -312,7 +321,7 @@ __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* 
 // BB: build with the boundary face / flux value handling (only launched when
 // the configuration has such values, so other runs carry none of its code).
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
-          bool BB = false>
+          bool BB = false, bool LEAN = false>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
@@ -340,12 +349,13 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 #pragma unroll
   for (int w = 0; w < WN - 1; ++w) {
     const int c = c_begin - 1 - RAD + w;
-    load_prim<NS, PD>(a, base + (long long)c * sm, in_core(c, a.h, n_march),
-                      pe_at(a, kk(c), ii(c), j), W[w]);
+    load_prim<NS, PD, LEAN>(a, base + (long long)c * sm, in_core(c, a.h, n_march),
+                            pe_at<LEAN>(a, kk(c), ii(c), j), W[w]);
   }
   float negPrev[NV];
   float Fprev[NF];
   float tPrev[2][2];
+  ViscCell vPrev{0.0f, 0.0f};  // heat-flux factors of cell c - 1
 #pragma unroll
   for (int v = 0; v < NV; ++v) negPrev[v] = 0.0f;
 #pragma unroll
@@ -376,8 +386,8 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 #pragma unroll
       for (int f = 0; f < NF; ++f) qc[f] = qn[f];
       if (c < c_end) load_raw<NF>(a, base + (long long)(cn + 1) * sm, qn);
-      prim_bc_round<NF>(a, in_core(cn, a.h, n_march), qc);
-      cons_to_prim<NS>(qc, pe_at(a, kk(cn), ii(cn), j), W[WN - 1]);
+      prim_bc_round<NF, LEAN>(a, in_core(cn, a.h, n_march), qc);
+      cons_to_prim<NS>(qc, pe_at<LEAN>(a, kk(cn), ii(cn), j), W[WN - 1]);
       set_pdyn<NS, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n_march), W[WN - 1]);
     }
     float negP[NV], posM[NV];
@@ -389,14 +399,19 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
       recon<SCHEME>(s, negP[v], posM[v]);
     }
     float tCur[2][2];
-    if constexpr (VISC) transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
+    ViscCell vCur{0.0f, 0.0f};
+    if constexpr (VISC) {
+      transverse_grads<DIR, LEAN>(a, kk(c), ii(c), j, tCur);
+      vCur = visc_cell(W[RAD], a.vc);
+    }
     if (c >= c_begin) {
       float F[NF];
       const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
       const BoundaryBc* bb =
           (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
-      face_flux<DIR, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, kk(c), ii(c), j),
-                                         W[RAD - 1], W[RAD], tPrev, tCur, a.vc, F);
+      face_flux<DIR, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at<LEAN>(a, kk(c), ii(c), j),
+                                         W[RAD - 1], W[RAD], tPrev, tCur, vPrev, vCur, a.vc,
+                                         F);
       if (c > c_begin) {
         const long long idx = base + (long long)(c - 1) * sm;
         if constexpr (PHASE == 0) {
@@ -404,7 +419,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
           const float rho = PD ? W[RAD - 1][kRho] - a.rho_ref[idx] : W[RAD - 1][kRho];
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            a.r.p[f][idx] = source_of(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
+            a.r.p[f][idx] = source_of<LEAN>(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
           }
         } else {
 #pragma unroll
@@ -421,6 +436,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
       tPrev[0][1] = tCur[0][1];
       tPrev[1][0] = tCur[1][0];
       tPrev[1][1] = tCur[1][1];
+      vPrev = vCur;
     }
 #pragma unroll
     for (int w = 0; w < WN - 1; ++w) {
@@ -431,11 +447,11 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 }
 
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
-          bool BB = false>
+          bool BB = false, bool LEAN = false>
 __global__ void __launch_bounds__(256)
 sweep_march_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
-  sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, PD, BB>(a, ti);
+  sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, PD, BB, LEAN>(a, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -464,7 +480,8 @@ __host__ __device__ inline int y_windows(int nx, int ny, int h, int outw) {
   return (int)((run - 2 * h + outw - 1) / outw);
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
+          bool LEAN = false>
 __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = YGeom<SCHEME>::RAD;
   constexpr int K = YGeom<SCHEME>::K;
@@ -508,7 +525,7 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   }
   const float dt = rk ? *a.dt : 0.0f;
   float W0[NV];
-  load_prim<NS, PD>(a, idx, in_core(yc, a.h, a.ny), pe_at(a, k, i, yc), W0);
+  load_prim<NS, PD, LEAN>(a, idx, in_core(yc, a.h, a.ny), pe_at<LEAN>(a, k, i, yc), W0);
   float negP[NV], posM[NV], Am[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -526,7 +543,12 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   for (int v = 0; v < NV; ++v) negPrev[v] = wave_shr1(negP[v]);
   float F[NF];
   float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, tPrev[2][2];
-  if constexpr (VISC) transverse_grads<1>(a, k, i, yc, tCur);
+  ViscCell vCur{0.0f, 0.0f}, vPrev{0.0f, 0.0f};
+  if constexpr (VISC) {
+    transverse_grads<1, LEAN>(a, k, i, yc, tCur);
+    vCur = visc_cell(W0, a.vc);
+    vPrev = ViscCell{wave_shr1(vCur.T), wave_shr1(vCur.k)};
+  }
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     tPrev[t][0] = wave_shr1(tCur[t][0]);
@@ -535,8 +557,8 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   const int side = yc == a.h ? 0 : (yc == a.h + a.ny ? 1 : -1);
   const BoundaryBc* bb =
       (BB && ((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
-  face_flux<1, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at(a, k, i, yc), Am, W0, tPrev,
-                                   tCur, a.vc, F);
+  face_flux<1, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at<LEAN>(a, k, i, yc), Am, W0,
+                                   tPrev, tCur, vPrev, vCur, a.vc, F);
   float Fn[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) Fn[f] = wave_shl1(F[f]);
@@ -553,11 +575,12 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
+          bool LEAN = false>
 __global__ void __launch_bounds__(256)
 sweep_y_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
-  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB>(a, ti);
+  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, ti);
 }
 
 // ---------------------------------------------------------------------------
