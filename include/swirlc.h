@@ -136,8 +136,9 @@ typedef struct sc_ctx sc_ctx;
  * (equations/compressible_navier_stokes.py:46-67) and PhysicsModels
  * (physics/physics_models.py:45-92): invalid scheme names, missing gravity
  * direction and halo widths too small for the stencil fail here, as do
- * grids with nx >= 2^20 or nx * (ny + 2 * halo) >= 2^30 (SC_ERR_UNSUPPORTED:
- * the y sweep addresses a z plane's rows with 32-bit positions). */
+ * grids with nx >= 2^20 or (nx + 2 * halo) * (ny + 2 * halo) >= 2^30
+ * (SC_ERR_UNSUPPORTED: the y sweep addresses a z plane with 32-bit byte
+ * offsets). */
 sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params,
                         sc_ctx** out);
 sc_status sc_ctx_destroy(sc_ctx* ctx);
@@ -165,10 +166,11 @@ sc_status sc_rk_stage(sc_ctx* ctx, int form, float c0, float c1,
 /* No counterpart in the reference: sc_rk_stage for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  Identical results (same kernels); the
- * RHS accumulator is a file-scope 64 MiB device array, one per GPU, held
- * until the process ends, instead of the context's scratch fields.  Limit:
- * n_fields * ((nx+2h)(ny+2h)(nz+2h) rounded up to 64) <= 16777216 floats
- * (e.g. 5 fields of a 128^3 grid with halo 3), else SC_ERR_UNSUPPORTED.  The
+ * RHS accumulator and z-divergence live in a file-scope 128 MiB device
+ * array, one per GPU, held until the process ends, instead of the context's
+ * scratch fields.  Limit: 2 * n_fields * ((nx+2h)(ny+2h)(nz+2h) rounded up
+ * to 64) <= 33554432 floats (e.g. 5 fields of a 128^3 grid with halo 3),
+ * else SC_ERR_UNSUPPORTED.  The
  * context's device must be current.  Allocates nothing; needs no zeroing
  * (the x sweep writes every interior accumulator cell before it is read).
  * Every call on one GPU shares that GPU's array: calls that overlap on one
@@ -292,6 +294,9 @@ typedef struct sc_tuning {
    * pass.  Ignored by batches. */
   int adj_seg, adj_by;
   int adj_by_y;
+  /* y sweep: consecutive 64-lane windows per wave (1..64; the operands of
+   * each next window are loaded while the current one is computed). */
+  int y_nw;
 } sc_tuning;
 
 sc_status sc_ctx_get_tuning(const sc_ctx* ctx, sc_tuning* out);

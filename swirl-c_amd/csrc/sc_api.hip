@@ -29,8 +29,10 @@ struct sc_ctx {
   float* pe_y;
   float* pe_z;
   int grav_mask;
-  // scratch RHS accumulator (device), one full-size array per field
+  // scratch RHS accumulator and z-divergence (device), one full-size array
+  // per field each
   float* rhs_scratch[SC_MAX_FIELDS];
+  float* dz_scratch[SC_MAX_FIELDS];
   unsigned* speed_bits;
   // boundary face / flux values per [dim][side] (device) and dims using them
   sc::BoundaryBc* bbc_dev;
@@ -51,7 +53,7 @@ struct sc_ctx {
 // device address of each GPU's copy is looked up once per device with
 // hipGetSymbolAddress (at context creation, with that device current) and
 // kept in g_scratch_ptr.
-constexpr long long kScratchFloats = 16LL << 20;  // 64 MiB
+constexpr long long kScratchFloats = 32LL << 20;  // 128 MiB
 static __device__ __align__(256) float scratch[kScratchFloats];
 
 namespace {
@@ -103,6 +105,8 @@ sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
     if (v < 1 || v > 4) return fail(SC_ERR_INVALID, "tuning: by_x, by_z, by_y, adj_by, adj_by_y must be in [1, 4]");
   if (t->xcd_remap != 0 && t->xcd_remap != 1)
     return fail(SC_ERR_INVALID, "tuning: xcd_remap must be 0 or 1");
+  if (t->y_nw < 1 || t->y_nw > 64)
+    return fail(SC_ERR_INVALID, "tuning: y_nw must be in [1, 64]");
   out->seg_x = t->seg_x;
   out->by_x = t->by_x;
   out->seg_z = t->seg_z;
@@ -112,6 +116,7 @@ sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
   out->adj_seg = t->adj_seg;
   out->adj_by = t->adj_by;
   out->adj_by_y = t->adj_by_y;
+  out->y_nw = t->y_nw;
   return SC_OK;
 }
 
@@ -169,6 +174,7 @@ void from_tuning(const sc::SweepTuning& t, sc_tuning* out) {
   out->adj_seg = t.adj_seg;
   out->adj_by = t.adj_by;
   out->adj_by_y = t.adj_by_y;
+  out->y_nw = t.y_nw;
 }
 
 }  // namespace
@@ -200,9 +206,10 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   if (h < radius_of(p.interpolation) + 1 || h > SC_MAX_HALO)
     return fail(SC_ERR_INVALID, "halo_width too small for the interpolation stencil"
                                 " (WENO_5 needs 3, WENO_3/MUSCL need 2) or above 4");
-  if (grid->nx >= (1 << 20) || (long long)grid->nx * (grid->ny + 2 * h) >= (1LL << 30))
+  if (grid->nx >= (1 << 20) ||
+      (long long)(grid->nx + 2 * h) * (grid->ny + 2 * h) >= (1LL << 30))
     return fail(SC_ERR_UNSUPPORTED, "grid too large: nx must be below 2^20 and "
-                                    "nx * (ny + 2 * halo_width) below 2^30");
+                                    "(nx + 2 * halo_width) * (ny + 2 * halo_width) below 2^30");
   if (!(grid->dx > 0 && grid->dy > 0 && grid->dz > 0))
     return fail(SC_ERR_INVALID, "grid spacing must be positive");
   const int gdir = gravity_direction(p.g);
@@ -291,8 +298,10 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   }
   c->grav_mask = 0;
   if (p.gravity_source) c->grav_mask = gdir == 3 ? 7 : (1 << gdir);
-  for (int f = 0; f < c->nf && st == SC_OK; ++f)
+  for (int f = 0; f < c->nf && st == SC_OK; ++f) {
     check(hipMalloc((void**)&c->rhs_scratch[f], sizeof(float) * c->cells_full), "hipMalloc rhs");
+    check(hipMalloc((void**)&c->dz_scratch[f], sizeof(float) * c->cells_full), "hipMalloc dz");
+  }
   if (st == SC_OK) check(hipMalloc((void**)&c->speed_bits, sizeof(unsigned) * 4), "hipMalloc speed");
   c->bbc_dims = bbc_dims;
   if (st == SC_OK && bbc_dims) {
@@ -302,8 +311,10 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
             "hipMemcpy boundary values");
   }
   if (st == SC_OK) {
-    for (int f = 0; f < c->nf; ++f)
+    for (int f = 0; f < c->nf; ++f) {
       check(hipMemset(c->rhs_scratch[f], 0, sizeof(float) * c->cells_full), "hipMemset rhs");
+      check(hipMemset(c->dz_scratch[f], 0, sizeof(float) * c->cells_full), "hipMemset dz");
+    }
   }
   if (st != SC_OK) {
     sc_ctx_destroy(c);
@@ -383,6 +394,8 @@ sc_status sc_ctx_destroy(sc_ctx* c) {
   if (c->pe_z) hipFree(c->pe_z);
   for (int f = 0; f < SC_MAX_FIELDS; ++f)
     if (c->rhs_scratch[f]) hipFree(c->rhs_scratch[f]);
+  for (int f = 0; f < SC_MAX_FIELDS; ++f)
+    if (c->dz_scratch[f]) hipFree(c->dz_scratch[f]);
   if (c->speed_bits) hipFree(c->speed_bits);
   if (c->bbc_dev) hipFree(c->bbc_dev);
   for (int f = 0; f < 5; ++f)
@@ -438,6 +451,7 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
   for (int f = 0; f < c->nf; ++f) {
     a.q.p[f] = q[f];
     a.r.p[f] = rhs[f];
+    a.dz.p[f] = c->dz_scratch[f];
   }
   a.rk_form = 0;
   return run_sweeps(c, a, (hipStream_t)stream);
@@ -446,25 +460,26 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               void* stream);
+                               float* const* dacc, void* stream);
 
 sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                       const float* const* u0, float* const* uk, float* const* out,
                       void* stream) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, stream);
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, c->dz_scratch,
+                       stream);
 }
 
 sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                               const float* const* u0, float* const* uk, float* const* out,
                               void* stream) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage_noalloc: null argument");
-  // Each field's accumulator starts on a 256-byte boundary of the array.
+  // Each field's accumulator and z-divergence start on a 256-byte boundary.
   const long long stride = (c->cells_full + 63) / 64 * 64;
-  if (stride * c->nf > kScratchFloats)
+  if (2 * stride * c->nf > kScratchFloats)
     return fail(SC_ERR_UNSUPPORTED,
-                "sc_rk_stage_noalloc: grid too large for the 64 MiB scratch array (needs"
-                " n_fields * (nx+2h)(ny+2h)(nz+2h), each rounded up to 64, <= 16777216"
+                "sc_rk_stage_noalloc: grid too large for the 128 MiB scratch array (needs"
+                " 2 * n_fields * (nx+2h)(ny+2h)(nz+2h), each rounded up to 64, <= 33554432"
                 " floats); use sc_rk_stage");
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess || dev != c->device)
@@ -473,14 +488,18 @@ sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const flo
   if (base == nullptr)
     return fail(SC_ERR_HIP, "sc_rk_stage_noalloc: scratch array address not resolved");
   float* racc[SC_MAX_FIELDS] = {};
-  for (int f = 0; f < c->nf; ++f) racc[f] = base + f * stride;
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, stream);
+  float* dacc[SC_MAX_FIELDS] = {};
+  for (int f = 0; f < c->nf; ++f) {
+    racc[f] = base + f * stride;
+    dacc[f] = base + (c->nf + f) * stride;
+  }
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, dacc, stream);
 }
 
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               void* stream) {
+                               float* const* dacc, void* stream) {
   if (c == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
     return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
   if (form == SC_RK_AVG)
@@ -506,6 +525,7 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
     a.q.p[f] = uk[f];
     a.u0.p[f] = form == SC_RK_EULER ? nullptr : u0[f];
     a.r.p[f] = racc[f];
+    a.dz.p[f] = dacc[f];
     a.out.p[f] = out[f];
   }
   a.rk_form = form;
@@ -527,6 +547,7 @@ sc_status sc_sweep_only(sc_ctx* c, int dim, int form, float c0, float c1,
     a.q.p[f] = uk[f];
     a.u0.p[f] = (form == 0 || form == SC_RK_EULER) ? nullptr : u0[f];
     a.r.p[f] = c->rhs_scratch[f];
+    a.dz.p[f] = c->dz_scratch[f];
     a.out.p[f] = form == 0 ? nullptr : out[f];
   }
   a.rk_form = form;
@@ -984,6 +1005,7 @@ struct sc_batch {
   float* hmin_dev = nullptr;
   unsigned* speed_bits = nullptr;
   float* r_arena[SC_MAX_FIELDS] = {};
+  float* dz_arena[SC_MAX_FIELDS] = {};
   sc::HaloArgs halo_cfg;
   sc::SweepArgs proto;
   sc::SweepTuning tune;
@@ -1075,6 +1097,7 @@ sc_status sc_batch_destroy(sc_batch* b) {
   hipFree(b->hmin_dev);
   hipFree(b->speed_bits);
   for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->r_arena[f]);
+  for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->dz_arena[f]);
   delete b;
   return SC_OK;
 }
@@ -1114,9 +1137,9 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
       return fail(SC_ERR_INVALID, "instance extents and spacings must be positive");
     if (halo < radius_of(p.interpolation) + 1 || halo > SC_MAX_HALO)
       return fail(SC_ERR_INVALID, "halo_width too small for the interpolation stencil");
-    if (g.nx >= (1 << 20) || (long long)g.nx * (g.ny + 2 * halo) >= (1LL << 30))
+    if (g.nx >= (1 << 20) || (long long)(g.nx + 2 * halo) * (g.ny + 2 * halo) >= (1LL << 30))
       return fail(SC_ERR_UNSUPPORTED, "batched instance too large: nx must be below 2^20 and "
-                                      "nx * (ny + 2 * halo_width) below 2^30");
+                                      "(nx + 2 * halo_width) * (ny + 2 * halo_width) below 2^30");
     const int nn[3] = {g.nx, g.ny, g.nz};
     for (int f = 0; f < 5; ++f)
       for (int d = 0; d < 3; ++d)
@@ -1198,6 +1221,10 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     st = hip_check(hipMalloc((void**)&b->r_arena[f], sizeof(float) * b->total), "hipMalloc R");
     if (st == SC_OK)
       st = hip_check(hipMemset(b->r_arena[f], 0, sizeof(float) * b->total), "hipMemset R");
+    if (st == SC_OK)
+      st = hip_check(hipMalloc((void**)&b->dz_arena[f], sizeof(float) * b->total), "hipMalloc Dz");
+    if (st == SC_OK)
+      st = hip_check(hipMemset(b->dz_arena[f], 0, sizeof(float) * b->total), "hipMemset Dz");
   }
   if (st != SC_OK) {
     sc_batch_destroy(b);
@@ -1273,6 +1300,7 @@ sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const flo
     ba.q.p[f] = uk[f];
     ba.u0.p[f] = form == SC_RK_BLEND ? u0[f] : nullptr;
     ba.r.p[f] = b->r_arena[f];
+    ba.dz.p[f] = b->dz_arena[f];
     ba.out.p[f] = out[f];
   }
   ba.inst = b->inst_dev;

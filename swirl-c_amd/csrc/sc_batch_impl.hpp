@@ -32,6 +32,7 @@ struct BatchArgs {
   CFields q;        // arena bases
   CFields u0;
   Fields r;
+  Fields dz;
   Fields out;
   int nf;
   const BatchInst* inst;
@@ -49,6 +50,7 @@ __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int d
     a.q.p[f] = on && b.q.p[f] ? b.q.p[f] + in.off : nullptr;
     a.u0.p[f] = on && b.u0.p[f] ? b.u0.p[f] + in.off : nullptr;
     a.r.p[f] = on && b.r.p[f] ? b.r.p[f] + in.off : nullptr;
+    a.dz.p[f] = on && b.dz.p[f] ? b.dz.p[f] + in.off : nullptr;
     a.out.p[f] = on && b.out.p[f] ? b.out.p[f] + in.off : nullptr;
   }
   a.nx = in.nx;
@@ -88,7 +90,7 @@ __global__ void __launch_bounds__(256) batch_y_kernel(const BatchArgs b) {
     const SweepArgs a = tile_args(b, tl.inst, 1);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_y_body<SCHEME, FLUX, VISC, NS, false, false, true>(a, ti);
+    sweep_y_body<SCHEME, FLUX, VISC, NS, false, false, true>(a, ti, 1);
   }
 }
 
@@ -115,7 +117,7 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
     bz.proto.seg_len = tune.seg_z;
     bz.tiles = tiles.t[2];
     bz.ntiles = tiles.n[2];
-    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 1>),
+    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>),
                        dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
   }
   {

@@ -39,22 +39,24 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB, LEAN>), grid,
                        dim3(64, tune.by_x), 0, stream, ax);
   }
-  // z sweep (middle): R -= div_z.
+  // z sweep (middle): Dz = div_z.
   if (only_dir < 0 || only_dir == 2) {
     SweepArgs az = a;
     az.inv_hdir = 1.0f / h[2];
     az.seg_len = tune.seg_z;
     const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
                     (a.nz + tune.seg_z - 1) / tune.seg_z);
-    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 1, PD, BB, LEAN>), grid,
+    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, PD, BB, LEAN>), grid,
                        dim3(64, tune.by_z), 0, stream, az);
   }
-  // y sweep (last): R -= div_y, then store R or the RK update.
+  // y sweep (last): R = (R - Dz) - div_y, then store R or the RK update.
   if (only_dir < 0 || only_dir == 1) {
     SweepArgs ay = a;
     ay.inv_hdir = 1.0f / h[1];
+    ay.y_nw = tune.y_nw;
     const int nwin = y_windows(a.nx, a.ny, a.h, YGeom<SCHEME>::OUTW);
-    const dim3 grid((nwin + tune.by_y - 1) / tune.by_y, 1, a.nz);
+    const int per_block = tune.by_y * tune.y_nw;
+    const dim3 grid((nwin + per_block - 1) / per_block, 1, a.nz);
     hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>), grid,
                        dim3(64, tune.by_y), 0, stream, ay);
   }

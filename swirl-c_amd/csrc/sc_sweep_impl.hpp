@@ -3,11 +3,15 @@
 // One launch per direction.  Each launch reads the conserved fields of the
 // stage state once, converts them to primitives, reconstructs the face states
 // (WENO / MUSCL), solves the Riemann problem (HLL / HLLC), adds the viscous
-// and heat flux, takes the flux difference and accumulates it into the RHS:
+// and heat flux and takes the flux difference:
 //   x sweep:  R  = src - (F_x[i+1] - F_x[i]) / dx          (writes R)
-//   z sweep:  R  = R   - (F_z[k+1] - F_z[k]) / dz          (reads, writes R)
-//   y sweep:  R  = R   - (F_y[j+1] - F_y[j]) / dy, then either writes R or
-//             the Runge-Kutta stage update of the state (RK axpy folded in).
+//   z sweep:  Dz = (F_z[k+1] - F_z[k]) / dz                 (writes Dz)
+//   y sweep:  R  = (R - Dz) - (F_y[j+1] - F_y[j]) / dy, then either writes R
+//             or the Runge-Kutta stage update of the state (RK axpy folded in).
+// The z sweep writes its divergence to an array of its own instead of
+// updating R in place: it then reads nothing but the state, which keeps its
+// registers within three waves per SIMD, and the y sweep, which reads R
+// anyway, reads Dz beside it (the same bytes move as with an in-place update).
 // The reference subtracts the divergences in x, y, z order
 // (equations/compressible_navier_stokes.py:255-264); here y is last because
 // its lanes run along the contiguous axis, so the RK operands (R, U0, Uk) of
@@ -42,6 +46,7 @@ struct SweepTuning {
   int adj_seg = 16;   // adjoint marching passes: owned cells per thread
   int adj_by = 4;     // adjoint marching passes: waves per block
   int adj_by_y = 4;   // adjoint y pass: waves per block
+  int y_nw = 4;       // y sweep: consecutive windows per wave
 };
 
 // Dirichlet values on the domain's boundary faces of one sweep direction
@@ -70,6 +75,7 @@ struct SweepArgs {
   CFields q;    // stage state uk, halos filled
   CFields u0;   // RK base state
   Fields r;     // RHS accumulator
+  Fields dz;    // z-divergence of the stage (written by the z sweep, read by y)
   Fields out;   // RK stage output
   int nx, ny, nz, h;
   int NX, NY, NZ;
@@ -91,6 +97,7 @@ struct SweepArgs {
   int bbc_dims;             // bit d: boundary face/flux values along dim d
   const BoundaryBc* bbc;    // device table [dim][side]
   int xcd_remap;            // XCD-aware block order (see tile_of_block)
+  int y_nw;                 // y sweep: consecutive windows per wave
   // Hydrostatic reference state (helper_var p_ref / rho_ref), or null:
   // p_dyn = p - p_ref (zero in halo cells) replaces p in the momentum flux,
   // d_rho = rho - rho_ref replaces rho in the gravity source
@@ -314,8 +321,7 @@ __device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* 
 // Marching sweep along x (DIR 0) or z (DIR 2).  Block (64, BY): lane -> y,
 // threadIdx.y -> the other transverse index, blockIdx.z -> segment of
 // seg_len cells along the sweep.  PHASE 0: first sweep (writes R = src - div);
-// PHASE 1: middle sweep (R -= div; the R of the next output cell is loaded
-// one iteration ahead).
+// PHASE 2: middle sweep (writes Dz = div).
 // ---------------------------------------------------------------------------
 
 // BB: build with the boundary face / flux value handling (only launched when
@@ -366,20 +372,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   // iteration ahead so the loads overlap the arithmetic of the current cell.
   float qn[NF];
   load_raw<NF>(a, base + (long long)(c_begin - 1 + RAD) * sm, qn);
-  float rpre[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) rpre[f] = 0.0f;
   for (int c = c_begin - 1; c <= c_end; ++c) {
-    float rcur[NF];
-    if constexpr (PHASE == 1) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) rcur[f] = rpre[f];
-      if (c >= c_begin && c < c_end) {
-        const long long ridx = base + (long long)c * sm;
-#pragma unroll
-        for (int f = 0; f < NF; ++f) rpre[f] = a.r.p[f][ridx];
-      }
-    }
     {
       const int cn = c + RAD;
       float qc[NF];
@@ -423,7 +416,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
           }
         } else {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) a.r.p[f][idx] = rcur[f] - (F[f] - Fprev[f]) * a.inv_hdir;
+          for (int f = 0; f < NF; ++f) a.dz.p[f][idx] = (F[f] - Fprev[f]) * a.inv_hdir;
         }
       }
 #pragma unroll
@@ -480,20 +473,21 @@ __host__ __device__ inline int y_windows(int nx, int ny, int h, int outw) {
   return (int)((run - 2 * h + outw - 1) / outw);
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
-          bool LEAN = false>
-__device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti) {
-  constexpr int RAD = YGeom<SCHEME>::RAD;
+// Position of a lane in window `win` of z plane k (see y_windows): idx is
+// its element index in the field, boff its byte offset within plane k (below
+// 2^32: a plane holds fewer than 2^30 cells, checked at context creation), so
+// loads and stores take a wave-uniform plane base and a 32-bit lane offset.
+struct YPos {
+  int yc, i;
+  long long idx;
+  unsigned boff;
+  bool out_lane;
+};
+
+template <int SCHEME>
+__device__ __forceinline__ YPos y_pos(const SweepArgs& a, int k, int win, int lane) {
   constexpr int K = YGeom<SCHEME>::K;
   constexpr int OUTW = YGeom<SCHEME>::OUTW;
-  constexpr int WN = 2 * RAD + 1;
-  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
-  constexpr int NF = 5 + NS;
-  const int lane = ti.tx;
-  const int win = ti.bx * ti.by_dim + ti.ty;
-  const int k = a.h + ti.bz;
-  // uniform per wave
-  if (k >= a.h + a.nz || win >= y_windows(a.nx, a.ny, a.h, OUTW)) return;
   const int run = a.nx * a.NY;
   const int pos = win * OUTW + a.h - K + lane;
   const int pc = min(pos, run - 1);
@@ -509,23 +503,109 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
     ++row;
     yc -= a.NY;
   }
-  const int i = a.h + row;
-  const long long idx = (long long)k * a.sz + (long long)a.h * a.sx + pc;
-  const bool out_lane = lane >= K && lane < K + OUTW && pos < run && yc >= a.h &&
-                        yc < a.h + a.ny;
-  // The RHS accumulator and the RK operands are read first so their latency
-  // overlaps the arithmetic.
-  const bool rk = a.rk_form != 0;
-  float rin[NF], ukv[NF], u0v[NF];
+  YPos p;
+  p.yc = yc;
+  p.i = a.h + row;
+  p.idx = (long long)k * a.sz + (long long)a.h * a.sx + pc;
+  p.boff = (unsigned)(a.h * (int)a.sx + pc) * 4u;
+  p.out_lane = lane >= K && lane < K + OUTW && pos < run && yc >= a.h && yc < a.h + a.ny;
+  return p;
+}
+
+// Global-memory operands of one lane of a y window: loaded one window ahead
+// of their use, so a wave's loads for its next window overlap the arithmetic
+// of the current one.
+template <int NF, bool VISC, bool PD>
+struct YLoad {
+  float q[NF];        // stage state (reconstructed, and uk of the RK update)
+  float r[NF], dz[NF], u0[NF];
+  float tv[2][2][3];  // transverse dims x, z; +1, -1 neighbour; rho, m_t, rho_v
+  float pref;         // p_ref (reference-state builds)
+};
+
+// Element at byte offset boff of the plane starting at `plane`.
+__device__ __forceinline__ float ld_plane(const float* plane, unsigned boff) {
+  return *(const float*)((const char*)plane + boff);
+}
+__device__ __forceinline__ void st_plane(float* plane, unsigned boff, float v) {
+  *(float*)((char*)plane + boff) = v;
+}
+
+// Every lane loads (the positions of lanes that do not output are valid cells
+// of the plane's interior rows), so no load is predicated.
+template <int NF, bool VISC, bool PD>
+__device__ __forceinline__ void y_load(const SweepArgs& a, int k, const YPos& p,
+                                       YLoad<NF, VISC, PD>& L) {
+  const long long pk = (long long)k * a.sz;
+  const unsigned b = p.boff;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    rin[f] = out_lane ? a.r.p[f][idx] : 0.0f;
-    ukv[f] = (out_lane && rk) ? a.q.p[f][idx] : 0.0f;
-    u0v[f] = (out_lane && rk && a.rk_form != 1) ? a.u0.p[f][idx] : 0.0f;
+    L.q[f] = ld_plane(a.q.p[f] + pk, b);
+    L.r[f] = ld_plane(a.r.p[f] + pk, b);
+    L.dz[f] = ld_plane(a.dz.p[f] + pk, b);
   }
-  const float dt = rk ? *a.dt : 0.0f;
+  if (a.rk_form == 2 || a.rk_form == 3) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) L.u0[f] = ld_plane(a.u0.p[f] + pk, b);
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) L.u0[f] = 0.0f;
+  }
+  if constexpr (VISC) {
+    // Transverse dims x (same plane, rows i +- 1) and z (planes k +- 1):
+    // rho, the transverse momentum (rho_u, rho_w) and rho_v.
+    const unsigned bx = (unsigned)a.sx * 4u;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const unsigned bn = s == 0 ? b + bx : b - bx;
+      L.tv[0][s][0] = ld_plane(a.q.p[0] + pk, bn);
+      L.tv[0][s][1] = ld_plane(a.q.p[1] + pk, bn);
+      L.tv[0][s][2] = ld_plane(a.q.p[2] + pk, bn);
+      const long long pn = s == 0 ? pk + a.sz : pk - a.sz;
+      L.tv[1][s][0] = ld_plane(a.q.p[0] + pn, b);
+      L.tv[1][s][1] = ld_plane(a.q.p[3] + pn, b);
+      L.tv[1][s][2] = ld_plane(a.q.p[2] + pn, b);
+    }
+  }
+  if constexpr (PD) L.pref = ld_plane(a.p_ref + pk, b);
+}
+
+// Velocity components (u_c0, u_c1) from a loaded (rho, m_c0, m_c1); cells of
+// fields with a primitive BC get the reference's rho * (q / rho) rounding.
+template <bool LEAN>
+__device__ __forceinline__ void vel2_of(const SweepArgs& a, float rho, float m0, float m1,
+                                        bool interior, int c0, int c1, float& v0, float& v1) {
+  if (!LEAN && a.prim_bc_mask && interior) {
+    if (a.prim_bc_mask & (1 << (1 + c0))) m0 = rho * dnn_exact(m0, rho);
+    if (a.prim_bc_mask & (1 << (1 + c1))) m1 = rho * dnn_exact(m1, rho);
+  }
+  const float inv = frcp(rho);
+  v0 = inv * m0;
+  v1 = inv * m1;
+}
+
+// One window of the y sweep from its loaded operands: (R - Dz) - div_y into R
+// (rk_form 0) or the RK stage update into out.
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD, bool BB, bool LEAN>
+__device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& p,
+                                         const YLoad<5 + NS, VISC, PD>& L, float dt) {
+  constexpr int RAD = YGeom<SCHEME>::RAD;
+  constexpr int WN = 2 * RAD + 1;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
+  constexpr int NF = 5 + NS;
+  const int yc = p.yc, i = p.i;
+  const bool rk = a.rk_form != 0;
+  const bool interior = in_core(yc, a.h, a.ny);
   float W0[NV];
-  load_prim<NS, PD, LEAN>(a, idx, in_core(yc, a.h, a.ny), pe_at<LEAN>(a, k, i, yc), W0);
+  {
+    float q[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) q[f] = L.q[f];
+    prim_bc_round<NF, LEAN>(a, interior, q);
+    cons_to_prim<NS>(q, pe_at<LEAN>(a, k, i, yc), W0);
+    // Dynamic pressure: zero in every halo cell (Dirichlet-0 helper halos).
+    if constexpr (PD) W0[7 + NS] = interior ? W0[kP] - L.pref : 0.0f;
+  }
   float negP[NV], posM[NV], Am[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -545,7 +625,29 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, tPrev[2][2];
   ViscCell vCur{0.0f, 0.0f}, vPrev{0.0f, 0.0f};
   if constexpr (VISC) {
-    transverse_grads<1, LEAN>(a, k, i, yc, tCur);
+    // Central transverse derivatives at (k, i, yc) (physics/fluid.py:52-63
+    // with numerics/gradient.py:71-94): [t][0] = d(u_t)/dx_t, [t][1] = dv/dx_t
+    // for t = x, z.
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int dd = t == 0 ? 0 : 2;
+      bool inp, inm;
+      if (dd == 0) {
+        const bool rest = interior && in_core(k, a.h, a.nz);
+        inp = in_core(i + 1, a.h, a.nx) && rest;
+        inm = in_core(i - 1, a.h, a.nx) && rest;
+      } else {
+        const bool rest = in_core(i, a.h, a.nx) && interior;
+        inp = rest && in_core(k + 1, a.h, a.nz);
+        inm = rest && in_core(k - 1, a.h, a.nz);
+      }
+      float ap0, ap1, am0, am1;
+      vel2_of<LEAN>(a, L.tv[t][0][0], L.tv[t][0][1], L.tv[t][0][2], inp, dd, 1, ap0, ap1);
+      vel2_of<LEAN>(a, L.tv[t][1][0], L.tv[t][1][1], L.tv[t][1][2], inm, dd, 1, am0, am1);
+      const float ith = a.vc.inv_two_h[dd];
+      tCur[t][0] = (ap0 - am0) * ith;
+      tCur[t][1] = (ap1 - am1) * ith;
+    }
     vCur = visc_cell(W0, a.vc);
     vPrev = ViscCell{wave_shr1(vCur.T), wave_shr1(vCur.k)};
   }
@@ -562,16 +664,53 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   float Fn[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) Fn[f] = wave_shl1(F[f]);
-  if (out_lane) {
+  if (p.out_lane) {
+    const long long pk = (long long)k * a.sz;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      const float val = rin[f] - (Fn[f] - F[f]) * a.inv_hdir;
+      const float rin = L.r[f] - L.dz[f];
+      const float val = rin - (Fn[f] - F[f]) * a.inv_hdir;
       if (rk) {
-        a.out.p[f][idx] = rk_update(a, ukv[f], u0v[f], val, dt);
+        st_plane(a.out.p[f] + pk, p.boff, rk_update(a, L.q[f], L.u0[f], val, dt));
       } else {
-        a.r.p[f][idx] = val;
+        st_plane(a.r.p[f] + pk, p.boff, val);
       }
     }
+  }
+}
+
+// A wave runs windows w0 .. w0 + nw - 1 of z plane k (tile (bx, ty) of the
+// launch), with the operands of each next window loaded while the current
+// one is computed (two operand sets used in turn).
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
+          bool LEAN = false>
+__device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti, int nw) {
+  constexpr int OUTW = YGeom<SCHEME>::OUTW;
+  constexpr int NF = 5 + NS;
+  const int lane = ti.tx;
+  const int w0 = (ti.bx * ti.by_dim + ti.ty) * nw;
+  const int k = a.h + ti.bz;
+  const int nwin = y_windows(a.nx, a.ny, a.h, OUTW);
+  // uniform per wave
+  if (k >= a.h + a.nz || w0 >= nwin) return;
+  const int wend = min(w0 + nw, nwin);
+  const float dt = a.rk_form != 0 ? *a.dt : 0.0f;
+  YLoad<NF, VISC, PD> A, B;
+  YPos pa = y_pos<SCHEME>(a, k, w0, lane), pb;
+  y_load(a, k, pa, A);
+  for (int w = w0; w < wend; w += 2) {
+    const bool b_ok = w + 1 < wend;  // uniform per wave
+    if (b_ok) {
+      pb = y_pos<SCHEME>(a, k, w + 1, lane);
+      y_load(a, k, pb, B);
+    }
+    y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, pa, A, dt);
+    if (!b_ok) break;
+    if (w + 2 < wend) {
+      pa = y_pos<SCHEME>(a, k, w + 2, lane);
+      y_load(a, k, pa, A);
+    }
+    y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, pb, B, dt);
   }
 }
 
@@ -580,7 +719,7 @@ template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = fa
 __global__ void __launch_bounds__(256)
 sweep_y_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
-  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, ti);
+  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, ti, a.y_nw);
 }
 
 // ---------------------------------------------------------------------------

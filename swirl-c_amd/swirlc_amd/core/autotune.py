@@ -2,7 +2,8 @@
 
 The reference leaves scheduling to XLA; here each sweep kernel's launch
 geometry (cells per thread and waves per block of the marching x / z sweeps,
-waves per block of the y sweep, and the XCD-aware block order) is chosen per
+waves per block and windows per wave of the y sweep, and the XCD-aware block
+order) is chosen per
 grid by timing the candidates on the actual state with HIP events, one
 direction at a time (a direction's time does not depend on the other
 directions' geometry), then the block order over all three.  Forward results
@@ -22,7 +23,7 @@ import torch
 from swirlc_amd import _native as N
 
 MARCH_SPACE = tuple(itertools.product((16, 32, 64), (2, 4)))  # seg, by
-Y_SPACE = (1, 2, 4)  # waves per block
+Y_SPACE = tuple(itertools.product((1, 2, 4), (1, 2, 4, 8)))  # by_y, y_nw
 ADJ_MARCH_SPACE = tuple(itertools.product((8, 16, 32), (1, 2, 4)))  # adj_seg, adj_by
 ADJ_Y_SPACE = (1, 2, 4)  # adj_by_y
 
@@ -54,13 +55,14 @@ def _candidates(base, march_space, y_space):
       t = dict(base)
       t.update(zip(keys, vals))
       yield d, t
-  for by in y_space:
+  for by, nw in y_space:
     t = dict(base)
     t['by_y'] = by
+    t['y_nw'] = nw
     yield 'y', t
 
 
-_KEYS = {'x': ('seg_x', 'by_x'), 'z': ('seg_z', 'by_z'), 'y': ('by_y',)}
+_KEYS = {'x': ('seg_x', 'by_x'), 'z': ('seg_z', 'by_z'), 'y': ('by_y', 'y_nw')}
 
 
 def _pick(timings, base):
@@ -123,11 +125,14 @@ def tune_context(nctx, q, out, dt_dev, u0=None, reps=5, march_space=MARCH_SPACE,
   return best, timings
 
 
-def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=Y_SPACE):
+def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=None):
   """Times one RK3 stage-2 update of the whole ensemble per candidate (the
   other directions held at the base geometry) and keeps the fastest per
-  direction.  Returns (tuning, [(direction, tuning, ms)])."""
+  direction (the batched y sweep runs one window per wave).  Returns
+  (tuning, [(direction, tuning, ms)])."""
   base = batch.get_tuning()
+  if y_space is None:
+    y_space = tuple((by, base['y_nw']) for by in (1, 2, 4))
   timings = []
   for d, t in _candidates(base, march_space, y_space):
     batch.set_tuning(t)
