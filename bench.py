@@ -6,11 +6,14 @@ synthetic initial state (shear layer rho = 2 / 1, u = +-0.5, p = 2.5,
 v = 0.01 sin(4 pi x) + U(-1e-3, 1e-3) noise from seed 0).
 
 Prints one JSON line: Mcell-updates/s per RK step (interior cells x steps /
-seconds / 1e6), with the roofline of the dominant kernel (timed with HIP
-events on its own stream) and the CPU oracle's throughput on a bounded
-sample of the same workload.
+seconds / 1e6) over graph-replayed steps, with the roofline of the dominant
+kernel from its in-step duration (the step's launch sequence issued with HIP
+events between the launches, on the stream they run on) and the CPU oracle's
+throughput on one RK3 step of the same workload on the host's cores.
 
   python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu]
+  python bench.py --profile-steps K   (only the graph-replayed steps, for a
+                                       rocprofv3 kernel trace of them)
 """
 
 import argparse
@@ -26,11 +29,14 @@ for _p in (ROOT, os.path.join(ROOT, 'swirl-c_amd')):
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # Algorithmic bytes per interior cell per launch of each sweep kernel (fp32,
-# 5 conserved fields), in launch order: x reads U, writes R; z reads U and R,
-# writes R; y (last) in an RK3 stage 2/3 reads U, R and U0 and writes the new
-# state.
-SWEEP_BYTES = {'x': 40, 'y': 80, 'z': 60}
-# Per RK3 step (viscous, per-direction-sweep accounting): 700 B per cell.
+# 5 conserved fields), in launch order: x reads U, writes R (40); z reads U,
+# writes its divergence Dz (40); y (last) reads U, R and Dz, in RK3 stages 2
+# and 3 also U0, and writes the new state (80 at stage 1, 100 at stages 2-3).
+# A step: 160 + 180 + 180 = 520 B per cell.
+SWEEP_BYTES = {'x': (40, 40, 40), 'z': (40, 40, 40), 'y': (80, 100, 100)}
+STEP_BYTES = 520
+# The same step with a separate viscous-flux kernel (read U, read and write
+# R: 60 B per stage) would move 700 B per cell.
 STEP_BYTES_VISCOUS = 700
 # Measured HBM bytes per sweep launch at 256^3 (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --pmc-probe`, scaled by a copy kernel's
@@ -75,14 +81,20 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 
 def parse():
   ap = argparse.ArgumentParser()
+  ap.add_argument('--profile-steps', type=int, default=0,
+                  help='run only the tuned, graph-replayed KH steps (for rocprofv3)')
+  ap.add_argument('--cpu-workers', type=int, default=0,
+                  help='processes of the CPU baseline (default: the affinity cores, at most 16)')
   ap.add_argument('--gpus', type=int, default=1)
   ap.add_argument('--steps', type=int, default=20)
   ap.add_argument('--warmup', type=int, default=3)
   ap.add_argument('--n', type=int, default=256)
   ap.add_argument('--kernel-reps', type=int, default=10)
+  ap.add_argument('--kernel-steps', type=int, default=10,
+                  help='steps of the in-step kernel timing (HIP events between launches)')
   ap.add_argument('--no-cpu', action='store_true')
-  ap.add_argument('--cpu-n', type=int, default=128)
-  ap.add_argument('--cpu-steps', type=int, default=3)
+  ap.add_argument('--cpu-n', type=int, default=256)
+  ap.add_argument('--cpu-single-n', type=int, default=128)
   ap.add_argument('--batch-instances', type=int, default=256)
   ap.add_argument('--batch-steps', type=int, default=10)
   ap.add_argument('--batch-loop-steps', type=int, default=3)
@@ -96,14 +108,71 @@ def parse():
   ap.add_argument('--large-steps', type=int, default=10)
   ap.add_argument('--no-tune', action='store_true',
                   help='keep the default launch geometry (no autotuning)')
+  ap.add_argument('--tuning', type=str, default='',
+                  help='launch geometry as JSON (sc_tuning fields); skips autotuning')
   ap.add_argument('--pmc-probe', action='store_true',
                   help='launch a 256 MiB copy kernel, then each sweep kernel twice on the '
                        'state, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes')
   return ap.parse_args()
 
 
+def time_step_kernels(solver, steps):
+  """Kernel durations inside RK3 steps: a step's launch sequence (CFL wave
+  speed, then per stage the halo fill and the x, z, y sweeps, the last one
+  with the stage update) issued through the C ABI on one stream.  Each sweep
+  launch records its own start and stop times into HIP events
+  (sc_sweep_only_timed), so nothing is inserted between the kernels; the
+  wave speed and halo fill are bracketed by events recorded on the stream.
+  Returns {kernel: [ms per launch at stage 1, 2, 3]} (the wave speed as a
+  one-element list) averaged over `steps`, and the mean step time (ms)."""
+  import torch
+  from swirlc_amd import _native as N
+  ctx = solver.ctx
+  stream = torch.cuda.current_stream()
+  u, s1, s2 = solver.u, solver.s1, solver.s2
+  stages = ((N.SC_RK_EULER, 0.0, 0.0, None, u, s1), (N.SC_RK_BLEND, 0.75, 0.25, u, s1, s2),
+            (N.SC_RK_BLEND, 1.0 / 3.0, 2.0 / 3.0, u, s2, u))
+  names = ('halo', 'sweep_x', 'sweep_z', 'sweep_y')
+  acc = {n: [0.0, 0.0, 0.0] for n in names}
+  acc['max_speed'] = [0.0]
+  total = 0.0
+
+  def event():
+    e = torch.cuda.Event(enable_timing=True)
+    e.record(stream)  # creates the HIP event
+    return e
+
+  for _ in range(steps):
+    t_begin, t_speed = event(), event()
+    halo = [(event(), event()) for _ in range(3)]
+    sweeps = [[(event(), event()) for _ in range(3)] for _ in range(3)]
+    t_end = event()
+    torch.cuda.synchronize()
+    t_begin.record(stream)
+    ctx.max_wave_speed(u, solver.speed, float(solver.cfl), solver.dt, stream)
+    t_speed.record(stream)
+    for st, (form, c0, c1, base, uk, out) in enumerate(stages):
+      halo[st][0].record(stream)
+      ctx.fill_halos(uk, stream)
+      halo[st][1].record(stream)
+      for j, d in enumerate((0, 2, 1)):
+        e0, e1 = sweeps[st][j]
+        ctx.sweep_only_timed(d, form, c0, c1, solver.dt, base, uk, out, e0, e1, stream)
+    t_end.record(stream)
+    t_end.synchronize()
+    acc['max_speed'][0] += t_begin.elapsed_time(t_speed)
+    for st in range(3):
+      acc['halo'][st] += halo[st][0].elapsed_time(halo[st][1])
+      for j, n in enumerate(('sweep_x', 'sweep_z', 'sweep_y')):
+        e0, e1 = sweeps[st][j]
+        acc[n][st] += e0.elapsed_time(e1)
+    total += t_begin.elapsed_time(t_end)
+  return {n: [v / steps for v in vals] for n, vals in acc.items()}, total / steps
+
+
 def time_sweeps(solver, reps):
-  """Average duration (s) of each sweep kernel, HIP events on its stream."""
+  """Average duration (s) of each sweep kernel launched alone, repeatedly,
+  HIP events on its stream (a side note beside the in-step figures)."""
   import torch
   from swirlc_amd import _native as N
   ctx = solver.ctx
@@ -111,7 +180,10 @@ def time_sweeps(solver, reps):
   ctx.fill_halos(solver.u)
   out = {}
   # RK3 stage-2 form with U0 (the last step's stage-1 state) distinct from Uk,
-  # as in a real stage.
+  # as in a real stage; the y sweep reads the x and z sweeps' outputs.
+  for d in (0, 2):
+    ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u, solver.s2,
+                   stream)
   for d, name in enumerate(('x', 'y', 'z')):
     ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u,
                    solver.s2, stream)  # warm
@@ -296,6 +368,8 @@ def bench_large(steps):
   sim = Solver(SwirlCParameters(cfg_d), {k: torch.from_numpy(v).cuda() for k, v in st.items()},
                cfl=0.5, use_graph=True)
   del st
+  from swirlc_amd.core import autotune
+  tuning, _ = autotune.tune_context(sim.ctx, sim.u, sim.s2, sim.dt, u0=sim.s1, reps=3)
   sim.step(2)
   torch.cuda.synchronize()
   ev0 = torch.cuda.Event(enable_timing=True)
@@ -311,12 +385,12 @@ def bench_large(steps):
   torch.cuda.empty_cache()
   return {'workload': 'buoyant_bubble_512x512x256_muscl_hll_viscous_gravity_ssprk3_cfl0.5',
           'steps': steps, 'ms_per_step': round(ms, 3),
-          'Mcell_updates_per_s': round(cells / ms / 1e3, 1), 'finite': finite}
+          'Mcell_updates_per_s': round(cells / ms / 1e3, 1), 'finite': finite,
+          'tuning': tuning}
 
 
-def cpu_baseline(n, steps):
-  """The fp32 numpy oracle on an n^3 sample of the same workload (1 thread)."""
-  import numpy as np
+def _kh_oracle_cfg(n):
+  """The KH workload at n^3 in the oracle's config form, and its state."""
   from oracle import swirlc_oracle as O
   from swirlc_amd import configs
   cfg_d, st = configs.kh(n=n)
@@ -324,24 +398,118 @@ def cpu_baseline(n, steps):
       'halo_width', 'core_nx', 'core_ny', 'core_nz', 'x', 'y', 'z', 'dx', 'dy', 'dz',
       'interpolation_scheme', 'numeric_flux_scheme', 'include_diffusion',
       'transport_parameters', 'bc')})
-  dt = np.float32(1e-4)
-  t0 = time.perf_counter()
-  for _ in range(steps):
-    st = O.rk3(st, {}, ocfg, dt=dt)
-  el = time.perf_counter() - t0
-  model = ''
+  return ocfg, st
+
+
+# State of the slab workers of the parallel CPU oracle (set before the pool
+# forks, read by _slab_rhs in the workers).
+_SLAB = {}
+
+
+def _slab_rhs(bounds):
+  """Oracle RHS of interior z planes [k0, k1) from the halo-filled state in
+  shared memory: the slab with its 2h halo planes is an oracle grid whose z
+  dimension has no BC (its halo planes are the filled neighbours)."""
+  import numpy as np
+  from multiprocessing import shared_memory
+  from oracle import swirlc_oracle as O
+  k0, k1 = bounds
+  g = _SLAB
+  h = g['h']
+  shm_in = shared_memory.SharedMemory(name=g['in'])
+  shm_out = shared_memory.SharedMemory(name=g['out'])
+  q = np.ndarray(g['shape'], np.float32, buffer=shm_in.buf)
+  r = np.ndarray(g['shape'], np.float32, buffer=shm_out.buf)
+  sub = {name: q[f, k0:k1 + 2 * h].copy() for f, name in enumerate(g['names'])}
+  cfg = dict(g['cfg'], core_nz=k1 - k0, nz=k1 - k0, z=g['cfg']['z'][k0:k1])
+  cfg['bc'] = {'cell_averages': {v: {d: b for d, b in spec.items() if d != 'z'}
+                                 for v, spec in g['cfg']['bc']['cell_averages'].items()}}
+  out = O.rhs(sub, {}, cfg)
+  for f, name in enumerate(g['names']):
+    r[f, h + k0:h + k1] = out[name][h:-h]
+  del q, r
+  shm_in.close()
+  shm_out.close()
+  return k1 - k0
+
+
+def parallel_rk3_step(pool, state, ocfg, dt, slabs, shm_in, shm_out, shape):
+  """One oracle RK3 step (numerics/time_integration/rk3.py:34-122) with each
+  RHS evaluated slab-parallel by `pool`: the parent fills the halos
+  (boundary.py:464-539) and writes the state to shared memory, the workers
+  compute the RHS of their z slabs.  Equal, bit for bit, to the serial
+  oracle's step (the RHS of a plane depends on its +-h neighbour planes only)."""
+  import numpy as np
+  from oracle import swirlc_oracle as O
+  names = list(state)
+  q = np.ndarray(shape, np.float32, buffer=shm_in.buf)
+  r = np.ndarray(shape, np.float32, buffer=shm_out.buf)
+
+  def rhs(st):
+    filled = O.update_conservative_cell_averages(st, ocfg)
+    for f, name in enumerate(names):
+      q[f] = filled[name]
+    pool.map(_slab_rhs, slabs)
+    return {name: r[f].copy() for f, name in enumerate(names)}
+
+  return O.rk3(state, {}, ocfg, dt=dt, rhs_fn=rhs)
+
+
+def _cpu_model():
   try:
     for line in open('/proc/cpuinfo'):
       if line.startswith('model name'):
-        model = line.split(':', 1)[1].strip()
-        break
+        return line.split(':', 1)[1].strip()
   except OSError:
     pass
-  return {'value': n ** 3 * steps / el / 1e6, 'unit': 'Mcell-updates/s', 'cores': 1,
+  return ''
+
+
+def cpu_baseline(n, single_n, workers):
+  """The fp32 numpy oracle on the host: one RK3 step of the n^3 workload with
+  the RHS split over `workers` processes (z slabs), and one serial RK3 step
+  at single_n^3 on one core."""
+  import multiprocessing as mp
+  from multiprocessing import shared_memory
+  import numpy as np
+  from oracle import swirlc_oracle as O
+  dt = np.float32(1e-4)
+  # Single thread.
+  ocfg, st = _kh_oracle_cfg(single_n)
+  t0 = time.perf_counter()
+  O.rk3(st, {}, ocfg, dt=dt)
+  el1 = time.perf_counter() - t0
+  # All workers.
+  ocfg, st = _kh_oracle_cfg(n)
+  h = ocfg['halo_width']
+  names = list(st)
+  shape = (len(names),) + st[names[0]].shape
+  nbytes = int(np.prod(shape)) * 4
+  shm_in = shared_memory.SharedMemory(create=True, size=nbytes)
+  shm_out = shared_memory.SharedMemory(create=True, size=nbytes)
+  try:
+    _SLAB.update({'in': shm_in.name, 'out': shm_out.name, 'shape': shape, 'h': h,
+                  'names': names, 'cfg': ocfg})
+    nz = ocfg['core_nz']
+    edges = np.linspace(0, nz, workers + 1).round().astype(int)
+    slabs = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+    with mp.get_context('fork').Pool(workers) as pool:
+      t0 = time.perf_counter()
+      parallel_rk3_step(pool, st, ocfg, dt, slabs, shm_in, shm_out, shape)
+      el = time.perf_counter() - t0
+  finally:
+    shm_in.close()
+    shm_in.unlink()
+    shm_out.close()
+    shm_out.unlink()
+  return {'value': round(n ** 3 / el / 1e6, 3), 'unit': 'Mcell-updates/s', 'cores': workers,
           'kind': 'port',
-          'sample': f'{steps} RK3 steps of the same workload at {n}^3 (fp32 numpy oracle, '
-                    f'single thread), {el:.1f} s',
-          'cpu_model': model}
+          'sample': f'one RK3 step of the same workload at {n}^3 (fp32 numpy oracle, RHS '
+                    f'split over {workers} processes by z slab), {el:.1f} s',
+          'single_thread_value': round(single_n ** 3 / el1 / 1e6, 3),
+          'single_thread_sample': f'one RK3 step at {single_n}^3 on one core, {el1:.1f} s',
+          'cpu_model': _cpu_model(),
+          'affinity_cores': len(os.sched_getaffinity(0))}
 
 
 def main():
@@ -366,12 +534,25 @@ def main():
   solver = Solver(cfg, state, cfl=0.5, use_graph=True)
   del state
   from swirlc_amd.core import autotune
-  if args.no_tune:
+  if args.tuning:
+    solver.ctx.set_tuning(json.loads(args.tuning))
+    tuning = solver.ctx.get_tuning()
+  elif args.no_tune:
     tuning = solver.ctx.get_tuning()
   else:
     tuning, _ = autotune.tune_context(solver.ctx, solver.u, solver.s2, solver.dt, u0=solver.s1)
   print(f'[bench] KH {n}^3 state on GPU; tuning {tuning}; warmup {args.warmup} steps',
         file=sys.stderr, flush=True)
+  if args.profile_steps:
+    solver.step(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.step(args.profile_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(f'[bench] profile: {args.profile_steps} graph-replayed steps, '
+          f'{el / args.profile_steps * 1e3:.3f} ms/step', flush=True)
+    return
   solver.step(args.warmup)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -385,19 +566,34 @@ def main():
   print(f'[bench] {args.steps} steps in {el:.3f} s ({ms:.3f} ms/step), finite={finite}',
         file=sys.stderr, flush=True)
 
+  # In-step kernel durations (the step's launch sequence with HIP events
+  # between launches) and the dominant kernel's roofline from them.
+  step_k, step_ev_ms = time_step_kernels(solver, args.kernel_steps)
+  per_step = {k: sum(v) for k, v in step_k.items()}
+  dom = max((k for k in per_step if k.startswith('sweep_')), key=per_step.get)
+  d = dom.split('_')[1]
+  bytes_launch = sum(SWEEP_BYTES[d]) / 3.0 * cells  # mean over the three stages
+  t_launch = per_step[dom] / 3.0 / 1e3  # s
+  achieved = bytes_launch / t_launch / 1e9
+  traffic = measured_traffic(dom, tuning['xcd_remap']) if n == 256 else None
   ktimes = time_sweeps(solver, args.kernel_reps)
-  dom = max(ktimes, key=ktimes.get)
-  bytes_launch = SWEEP_BYTES[dom] * cells
-  achieved = bytes_launch / ktimes[dom] / 1e9
-  traffic = measured_traffic(f'sweep_{dom}', tuning['xcd_remap']) if n == 256 else None
   copy_bw = copy_bandwidth()
   roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK / 1e9,
               'unit': 'GB/s', 'frac': round(achieved * 1e9 / HBM_PEAK, 4),
-              'traffic': traffic, 'kernel': f'sweep_{dom}',
-              'algorithmic_bytes': bytes_launch,
-              'kernel_ms': {k: round(v * 1e3, 4) for k, v in ktimes.items()},
+              'traffic': traffic, 'kernel': dom,
+              'algorithmic_bytes': int(round(bytes_launch)),
+              'kernel_ms': round(t_launch * 1e3, 4),
+              'in_step_ms': {k: [round(x, 4) for x in v] for k, v in step_k.items()},
+              'in_step_sum_ms': round(sum(per_step.values()), 4),
+              'in_step_event_step_ms': round(step_ev_ms, 4),
+              'isolated_kernel_ms': {k: round(v * 1e3, 4) for k, v in ktimes.items()},
               'copy_GBps_measured': round(copy_bw / 1e9, 1)}
-  step_frac = cells * STEP_BYTES_VISCOUS / (el / args.steps) / HBM_PEAK
+  step_s = el / args.steps
+  step_frac = {'520B': round(cells * STEP_BYTES / step_s / HBM_PEAK, 4),
+               '700B': round(cells * STEP_BYTES_VISCOUS / step_s / HBM_PEAK, 4)}
+  print(f'[bench] in-step kernels (ms per launch, stages 1-3, each launch timing itself): '
+        f'{roofline["in_step_ms"]}; sum {roofline["in_step_sum_ms"]} ms vs {ms:.3f} ms per '
+        'graph-replayed step', file=sys.stderr, flush=True)
 
   batched = None
   if not args.no_batch:
@@ -426,9 +622,11 @@ def main():
 
   cpu = None
   if not args.no_cpu:
-    print(f'[bench] CPU oracle sample {args.cpu_n}^3 x {args.cpu_steps} steps',
-          file=sys.stderr, flush=True)
-    cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
+    workers = args.cpu_workers or min(16, len(os.sched_getaffinity(0)))
+    print(f'[bench] CPU oracle: one RK3 step at {args.cpu_n}^3 on {workers} processes, one at '
+          f'{args.cpu_single_n}^3 on one core', file=sys.stderr, flush=True)
+    cpu = cpu_baseline(args.cpu_n, args.cpu_single_n, workers)
+    print(f'[bench] cpu: {cpu}', file=sys.stderr, flush=True)
 
   line = {
       'metric': 'Mcell-updates/s/RK-step', 'value': round(value, 2),
@@ -438,7 +636,7 @@ def main():
       'config': {'workload': f'kelvin_helmholtz_{n}^3_weno5_hllc_viscous_ssprk3_cfl0.5',
                  'cells': cells, 'parallelism': 'single-gpu', 'tuning': tuning},
       'roofline': roofline,
-      'step_hbm_frac_700B': round(step_frac, 4),
+      'step_hbm_frac': step_frac,
       'cpu_baseline': cpu,
       'batched': batched,
       'large': large,

@@ -279,6 +279,15 @@ sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0,
                         float* const* uk, float* const* out, void* stream);
 
+/* Profiling: sc_sweep_only whose kernel launch records its own start and stop
+ * times into the HIP events ev_start / ev_stop (hipExtLaunchKernel: no extra
+ * packets between kernels), so a kernel's duration inside a step sequence
+ * can be read with hipEventElapsedTime.  No reference counterpart. */
+sc_status sc_sweep_only_timed(sc_ctx* ctx, int dim, int form, float c0, float c1,
+                              const float* dt_dev, const float* const* u0,
+                              float* const* uk, float* const* out, void* ev_start,
+                              void* ev_stop, void* stream);
+
 /* Launch geometry of the sweep kernels (no reference counterpart: the
  * reference leaves scheduling to XLA).  seg_*: cells per thread of a marching
  * sweep (>= 4); by_*: 64-lane waves per block (1..4).  Chosen per grid by

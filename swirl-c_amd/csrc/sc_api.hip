@@ -423,18 +423,19 @@ sc_status sc_fill_halos(sc_ctx* c, float* const* q, void* stream) {
 }
 
 static sc_status run_sweeps(sc_ctx* c, const sc::SweepArgs& a, hipStream_t s,
-                            int only_dir = -1) {
+                            int only_dir = -1, const sc::SweepTuning* tune = nullptr) {
+  const sc::SweepTuning& t = tune != nullptr ? *tune : c->tune;
   hipError_t e;
   const int scheme = c->params.interpolation;
   const int flux = c->params.numeric_flux;
   const bool visc = c->params.include_diffusion != 0;
   const int ns = c->nf - 5;
   if (scheme == SC_WENO_5) {
-    e = sc::launch_rhs_sweeps<sc::kWeno5>(a, c->h, flux, visc, ns, c->tune, s, only_dir);
+    e = sc::launch_rhs_sweeps<sc::kWeno5>(a, c->h, flux, visc, ns, t, s, only_dir);
   } else if (scheme == SC_WENO_3) {
-    e = sc::launch_rhs_sweeps<sc::kWeno3>(a, c->h, flux, visc, ns, c->tune, s, only_dir);
+    e = sc::launch_rhs_sweeps<sc::kWeno3>(a, c->h, flux, visc, ns, t, s, only_dir);
   } else {
-    e = sc::launch_rhs_sweeps<sc::kMuscl>(a, c->h, flux, visc, ns, c->tune, s, only_dir);
+    e = sc::launch_rhs_sweeps<sc::kMuscl>(a, c->h, flux, visc, ns, t, s, only_dir);
   }
   return hip_check(e, "sweep kernels");
 }
@@ -535,9 +536,30 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
   return run_sweeps(c, a, (hipStream_t)stream);
 }
 
+static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c1,
+                                 const float* dt_dev, const float* const* u0,
+                                 float* const* uk, float* const* out, void* stream,
+                                 void* ev_start, void* ev_stop);
+
 sc_status sc_sweep_only(sc_ctx* c, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0, float* const* uk,
                         float* const* out, void* stream) {
+  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, stream, nullptr, nullptr);
+}
+
+sc_status sc_sweep_only_timed(sc_ctx* c, int dim, int form, float c0, float c1,
+                              const float* dt_dev, const float* const* u0, float* const* uk,
+                              float* const* out, void* ev_start, void* ev_stop,
+                              void* stream) {
+  if (ev_start == nullptr || ev_stop == nullptr)
+    return fail(SC_ERR_INVALID, "sc_sweep_only_timed: null event");
+  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, stream, ev_start, ev_stop);
+}
+
+static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c1,
+                                 const float* dt_dev, const float* const* u0,
+                                 float* const* uk, float* const* out, void* stream,
+                                 void* ev_start, void* ev_stop) {
   if (c == nullptr || uk == nullptr) return fail(SC_ERR_INVALID, "sc_sweep_only: null argument");
   if (dim < 0 || dim > 2) return fail(SC_ERR_INVALID, "sc_sweep_only: dim must be 0..2");
   if (form != 0 && (out == nullptr || dt_dev == nullptr || (form != SC_RK_EULER && u0 == nullptr)))
@@ -554,7 +576,10 @@ sc_status sc_sweep_only(sc_ctx* c, int dim, int form, float c0, float c1,
   a.c0 = c0;
   a.c1 = c1;
   a.dt = dt_dev;
-  return run_sweeps(c, a, (hipStream_t)stream, dim);
+  sc::SweepTuning t = c->tune;
+  t.ev_start = (hipEvent_t)ev_start;
+  t.ev_stop = (hipEvent_t)ev_stop;
+  return run_sweeps(c, a, (hipStream_t)stream, dim, &t);
 }
 
 sc_status sc_max_wave_speed(sc_ctx* c, const float* const* q, float cfl, float* out_dev,

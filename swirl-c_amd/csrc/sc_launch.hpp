@@ -1,11 +1,24 @@
 // Host-side launchers shared between the translation units of libswirlc_amd.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "sc_sweep_impl.hpp"
 
 namespace sc {
+
+// Launches a sweep kernel; with the tuning's profiling events set, the
+// launch records its own start and stop times into them.
+template <typename K>
+inline void launch_sweep(K kernel, dim3 grid, dim3 block, hipStream_t stream,
+                         const SweepTuning& tune, const SweepArgs& a) {
+  if (tune.ev_start != nullptr && tune.ev_stop != nullptr) {
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, tune.ev_start, tune.ev_stop, 0, a);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, stream, a);
+  }
+}
 
 // Launches the x, z and y sweeps of one RHS evaluation (plus the RK update
 // folded into the y sweep when a.rk_form != 0) for interpolation scheme SCHEME.
@@ -36,8 +49,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     ax.seg_len = tune.seg_x;
     const dim3 grid((a.ny + 63) / 64, (a.nz + tune.by_x - 1) / tune.by_x,
                     (a.nx + tune.seg_x - 1) / tune.seg_x);
-    hipLaunchKernelGGL((sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB, LEAN>), grid,
-                       dim3(64, tune.by_x), 0, stream, ax);
+    launch_sweep(sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB, LEAN>, grid,
+                 dim3(64, tune.by_x), stream, tune, ax);
   }
   // z sweep (middle): Dz = div_z.
   if (only_dir < 0 || only_dir == 2) {
@@ -46,8 +59,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     az.seg_len = tune.seg_z;
     const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
                     (a.nz + tune.seg_z - 1) / tune.seg_z);
-    hipLaunchKernelGGL((sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, PD, BB, LEAN>), grid,
-                       dim3(64, tune.by_z), 0, stream, az);
+    launch_sweep(sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, PD, BB, LEAN>, grid,
+                 dim3(64, tune.by_z), stream, tune, az);
   }
   // y sweep (last): R = (R - Dz) - div_y, then store R or the RK update.
   if (only_dir < 0 || only_dir == 1) {
@@ -57,8 +70,8 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     const int nwin = y_windows(a.nx, a.ny, a.h, YGeom<SCHEME>::OUTW);
     const int per_block = tune.by_y * tune.y_nw;
     const dim3 grid((nwin + per_block - 1) / per_block, 1, a.nz);
-    hipLaunchKernelGGL((sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>), grid,
-                       dim3(64, tune.by_y), 0, stream, ay);
+    launch_sweep(sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>, grid,
+                 dim3(64, tune.by_y), stream, tune, ay);
   }
   return hipGetLastError();
 }

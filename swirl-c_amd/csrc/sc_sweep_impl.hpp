@@ -47,6 +47,10 @@ struct SweepTuning {
   int adj_by = 4;     // adjoint marching passes: waves per block
   int adj_by_y = 4;   // adjoint y pass: waves per block
   int y_nw = 4;       // y sweep: consecutive windows per wave
+  // Profiling only: when set, each sweep launch records its own start and
+  // stop time into these events (hipExtLaunchKernel), adding no packets.
+  hipEvent_t ev_start = nullptr;
+  hipEvent_t ev_stop = nullptr;
 };
 
 // Dirichlet values on the domain's boundary faces of one sweep direction

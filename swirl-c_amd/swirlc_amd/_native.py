@@ -33,7 +33,7 @@ EXPORTED = (
     'sc_ctx_set_reference_state', 'sc_face_interp', 'sc_linear_interp', 'sc_riemann_flux',
     'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
     'sc_face_bc_elementwise', 'sc_last_error',
-    'sc_version', 'sc_rk_stage_noalloc',
+    'sc_version', 'sc_rk_stage_noalloc', 'sc_sweep_only_timed',
 )
 
 
@@ -153,6 +153,7 @@ def lib():
     L.sc_face_bc_elementwise.argtypes = [i, vp, vp, vp, ctypes.c_longlong, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_sweep_only_timed.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp, vp, vp]
     L.sc_batch_create.argtypes = [ctypes.POINTER(ScParams), i, i, ctypes.POINTER(ScInstance),
                                   ctypes.POINTER(vp)]
     L.sc_batch_destroy.argtypes = [vp]
@@ -178,7 +179,7 @@ def lib():
                  'sc_batch_set_tuning', 'sc_ctx_set_reference_state', 'sc_face_interp',
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
                  'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise',
-                 'sc_rk_stage_noalloc'):
+                 'sc_rk_stage_noalloc', 'sc_sweep_only_timed'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

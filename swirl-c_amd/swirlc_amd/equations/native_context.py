@@ -381,6 +381,18 @@ class NativeContext:
         N.ptr_array(out) if out is not None else None,
         N.stream_handle(stream)), 'sc_sweep_only')
 
+  def sweep_only_timed(self, dim, form, c0, c1, dt_dev, u0, uk, out, ev_start, ev_stop,
+                       stream=None):
+    """sweep_only with the kernel's own start / stop times recorded into two
+    torch.cuda.Event objects (each recorded once before, so it exists)."""
+    N.check(N.lib().sc_sweep_only_timed(
+        self._ctx, int(dim), int(form), ctypes.c_float(c0), ctypes.c_float(c1),
+        ctypes.c_void_p(dt_dev.data_ptr()) if dt_dev is not None else None,
+        N.ptr_array(u0) if u0 is not None else None, N.ptr_array(uk),
+        N.ptr_array(out) if out is not None else None,
+        ctypes.c_void_p(ev_start.cuda_event), ctypes.c_void_p(ev_stop.cuda_event),
+        N.stream_handle(stream)), 'sc_sweep_only_timed')
+
   def max_wave_speed(self, q, out_dev, cfl=0.0, dt_dev=None, stream=None):
     N.check(N.lib().sc_max_wave_speed(
         self._ctx, N.ptr_array(q), ctypes.c_float(cfl),
