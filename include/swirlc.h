@@ -303,9 +303,10 @@ typedef struct sc_tuning {
    * pass.  Ignored by batches. */
   int adj_seg, adj_by;
   int adj_by_y;
-  /* y sweep: consecutive 64-lane windows per wave (1..64; the operands of
-   * each next window are loaded while the current one is computed). */
-  int y_nw;
+  /* y sweep: z planes per wave (1..2^20): a wave keeps its 64-lane window
+   * and marches over seg_y planes, loading each next plane's operands while
+   * the current one is computed. */
+  int seg_y;
 } sc_tuning;
 
 sc_status sc_ctx_get_tuning(const sc_ctx* ctx, sc_tuning* out);

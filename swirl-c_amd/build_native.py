@@ -26,6 +26,10 @@ CFLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC',
     # Packed FP32 instructions are kept out of all device code.
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    # Multiply-adds are fused within one expression only, so a function's
+    # rounding does not depend on the kernel it is inlined into (the batched
+    # and single-grid sweeps agree bit for bit).
+    '-ffp-contract=on',
     '-Wno-unused-command-line-argument',
 ]
 

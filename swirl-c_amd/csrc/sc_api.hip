@@ -105,8 +105,8 @@ sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
     if (v < 1 || v > 4) return fail(SC_ERR_INVALID, "tuning: by_x, by_z, by_y, adj_by, adj_by_y must be in [1, 4]");
   if (t->xcd_remap != 0 && t->xcd_remap != 1)
     return fail(SC_ERR_INVALID, "tuning: xcd_remap must be 0 or 1");
-  if (t->y_nw < 1 || t->y_nw > 64)
-    return fail(SC_ERR_INVALID, "tuning: y_nw must be in [1, 64]");
+  if (t->seg_y < 1 || t->seg_y > (1 << 20))
+    return fail(SC_ERR_INVALID, "tuning: seg_y must be in [1, 2^20]");
   out->seg_x = t->seg_x;
   out->by_x = t->by_x;
   out->seg_z = t->seg_z;
@@ -116,7 +116,7 @@ sc_status to_tuning(const sc_tuning* t, sc::SweepTuning* out) {
   out->adj_seg = t->adj_seg;
   out->adj_by = t->adj_by;
   out->adj_by_y = t->adj_by_y;
-  out->y_nw = t->y_nw;
+  out->seg_y = t->seg_y;
   return SC_OK;
 }
 
@@ -174,7 +174,7 @@ void from_tuning(const sc::SweepTuning& t, sc_tuning* out) {
   out->adj_seg = t.adj_seg;
   out->adj_by = t.adj_by;
   out->adj_by_y = t.adj_by_y;
-  out->y_nw = t.y_nw;
+  out->seg_y = t.seg_y;
 }
 
 }  // namespace

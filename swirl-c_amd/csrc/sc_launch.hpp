@@ -66,10 +66,10 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
   if (only_dir < 0 || only_dir == 1) {
     SweepArgs ay = a;
     ay.inv_hdir = 1.0f / h[1];
-    ay.y_nw = tune.y_nw;
+    ay.seg_y = tune.seg_y;
     const int nwin = y_windows(a.nx, a.ny, a.h, YGeom<SCHEME>::OUTW);
-    const int per_block = tune.by_y * tune.y_nw;
-    const dim3 grid((nwin + per_block - 1) / per_block, 1, a.nz);
+    const dim3 grid((nwin + tune.by_y - 1) / tune.by_y, 1,
+                    (a.nz + tune.seg_y - 1) / tune.seg_y);
     launch_sweep(sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>, grid,
                  dim3(64, tune.by_y), stream, tune, ay);
   }

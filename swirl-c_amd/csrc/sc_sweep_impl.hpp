@@ -46,7 +46,7 @@ struct SweepTuning {
   int adj_seg = 16;   // adjoint marching passes: owned cells per thread
   int adj_by = 4;     // adjoint marching passes: waves per block
   int adj_by_y = 4;   // adjoint y pass: waves per block
-  int y_nw = 4;       // y sweep: consecutive windows per wave
+  int seg_y = 16;     // y sweep: z planes per wave
   // Profiling only: when set, each sweep launch records its own start and
   // stop time into these events (hipExtLaunchKernel), adding no packets.
   hipEvent_t ev_start = nullptr;
@@ -101,7 +101,7 @@ struct SweepArgs {
   int bbc_dims;             // bit d: boundary face/flux values along dim d
   const BoundaryBc* bbc;    // device table [dim][side]
   int xcd_remap;            // XCD-aware block order (see tile_of_block)
-  int y_nw;                 // y sweep: consecutive windows per wave
+  int seg_y;                // y sweep: z planes per wave
   // Hydrostatic reference state (helper_var p_ref / rho_ref), or null:
   // p_dyn = p - p_ref (zero in halo cells) replaces p in the momentum flux,
   // d_rho = rho - rho_ref replaces rho in the gravity source
@@ -477,19 +477,18 @@ __host__ __device__ inline int y_windows(int nx, int ny, int h, int outw) {
   return (int)((run - 2 * h + outw - 1) / outw);
 }
 
-// Position of a lane in window `win` of z plane k (see y_windows): idx is
-// its element index in the field, boff its byte offset within plane k (below
-// 2^32: a plane holds fewer than 2^30 cells, checked at context creation), so
-// loads and stores take a wave-uniform plane base and a 32-bit lane offset.
+// Position of a lane in window `win` of a z plane (see y_windows): its row
+// i and column yc, and its byte offset within the plane (below 2^32: a plane
+// holds fewer than 2^30 cells, checked at context creation), so loads and
+// stores take a wave-uniform plane base and a 32-bit lane offset.
 struct YPos {
   int yc, i;
-  long long idx;
   unsigned boff;
   bool out_lane;
 };
 
 template <int SCHEME>
-__device__ __forceinline__ YPos y_pos(const SweepArgs& a, int k, int win, int lane) {
+__device__ __forceinline__ YPos y_pos(const SweepArgs& a, int win, int lane) {
   constexpr int K = YGeom<SCHEME>::K;
   constexpr int OUTW = YGeom<SCHEME>::OUTW;
   const int run = a.nx * a.NY;
@@ -510,20 +509,19 @@ __device__ __forceinline__ YPos y_pos(const SweepArgs& a, int k, int win, int la
   YPos p;
   p.yc = yc;
   p.i = a.h + row;
-  p.idx = (long long)k * a.sz + (long long)a.h * a.sx + pc;
   p.boff = (unsigned)(a.h * (int)a.sx + pc) * 4u;
   p.out_lane = lane >= K && lane < K + OUTW && pos < run && yc >= a.h && yc < a.h + a.ny;
   return p;
 }
 
-// Global-memory operands of one lane of a y window: loaded one window ahead
-// of their use, so a wave's loads for its next window overlap the arithmetic
+// Global-memory operands of one lane at one z plane: loaded one plane ahead
+// of their use, so a wave's loads for its next plane overlap the arithmetic
 // of the current one.
 template <int NF, bool VISC, bool PD>
 struct YLoad {
   float q[NF];        // stage state (reconstructed, and uk of the RK update)
   float r[NF], dz[NF], u0[NF];
-  float tv[2][2][3];  // transverse dims x, z; +1, -1 neighbour; rho, m_t, rho_v
+  float tx[2][3];     // rows i + 1, i - 1: rho, rho_u, rho_v
   float pref;         // p_ref (reference-state builds)
 };
 
@@ -536,7 +534,8 @@ __device__ __forceinline__ void st_plane(float* plane, unsigned boff, float v) {
 }
 
 // Every lane loads (the positions of lanes that do not output are valid cells
-// of the plane's interior rows), so no load is predicated.
+// of the plane's interior rows, and plane k may be the first halo plane), so
+// no load is predicated.
 template <int NF, bool VISC, bool PD>
 __device__ __forceinline__ void y_load(const SweepArgs& a, int k, const YPos& p,
                                        YLoad<NF, VISC, PD>& L) {
@@ -556,19 +555,13 @@ __device__ __forceinline__ void y_load(const SweepArgs& a, int k, const YPos& p,
     for (int f = 0; f < NF; ++f) L.u0[f] = 0.0f;
   }
   if constexpr (VISC) {
-    // Transverse dims x (same plane, rows i +- 1) and z (planes k +- 1):
-    // rho, the transverse momentum (rho_u, rho_w) and rho_v.
     const unsigned bx = (unsigned)a.sx * 4u;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const unsigned bn = s == 0 ? b + bx : b - bx;
-      L.tv[0][s][0] = ld_plane(a.q.p[0] + pk, bn);
-      L.tv[0][s][1] = ld_plane(a.q.p[1] + pk, bn);
-      L.tv[0][s][2] = ld_plane(a.q.p[2] + pk, bn);
-      const long long pn = s == 0 ? pk + a.sz : pk - a.sz;
-      L.tv[1][s][0] = ld_plane(a.q.p[0] + pn, b);
-      L.tv[1][s][1] = ld_plane(a.q.p[3] + pn, b);
-      L.tv[1][s][2] = ld_plane(a.q.p[2] + pn, b);
+      L.tx[s][0] = ld_plane(a.q.p[0] + pk, bn);
+      L.tx[s][1] = ld_plane(a.q.p[1] + pk, bn);
+      L.tx[s][2] = ld_plane(a.q.p[2] + pk, bn);
     }
   }
   if constexpr (PD) L.pref = ld_plane(a.p_ref + pk, b);
@@ -588,11 +581,13 @@ __device__ __forceinline__ void vel2_of(const SweepArgs& a, float rho, float m0,
   v1 = inv * m1;
 }
 
-// One window of the y sweep from its loaded operands: (R - Dz) - div_y into R
+// One window of the y sweep at plane k from its loaded operands (zp, zm:
+// rho, rho_w, rho_v of planes k + 1 and k - 1): (R - Dz) - div_y into R
 // (rk_form 0) or the RK stage update into out.
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD, bool BB, bool LEAN>
 __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& p,
-                                         const YLoad<5 + NS, VISC, PD>& L, float dt) {
+                                         const YLoad<5 + NS, VISC, PD>& L, const float* zp,
+                                         const float* zm, float dt) {
   constexpr int RAD = YGeom<SCHEME>::RAD;
   constexpr int WN = 2 * RAD + 1;
   constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
@@ -632,25 +627,25 @@ __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& 
     // Central transverse derivatives at (k, i, yc) (physics/fluid.py:52-63
     // with numerics/gradient.py:71-94): [t][0] = d(u_t)/dx_t, [t][1] = dv/dx_t
     // for t = x, z.
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int dd = t == 0 ? 0 : 2;
-      bool inp, inm;
-      if (dd == 0) {
-        const bool rest = interior && in_core(k, a.h, a.nz);
-        inp = in_core(i + 1, a.h, a.nx) && rest;
-        inm = in_core(i - 1, a.h, a.nx) && rest;
-      } else {
-        const bool rest = in_core(i, a.h, a.nx) && interior;
-        inp = rest && in_core(k + 1, a.h, a.nz);
-        inm = rest && in_core(k - 1, a.h, a.nz);
-      }
+    {
+      const bool rest = interior && in_core(k, a.h, a.nz);
       float ap0, ap1, am0, am1;
-      vel2_of<LEAN>(a, L.tv[t][0][0], L.tv[t][0][1], L.tv[t][0][2], inp, dd, 1, ap0, ap1);
-      vel2_of<LEAN>(a, L.tv[t][1][0], L.tv[t][1][1], L.tv[t][1][2], inm, dd, 1, am0, am1);
-      const float ith = a.vc.inv_two_h[dd];
-      tCur[t][0] = (ap0 - am0) * ith;
-      tCur[t][1] = (ap1 - am1) * ith;
+      vel2_of<LEAN>(a, L.tx[0][0], L.tx[0][1], L.tx[0][2], in_core(i + 1, a.h, a.nx) && rest,
+                    0, 1, ap0, ap1);
+      vel2_of<LEAN>(a, L.tx[1][0], L.tx[1][1], L.tx[1][2], in_core(i - 1, a.h, a.nx) && rest,
+                    0, 1, am0, am1);
+      const float ith = a.vc.inv_two_h[0];
+      tCur[0][0] = (ap0 - am0) * ith;
+      tCur[0][1] = (ap1 - am1) * ith;
+    }
+    {
+      const bool rest = in_core(i, a.h, a.nx) && interior;
+      float ap0, ap1, am0, am1;
+      vel2_of<LEAN>(a, zp[0], zp[1], zp[2], rest && in_core(k + 1, a.h, a.nz), 2, 1, ap0, ap1);
+      vel2_of<LEAN>(a, zm[0], zm[1], zm[2], rest && in_core(k - 1, a.h, a.nz), 2, 1, am0, am1);
+      const float ith = a.vc.inv_two_h[2];
+      tCur[1][0] = (ap0 - am0) * ith;
+      tCur[1][1] = (ap1 - am1) * ith;
     }
     vCur = visc_cell(W0, a.vc);
     vPrev = ViscCell{wave_shr1(vCur.T), wave_shr1(vCur.k)};
@@ -683,38 +678,53 @@ __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& 
   }
 }
 
-// A wave runs windows w0 .. w0 + nw - 1 of z plane k (tile (bx, ty) of the
-// launch), with the operands of each next window loaded while the current
-// one is computed (two operand sets used in turn).
+// A wave holds window (bx, ty) of the launch and marches over z planes
+// k0 .. k0 + zseg - 1 (tile bz): the operands of each next plane are loaded
+// while the current one is computed (two operand sets used in turn), and the
+// z neighbours of the viscous transverse derivatives are the planes before
+// and after, already in registers.
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
           bool LEAN = false>
-__device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti, int nw) {
+__device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti, int zseg) {
   constexpr int OUTW = YGeom<SCHEME>::OUTW;
   constexpr int NF = 5 + NS;
   const int lane = ti.tx;
-  const int w0 = (ti.bx * ti.by_dim + ti.ty) * nw;
-  const int k = a.h + ti.bz;
-  const int nwin = y_windows(a.nx, a.ny, a.h, OUTW);
+  const int win = ti.bx * ti.by_dim + ti.ty;
+  const int k0 = a.h + ti.bz * zseg;
   // uniform per wave
-  if (k >= a.h + a.nz || w0 >= nwin) return;
-  const int wend = min(w0 + nw, nwin);
+  if (k0 >= a.h + a.nz || win >= y_windows(a.nx, a.ny, a.h, OUTW)) return;
+  const int k1 = min(k0 + zseg, a.h + a.nz);
   const float dt = a.rk_form != 0 ? *a.dt : 0.0f;
+  const YPos p = y_pos<SCHEME>(a, win, lane);
+  float zm[3] = {0.0f, 0.0f, 0.0f};
+  if constexpr (VISC) {
+    const long long pm = (long long)(k0 - 1) * a.sz;
+    zm[0] = ld_plane(a.q.p[0] + pm, p.boff);
+    zm[1] = ld_plane(a.q.p[3] + pm, p.boff);
+    zm[2] = ld_plane(a.q.p[2] + pm, p.boff);
+  }
   YLoad<NF, VISC, PD> A, B;
-  YPos pa = y_pos<SCHEME>(a, k, w0, lane), pb;
-  y_load(a, k, pa, A);
-  for (int w = w0; w < wend; w += 2) {
-    const bool b_ok = w + 1 < wend;  // uniform per wave
-    if (b_ok) {
-      pb = y_pos<SCHEME>(a, k, w + 1, lane);
-      y_load(a, k, pb, B);
+  y_load(a, k0, p, A);
+  // Plane k + 1 is always loaded (at the segment's end it supplies only the
+  // z neighbour; it is at most the first halo plane).
+  for (int k = k0; k < k1; k += 2) {
+    y_load(a, k + 1, p, B);
+    {
+      const float zp[3] = {B.q[0], B.q[3], B.q[2]};
+      y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, p, A, zp, zm, dt);
+      zm[0] = A.q[0];
+      zm[1] = A.q[3];
+      zm[2] = A.q[2];
     }
-    y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, pa, A, dt);
-    if (!b_ok) break;
-    if (w + 2 < wend) {
-      pa = y_pos<SCHEME>(a, k, w + 2, lane);
-      y_load(a, k, pa, A);
+    if (k + 1 >= k1) break;
+    y_load(a, k + 2, p, A);
+    {
+      const float zp[3] = {A.q[0], A.q[3], A.q[2]};
+      y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k + 1, p, B, zp, zm, dt);
+      zm[0] = B.q[0];
+      zm[1] = B.q[3];
+      zm[2] = B.q[2];
     }
-    y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, pb, B, dt);
   }
 }
 
@@ -723,7 +733,7 @@ template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = fa
 __global__ void __launch_bounds__(256)
 sweep_y_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
-  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, ti, a.y_nw);
+  sweep_y_body<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, ti, a.seg_y);
 }
 
 // ---------------------------------------------------------------------------

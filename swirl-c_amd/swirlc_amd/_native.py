@@ -101,7 +101,7 @@ class ScTuning(ctypes.Structure):
       ('seg_z', ctypes.c_int), ('by_z', ctypes.c_int),
       ('by_y', ctypes.c_int), ('xcd_remap', ctypes.c_int),
       ('adj_seg', ctypes.c_int), ('adj_by', ctypes.c_int), ('adj_by_y', ctypes.c_int),
-      ('y_nw', ctypes.c_int),
+      ('seg_y', ctypes.c_int),
   ]
 
   def as_dict(self):

@@ -2,8 +2,8 @@
 
 The reference leaves scheduling to XLA; here each sweep kernel's launch
 geometry (cells per thread and waves per block of the marching x / z sweeps,
-waves per block and windows per wave of the y sweep, and the XCD-aware block
-order) is chosen per
+waves per block and z planes per wave of the y sweep, and the XCD-aware
+block order) is chosen per
 grid by timing the candidates on the actual state with HIP events, one
 direction at a time (a direction's time does not depend on the other
 directions' geometry), then the block order over all three.  Forward results
@@ -23,7 +23,7 @@ import torch
 from swirlc_amd import _native as N
 
 MARCH_SPACE = tuple(itertools.product((16, 32, 64), (2, 4)))  # seg, by
-Y_SPACE = tuple(itertools.product((1, 2, 4), (1, 2, 4, 8)))  # by_y, y_nw
+Y_SPACE = tuple(itertools.product((1, 2, 4), (4, 8, 16, 32)))  # by_y, seg_y
 ADJ_MARCH_SPACE = tuple(itertools.product((8, 16, 32), (1, 2, 4)))  # adj_seg, adj_by
 ADJ_Y_SPACE = (1, 2, 4)  # adj_by_y
 
@@ -58,11 +58,11 @@ def _candidates(base, march_space, y_space):
   for by, nw in y_space:
     t = dict(base)
     t['by_y'] = by
-    t['y_nw'] = nw
+    t['seg_y'] = nw
     yield 'y', t
 
 
-_KEYS = {'x': ('seg_x', 'by_x'), 'z': ('seg_z', 'by_z'), 'y': ('by_y', 'y_nw')}
+_KEYS = {'x': ('seg_x', 'by_x'), 'z': ('seg_z', 'by_z'), 'y': ('by_y', 'seg_y')}
 
 
 def _pick(timings, base):
@@ -128,11 +128,11 @@ def tune_context(nctx, q, out, dt_dev, u0=None, reps=5, march_space=MARCH_SPACE,
 def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=None):
   """Times one RK3 stage-2 update of the whole ensemble per candidate (the
   other directions held at the base geometry) and keeps the fastest per
-  direction (the batched y sweep runs one window per wave).  Returns
+  direction (the batched y sweep runs one plane per wave).  Returns
   (tuning, [(direction, tuning, ms)])."""
   base = batch.get_tuning()
   if y_space is None:
-    y_space = tuple((by, base['y_nw']) for by in (1, 2, 4))
+    y_space = tuple((by, base['seg_y']) for by in (1, 2, 4))
   timings = []
   for d, t in _candidates(base, march_space, y_space):
     batch.set_tuning(t)

@@ -18,10 +18,11 @@ from swirlc_amd.equations import compressible_navier_stokes as cns
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [
-    dict(seg_x=16, by_x=2, seg_z=64, by_z=1, by_y=2, xcd_remap=1, y_nw=1),
-    dict(seg_x=64, by_x=4, seg_z=16, by_z=4, by_y=1, xcd_remap=0, y_nw=3),
-    dict(seg_x=5, by_x=3, seg_z=7, by_z=3, by_y=3, xcd_remap=1, y_nw=8),
-    dict(seg_x=32, by_x=4, seg_z=32, by_z=4, by_y=4, xcd_remap=1, y_nw=64),
+    dict(seg_x=16, by_x=2, seg_z=64, by_z=1, by_y=2, xcd_remap=1, seg_y=1),
+    dict(seg_x=64, by_x=4, seg_z=16, by_z=4, by_y=1, xcd_remap=0, seg_y=3),
+    dict(seg_x=5, by_x=3, seg_z=7, by_z=3, by_y=3, xcd_remap=1, seg_y=8),
+    dict(seg_x=32, by_x=4, seg_z=32, by_z=4, by_y=4, xcd_remap=1, seg_y=64),
+    dict(seg_x=32, by_x=4, seg_z=32, by_z=4, by_y=4, xcd_remap=1, seg_y=7),
 ]
 
 
@@ -53,7 +54,7 @@ def test_tuning_is_bitwise_neutral_single_grid():
   with pytest.raises(N.NativeError):
     ctx.set_tuning(dict(base, seg_x=2))
   with pytest.raises(N.NativeError):
-    ctx.set_tuning(dict(base, y_nw=0))
+    ctx.set_tuning(dict(base, seg_y=0))
   with pytest.raises(N.NativeError):
     ctx.set_tuning(dict(base, adj_by=5))
   with pytest.raises(N.NativeError):
