@@ -103,6 +103,8 @@ def parse():
   ap.add_argument('--adjoint-steps', type=int, default=200)
   ap.add_argument('--adjoint-every', type=int, default=20)
   ap.add_argument('--no-adjoint', action='store_true')
+  ap.add_argument('--adjoint-only', action='store_true',
+                  help='run only the adjoint leg (for a rocprofv3 kernel trace of it)')
   ap.add_argument('--no-large', action='store_true',
                   help='skip the 512 x 512 x 256 buoyant-bubble leg')
   ap.add_argument('--large-steps', type=int, default=10)
@@ -520,6 +522,10 @@ def main():
     sys.exit(2)
   if args.pmc_probe:
     pmc_probe(args.n)
+    return
+  if args.adjoint_only:
+    print(json.dumps(bench_adjoint(args.adjoint_n, args.adjoint_steps, args.adjoint_every)),
+          flush=True)
     return
   import torch
   from swirlc_amd import configs
