@@ -94,7 +94,12 @@ typedef struct sc_grid {
 /* A Dirichlet value on a boundary face of the domain (the faces between halo
  * and interior cells): cfg.bc['intercell_fluxes'][type][var][dim][side]
  * (boundary/boundary.py:89-183) and cfg.bc['cell_faces'][var][dim][side]
- * (boundary/boundary.py:230-461), scalar values. */
+ * (boundary/boundary.py:230-461).  A scalar `value`, or with `plane` set a
+ * value per cell of the boundary plane (the reference's
+ * tensor_scatter_1d_update_global of a plane): a device array [A][B] over
+ * the full extents of the two other axes in [z][x][y] order (x faces
+ * [nz+2h][ny+2h], y faces [nz+2h][nx+2h], z faces [nx+2h][ny+2h]), kept
+ * alive by the caller while the context is used. */
 #define SC_MAX_BOUNDARY_VALUES 64
 #define SC_MAX_FACE_VALUES 8      /* cell-face values per (dim, side) */
 #define SC_BV_FLUX_CONVECTIVE 0   /* var: conserved field index */
@@ -107,6 +112,7 @@ typedef struct sc_boundary_value {
   int dim, side;            /* 0 x, 1 y, 2 z; 0 low, 1 high */
   int var;
   float value;
+  const float* plane;       /* device plane of values, or NULL */
 } sc_boundary_value;
 
 typedef struct sc_params {
@@ -186,7 +192,8 @@ sc_status sc_rk_stage_noalloc(sc_ctx* ctx, int form, float c0, float c1,
  * (zero in every halo cell) is reconstructed to the faces and replaces p in
  * the momentum flux (and, for HLLC, in the wave speeds and star states,
  * hllc.py:109-120); d_rho = rho - rho_ref replaces rho in the gravity source.
- * The caller keeps the fields alive.  5 fields only. */
+ * The caller keeps the fields alive.  Any number of fields (transported
+ * scalars included). */
 sc_status sc_ctx_set_reference_state(sc_ctx* ctx, const float* p_ref, const float* rho_ref);
 
 /* Maximum over interior cells of max_d(|u_d| + sqrt(gamma p / rho)) written to
@@ -212,6 +219,14 @@ sc_status sc_face_interp(const float* v, const int* extents, int dim, int scheme
  * write of update_fluxes / update_flux_1d (boundary/boundary.py:89-183). */
 sc_status sc_fill_plane(float* f, const int* extents, int dim, int index, float value,
                         void* stream);
+
+/* sc_fill_plane with one value per plane cell (a plane-valued update of
+ * tensor_scatter_1d_update_global, called at boundary/boundary.py:144,281,
+ * 302,324): `values` is a device array [A][B] over the full extents of the two
+ * other axes in [z][x][y] order: dim 0 [Nz][Ny], dim 1 [Nz][Nx], dim 2
+ * [Nx][Ny]. */
+sc_status sc_fill_plane_values(float* f, const int* extents, int dim, int index,
+                               const float* values, void* stream);
 
 /* Whole-array steps of the face BC update _face_boundary_update_1d
  * (boundary/boundary.py:230-337) in IEEE fp32 over n elements:

@@ -136,6 +136,7 @@ sc_status build_boundary_tables(const sc_params& p, sc::BoundaryBc (*t)[2], int*
         return fail(SC_ERR_INVALID, "flux boundary value: var must be a conserved field index");
       b.flux_mask |= 1 << (8 * v.kind + v.var);
       b.flux_val[v.kind][v.var] = v.value;
+      b.flux_plane[v.kind][v.var] = v.plane;
     } else if (v.kind == SC_BV_FACE_CONSERVED || v.kind == SC_BV_FACE_PRIMITIVE) {
       if (p.include_diffusion)
         return fail(SC_ERR_UNSUPPORTED,
@@ -155,7 +156,7 @@ sc_status build_boundary_tables(const sc_params& p, sc::BoundaryBc (*t)[2], int*
       }
       if (b.n_face >= SC_MAX_FACE_VALUES)
         return fail(SC_ERR_INVALID, "too many face boundary values on one boundary");
-      b.face[b.n_face++] = {v.kind == SC_BV_FACE_CONSERVED ? 1 : 2, var, v.value};
+      b.face[b.n_face++] = {v.kind == SC_BV_FACE_CONSERVED ? 1 : 2, var, v.value, v.plane};
     } else {
       return fail(SC_ERR_INVALID, "boundary value: unknown kind");
     }
@@ -635,17 +636,30 @@ sc_status sc_face_bc_elementwise(int op, const float* a, const float* b, float* 
                    "elementwise kernel");
 }
 
+static sc_status fill_plane_with(const char* name, float* f, const int* extents, int dim,
+                                 int index, float value, const float* values, void* stream) {
+  if (f == nullptr || extents == nullptr)
+    return fail(SC_ERR_INVALID, std::string(name) + ": null argument");
+  if (dim < 0 || dim > 2)
+    return fail(SC_ERR_INVALID, std::string(name) + ": dim must be 0 (x), 1 (y) or 2 (z)");
+  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
+    return fail(SC_ERR_INVALID, std::string(name) + ": extents must be positive");
+  if (index < 0 || index >= extents[dim])
+    return fail(SC_ERR_INVALID, std::string(name) + ": plane index outside the array");
+  return hip_check(sc::launch_fill_plane(f, extents, dim, index, value, values,
+                                         (hipStream_t)stream),
+                   "plane fill kernel");
+}
+
 sc_status sc_fill_plane(float* f, const int* extents, int dim, int index, float value,
                         void* stream) {
-  if (f == nullptr || extents == nullptr) return fail(SC_ERR_INVALID, "sc_fill_plane: null argument");
-  if (dim < 0 || dim > 2)
-    return fail(SC_ERR_INVALID, "sc_fill_plane: dim must be 0 (x), 1 (y) or 2 (z)");
-  if (extents[0] < 1 || extents[1] < 1 || extents[2] < 1)
-    return fail(SC_ERR_INVALID, "sc_fill_plane: extents must be positive");
-  if (index < 0 || index >= extents[dim])
-    return fail(SC_ERR_INVALID, "sc_fill_plane: plane index outside the array");
-  return hip_check(sc::launch_fill_plane(f, extents, dim, index, value, (hipStream_t)stream),
-                   "plane fill kernel");
+  return fill_plane_with("sc_fill_plane", f, extents, dim, index, value, nullptr, stream);
+}
+
+sc_status sc_fill_plane_values(float* f, const int* extents, int dim, int index,
+                               const float* values, void* stream) {
+  if (values == nullptr) return fail(SC_ERR_INVALID, "sc_fill_plane_values: null values");
+  return fill_plane_with("sc_fill_plane_values", f, extents, dim, index, 0.0f, values, stream);
 }
 
 sc_status sc_linear_interp(const float* v, const int* extents, int dim, float* out,
@@ -830,8 +844,6 @@ sc_status sc_ctx_set_reference_state(sc_ctx* c, const float* p_ref, const float*
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_set_reference_state: null context");
   if ((p_ref == nullptr) != (rho_ref == nullptr))
     return fail(SC_ERR_INVALID, "sc_ctx_set_reference_state: give both p_ref and rho_ref, or neither");
-  if (p_ref != nullptr && c->nf != 5)
-    return fail(SC_ERR_UNSUPPORTED, "a reference state is supported without transported scalars only");
   c->sweep.p_ref = p_ref;
   c->sweep.rho_ref = rho_ref;
   return SC_OK;

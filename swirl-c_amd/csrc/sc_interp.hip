@@ -76,7 +76,8 @@ linear_interp_kernel(const float* __restrict__ v, LineGeom g, float* __restrict_
 
 // f = value on the plane coordinate[dim] == index (one thread per plane cell).
 __global__ void __launch_bounds__(256)
-fill_plane_kernel(float* f, int nx, int ny, int nz, int dim, int index, float value) {
+fill_plane_kernel(float* f, int nx, int ny, int nz, int dim, int index, float value,
+                  const float* values) {
   const int na = dim == 0 ? nz : (dim == 1 ? nz : nx);  // slow plane axis
   const int nb = dim == 1 ? nx : ny;                     // fast plane axis
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,7 +91,8 @@ fill_plane_kernel(float* f, int nx, int ny, int nz, int dim, int index, float va
   } else {
     z = index; x = ia; y = ib;
   }
-  f[((long long)z * nx + x) * ny + y] = value;
+  // values: one per plane cell, [ia][ib] (null: the scalar value everywhere).
+  f[((long long)z * nx + x) * ny + y] = values != nullptr ? values[t] : value;
 }
 
 // Whole-array steps of the face BC update (boundary.py:230-337), IEEE fp32:
@@ -122,12 +124,12 @@ hipError_t launch_elementwise(int op, const float* a, const float* b, float* out
 }
 
 hipError_t launch_fill_plane(float* f, const int N[3], int dim, int index, float value,
-                             hipStream_t stream) {
+                             const float* values, hipStream_t stream) {
   const int na = dim == 2 ? N[0] : N[2];
   const int nb = dim == 1 ? N[0] : N[1];
   const long long n = (long long)na * nb;
   hipLaunchKernelGGL(fill_plane_kernel, dim3((unsigned)((n + 255) / 256)), 256, 0, stream, f,
-                     N[0], N[1], N[2], dim, index, value);
+                     N[0], N[1], N[2], dim, index, value, values);
   return hipGetLastError();
 }
 

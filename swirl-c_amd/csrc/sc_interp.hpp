@@ -17,6 +17,7 @@ hipError_t launch_elementwise(int op, const float* a, const float* b, float* out
                               hipStream_t stream);
 // f = value on the plane coordinate[dim] == index of the [z][x][y] array.
 hipError_t launch_fill_plane(float* f, const int N[3], int dim, int index, float value,
+                             const float* values,
                              hipStream_t stream);
 // out[i] = 0.5 * (v[i-1] + v[i]) along dim, v[-1] = 0.
 hipError_t launch_linear_interp(const float* v, const int N[3], int dim, float* out,

@@ -76,32 +76,28 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
   return hipGetLastError();
 }
 
-// Picks the build for the configuration: boundary values (BB) or a reference
-// state (PD, scalars-free runs only) or neither; without either and without
-// gravity or primitive-variable BCs, the LEAN build.
+// Picks the build for the configuration: boundary values (BB) and / or a
+// reference state (PD); without either and without gravity or
+// primitive-variable BCs, the LEAN build.
+template <int SCHEME, int FLUX, bool VISC, int NS>
+hipError_t launch_rhs_ns(const SweepArgs& a, const float h[3], const SweepTuning& tune,
+                         hipStream_t stream, int od) {
+  const bool bb = a.bbc_dims != 0;
+  if (!bb && a.p_ref == nullptr && a.prim_bc_mask == 0 && !a.has_pe && !a.grav_src)
+    return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, false, false, true>(a, h, tune, stream, od);
+  if (a.p_ref != nullptr)
+    return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, true, true>(a, h, tune, stream, od)
+              : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, true, false>(a, h, tune, stream, od);
+  return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, false, true>(a, h, tune, stream, od)
+            : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, false, false>(a, h, tune, stream, od);
+}
+
 template <int SCHEME, int FLUX, bool VISC>
 hipError_t launch_rhs_variant(const SweepArgs& a, const float h[3], int ns,
                               const SweepTuning& tune, hipStream_t stream, int od) {
-  const bool bb = a.bbc_dims != 0;
-  if (!bb && a.p_ref == nullptr && a.prim_bc_mask == 0 && !a.has_pe && !a.grav_src) {
-    if (ns == 0)
-      return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, false, true>(a, h, tune, stream, od);
-    if (ns == 1)
-      return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, false, true>(a, h, tune, stream, od);
-    return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 2, false, false, true>(a, h, tune, stream, od);
-  }
-  if (ns == 0) {
-    if (a.p_ref != nullptr)
-      return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, true, true>(a, h, tune, stream, od)
-                : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, true, false>(a, h, tune, stream, od);
-    return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, true>(a, h, tune, stream, od)
-              : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 0, false, false>(a, h, tune, stream, od);
-  }
-  if (ns == 1)
-    return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, true>(a, h, tune, stream, od)
-              : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(a, h, tune, stream, od);
-  return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 2, false, true>(a, h, tune, stream, od)
-            : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, 2, false, false>(a, h, tune, stream, od);
+  if (ns == 0) return launch_rhs_ns<SCHEME, FLUX, VISC, 0>(a, h, tune, stream, od);
+  if (ns == 1) return launch_rhs_ns<SCHEME, FLUX, VISC, 1>(a, h, tune, stream, od);
+  return launch_rhs_ns<SCHEME, FLUX, VISC, 2>(a, h, tune, stream, od);
 }
 
 template <int SCHEME, int NS, bool PD = false>
@@ -136,11 +132,15 @@ hipError_t launch_branch_codes_t(const SweepArgs& a, int dim, uint8_t* codes,
   template <>                                                                    \
   hipError_t launch_branch_codes<SCHEME>(const SweepArgs& a, int dim, int ns,    \
                                          uint8_t* codes, hipStream_t stream) {   \
-    if (ns == 0 && a.p_ref != nullptr)                                           \
-      return launch_branch_codes_t<SCHEME, 0, true>(a, dim, codes, stream);      \
-    if (ns == 0) return launch_branch_codes_t<SCHEME, 0>(a, dim, codes, stream); \
-    return ns == 1 ? launch_branch_codes_t<SCHEME, 1>(a, dim, codes, stream)     \
-                   : launch_branch_codes_t<SCHEME, 2>(a, dim, codes, stream);    \
+    const bool pd = a.p_ref != nullptr;                                          \
+    if (ns == 0)                                                                 \
+      return pd ? launch_branch_codes_t<SCHEME, 0, true>(a, dim, codes, stream)  \
+                : launch_branch_codes_t<SCHEME, 0>(a, dim, codes, stream);       \
+    if (ns == 1)                                                                 \
+      return pd ? launch_branch_codes_t<SCHEME, 1, true>(a, dim, codes, stream)  \
+                : launch_branch_codes_t<SCHEME, 1>(a, dim, codes, stream);       \
+    return pd ? launch_branch_codes_t<SCHEME, 2, true>(a, dim, codes, stream)    \
+              : launch_branch_codes_t<SCHEME, 2>(a, dim, codes, stream);         \
   }
 
 }  // namespace sc

@@ -56,16 +56,21 @@ struct SweepTuning {
 // Dirichlet values on the domain's boundary faces of one sweep direction
 // (boundary/boundary.py:89-183 intercell fluxes, :230-461 cell faces).
 constexpr int kMaxFaceEntries = 8;
+// A value is a scalar, or (plane != null) one value per cell of the boundary
+// plane, [A][B] over the full extents of the two other axes in [z][x][y]
+// order: x faces [z][y], y faces [z][x], z faces [x][y].
 struct FaceEntry {
   int kind;   // 1: conserved field (face primitive = value / rho), 2: primitive
   int var;    // index into the reconstructed primitives (1 u .. 6 h, 7+s scalar)
   float val;
+  const float* plane;
 };
 struct BoundaryBc {
   int n_face;                         // face-state entries, applied in order
   FaceEntry face[kMaxFaceEntries];
   int flux_mask;                      // bit 8*t + f: flux type t of field f set
   float flux_val[3][kMaxFields];      // t: 0 convective, 1 diffusive, 2 total
+  const float* flux_plane[3][kMaxFields];
 };
 
 // Block geometry of one tile: block indices (bx, by, bz) of the plain launch
@@ -268,12 +273,14 @@ __device__ __forceinline__ float rk_update(const SweepArgs& a, float uk, float u
 // Face-state overwrite of boundary_face_update_1d on one side's reconstructed
 // primitives: a conserved value gives primitive = divide_no_nan(value, rho),
 // a primitive value is taken as is.
+// pidx: the face's index in the boundary plane's [A][B] values.
 template <int NS>
-__device__ __forceinline__ void apply_face_bc(const BoundaryBc& b, float* W) {
+__device__ __forceinline__ void apply_face_bc(const BoundaryBc& b, float* W, long long pidx) {
   constexpr int NV = NVar<NS>::value;
   for (int e = 0; e < b.n_face; ++e) {
     const FaceEntry fe = b.face[e];
-    const float v = fe.kind == 1 ? dnn_exact(fe.val, W[kRho]) : fe.val;
+    const float val = fe.plane != nullptr ? fe.plane[pidx] : fe.val;
+    const float v = fe.kind == 1 ? dnn_exact(val, W[kRho]) : val;
 #pragma unroll
     for (int k = 1; k < NV; ++k)
       if (k == fe.var) W[k] = v;
@@ -281,10 +288,12 @@ __device__ __forceinline__ void apply_face_bc(const BoundaryBc& b, float* W) {
 }
 
 template <int NF>
-__device__ __forceinline__ void apply_flux_bc(const BoundaryBc& b, int t, float* F) {
+__device__ __forceinline__ void apply_flux_bc(const BoundaryBc& b, int t, float* F,
+                                              long long pidx) {
 #pragma unroll
   for (int f = 0; f < NF; ++f)
-    if (b.flux_mask & (1 << (8 * t + f))) F[f] = b.flux_val[t][f];
+    if (b.flux_mask & (1 << (8 * t + f)))
+      F[f] = b.flux_plane[t][f] != nullptr ? b.flux_plane[t][f][pidx] : b.flux_val[t][f];
 }
 
 // Total flux F of one face: Riemann flux of the face states L, R plus the
@@ -293,32 +302,32 @@ __device__ __forceinline__ void apply_flux_bc(const BoundaryBc& b, int t, float*
 // convective flux, the diffusive flux (viscous runs only) and their sum are
 // overwritten in the reference's order (compressible_navier_stokes.py:147-234).
 template <int DIR, int FLUX, bool VISC, int NS, bool PD>
-__device__ __forceinline__ void face_flux(const BoundaryBc* b, float* L, float* R, float pe,
-                                          const float* A, const float* B,
+__device__ __forceinline__ void face_flux(const BoundaryBc* b, long long pidx, float* L,
+                                          float* R, float pe, const float* A, const float* B,
                                           const float (*ta)[2], const float (*tb)[2],
                                           ViscCell ca, ViscCell cb,
                                           const ViscConst& vc, float* F) {
   constexpr int NF = 5 + NS;
   if (b != nullptr) {
-    apply_face_bc<NS>(*b, L);
-    apply_face_bc<NS>(*b, R);
+    apply_face_bc<NS>(*b, L, pidx);
+    apply_face_bc<NS>(*b, R, pidx);
   }
   riemann<FLUX, DIR, NS, PD>(L, R, pe, F);
-  if (b != nullptr) apply_flux_bc<NF>(*b, 0, F);
+  if (b != nullptr) apply_flux_bc<NF>(*b, 0, F, pidx);
   if constexpr (VISC) {
     if (b != nullptr) {
       float Fd[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) Fd[f] = 0.0f;
       viscous_flux_cells<DIR, NS>(A, B, ta, tb, ca, cb, vc, Fd);
-      apply_flux_bc<NF>(*b, 1, Fd);
+      apply_flux_bc<NF>(*b, 1, Fd, pidx);
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] = F[f] + Fd[f];
     } else {
       viscous_flux_cells<DIR, NS>(A, B, ta, tb, ca, cb, vc, F);
     }
   }
-  if (b != nullptr) apply_flux_bc<NF>(*b, 2, F);
+  if (b != nullptr) apply_flux_bc<NF>(*b, 2, F, pidx);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,7 +415,9 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
       const int side = c == a.h ? 0 : (c == a.h + n_march ? 1 : -1);
       const BoundaryBc* bb =
           (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
-      face_flux<DIR, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at<LEAN>(a, kk(c), ii(c), j),
+      // Boundary plane cell of this face: x faces [z][y], z faces [x][y].
+      face_flux<DIR, FLUX, VISC, NS, PD>(bb, (long long)o * a.NY + j, negPrev, posM,
+                                         pe_at<LEAN>(a, kk(c), ii(c), j),
                                          W[RAD - 1], W[RAD], tPrev, tCur, vPrev, vCur, a.vc,
                                          F);
       if (c > c_begin) {
@@ -658,7 +669,9 @@ __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& 
   const int side = yc == a.h ? 0 : (yc == a.h + a.ny ? 1 : -1);
   const BoundaryBc* bb =
       (BB && ((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
-  face_flux<1, FLUX, VISC, NS, PD>(bb, negPrev, posM, pe_at<LEAN>(a, k, i, yc), Am, W0,
+  // Boundary plane cell of this face: y faces [z][x].
+  face_flux<1, FLUX, VISC, NS, PD>(bb, (long long)k * a.NX + i, negPrev, posM,
+                                   pe_at<LEAN>(a, k, i, yc), Am, W0,
                                    tPrev, tCur, vPrev, vCur, a.vc, F);
   float Fn[NF];
 #pragma unroll
@@ -779,8 +792,10 @@ branch_codes_kernel(const SweepArgs a, uint8_t* codes) {
   }
   const int side = pc == a.h ? 0 : (pc == a.h + n ? 1 : -1);
   if (((a.bbc_dims >> D) & 1) && side >= 0) {
-    apply_face_bc<NS>(a.bbc[2 * D + side], Wl);
-    apply_face_bc<NS>(a.bbc[2 * D + side], Wr);
+    const long long pidx = D == 0 ? (long long)k * a.NY + j
+                                  : (D == 1 ? (long long)k * a.NX + i : (long long)i * a.NY + j);
+    apply_face_bc<NS>(a.bbc[2 * D + side], Wl, pidx);
+    apply_face_bc<NS>(a.bbc[2 * D + side], Wr, pidx);
   }
   float F[5 + NS];
   codes[c] = (uint8_t)hllc_flux<D, NS, PD>(Wl, Wr, pe_at(a, k, i, j), F);

@@ -33,7 +33,7 @@ EXPORTED = (
     'sc_ctx_set_reference_state', 'sc_face_interp', 'sc_linear_interp', 'sc_riemann_flux',
     'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
     'sc_face_bc_elementwise', 'sc_last_error',
-    'sc_version', 'sc_rk_stage_noalloc', 'sc_sweep_only_timed',
+    'sc_version', 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
 )
 
 
@@ -67,7 +67,7 @@ SC_MAX_FACE_VALUES = 8
 class ScBoundaryValue(ctypes.Structure):
   _fields_ = [
       ('kind', ctypes.c_int), ('dim', ctypes.c_int), ('side', ctypes.c_int),
-      ('var', ctypes.c_int), ('value', ctypes.c_float),
+      ('var', ctypes.c_int), ('value', ctypes.c_float), ('plane', ctypes.c_void_p),
   ]
 
 
@@ -150,6 +150,7 @@ def lib():
     L.sc_heat_flux.argtypes = [vp, vp, ip3, dp3, ctypes.c_double, ctypes.c_double, fpp, vp]
     L.sc_exchange_halos.argtypes = [vp, ip3, i, ctypes.POINTER(ScBcFace), vp]
     L.sc_fill_plane.argtypes = [vp, ip3, i, i, f, vp]
+    L.sc_fill_plane_values.argtypes = [vp, ip3, i, i, vp, vp]
     L.sc_face_bc_elementwise.argtypes = [i, vp, vp, vp, ctypes.c_longlong, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
@@ -179,7 +180,7 @@ def lib():
                  'sc_batch_set_tuning', 'sc_ctx_set_reference_state', 'sc_face_interp',
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
                  'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise',
-                 'sc_rk_stage_noalloc', 'sc_sweep_only_timed'):
+                 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

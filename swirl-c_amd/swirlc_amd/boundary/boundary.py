@@ -23,6 +23,7 @@ by later calls with an equal configuration.
 """
 
 import ctypes
+import numbers
 
 import torch
 
@@ -116,7 +117,8 @@ def update_fluxes(replica_id, replicas, fluxes, cfg, flux_type):
   (index halo_width on the low face, halo_width + core_n on the high face);
   unchanged when no such entry exists.  Only conservative variables may
   carry a flux BC (AssertionError, as in the reference); a None type is
-  skipped; values must be scalars.  Modified fields are new tensors."""
+  skipped; values are scalars or planes (see _plane).  Modified fields are
+  new tensors."""
   del replica_id, replicas
   spec = _flux_spec(cfg, flux_type)
   if spec is None:
@@ -137,18 +139,8 @@ def update_fluxes(replica_id, replicas, fluxes, cfg, flux_type):
         if kind is None:
           continue
         assert kind == 'DIRICHLET', f'Unsupported interface (face or flux) BC type {kind}.'
-        val = entry[1]
-        if not isinstance(val, (int, float)):
-          raise NotImplementedError('flux boundary values must be scalars')
-        f = out[dim][var].clone()
-        N.require_gpu_tensor(f, 'flux')
-        nz, nx, ny = f.shape
         index = hw if side == 0 else hw + int(core_n[dim])
-        N.check(N.lib().sc_fill_plane(ctypes.c_void_p(f.data_ptr()),
-                                      (ctypes.c_int * 3)(nx, ny, nz), d, index,
-                                      ctypes.c_float(float(val)), N.stream_handle()),
-                'sc_fill_plane')
-        out[dim][var] = f
+        out[dim][var] = _plane(out[dim][var], d, index, entry[1], 'flux')
   return out
 
 
@@ -166,15 +158,27 @@ def _ew(op, a, b):
   return out
 
 
-def _plane(f, d, index, val):
-  N.require_gpu_tensor(f, 'face state')
+def _plane(f, d, index, val, what='face state'):
+  """A copy of f with plane `index` normal to dim d set to val: a scalar, or
+  a plane of values (2-D over the two other axes, a 3-D slice with unit
+  extent along d, or a list of such slices), as
+  tensor_scatter_1d_update_global takes them."""
+  N.require_gpu_tensor(f, what)
   if f.dim() != 3:
-    raise ValueError(f'expected a 3-D face state [nz, nx, ny], got shape {tuple(f.shape)}')
+    raise ValueError(f'expected a 3-D {what} [nz, nx, ny], got shape {tuple(f.shape)}')
   f = f.clone()
   nz, nx, ny = f.shape
-  N.check(N.lib().sc_fill_plane(ctypes.c_void_p(f.data_ptr()), (ctypes.c_int * 3)(nx, ny, nz),
-                                d, index, ctypes.c_float(float(val)), N.stream_handle()),
-          'sc_fill_plane')
+  ext = (ctypes.c_int * 3)(nx, ny, nz)
+  if isinstance(val, numbers.Number):
+    N.check(N.lib().sc_fill_plane(ctypes.c_void_p(f.data_ptr()), ext, d, index,
+                                  ctypes.c_float(float(val)), N.stream_handle()),
+            'sc_fill_plane')
+    return f
+  host = _OneField(f.shape, 0, f.device)._layer_plane(val, d)
+  plane = torch.from_numpy(host).to(f.device)
+  N.check(N.lib().sc_fill_plane_values(ctypes.c_void_p(f.data_ptr()), ext, d, index,
+                                       ctypes.c_void_p(plane.data_ptr()), N.stream_handle()),
+          'sc_fill_plane_values')
   return f
 
 
@@ -214,8 +218,8 @@ def update_faces(replica_id, replicas, conservative_neg, conservative_pos, primi
   reference's update of its partner variable over the whole array
   (primitive = divide_no_nan(conservative, rho), or conservative = rho *
   primitive); a density value rebuilds the primitive dict from the
-  conservative one, as the reference does.  Values must be scalars; inputs
-  are not modified."""
+  conservative one, as the reference does.  Values are scalars or planes
+  (see _plane); inputs are not modified."""
   del replica_id, replicas
   table = (cfg.bc or {}).get('cell_faces')
   if not table:
@@ -235,8 +239,6 @@ def update_faces(replica_id, replicas, conservative_neg, conservative_pos, primi
         if kind is None:
           continue
         assert kind == 'DIRICHLET', f'Unsupported interface (face or flux) BC type {kind}.'
-        if not isinstance(entry[1], (int, float)):
-          raise NotImplementedError('face boundary values must be scalars')
         spec[var] = entry[1]
       index = hw if side == 0 else hw + int(core_n[dim])
       cn[dim], pn[dim] = _face_update_1d(cn[dim], pn[dim], spec, d, index)

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from swirlc_amd.common import types
-from swirlc_amd.equations.native_context import NativeContext
+from swirlc_amd.equations.native_context import make_context
 
 
 class InterpolationSchemes(enum.Enum):
@@ -49,7 +49,7 @@ def get_rhs_fn(replicas: np.ndarray, cfg, physics_models=None):
   configuration, not inside a loop: it creates the native context.
   """
   del replicas, physics_models  # one replica per GPU; models come from cfg
-  ctx = NativeContext(cfg)
+  ctx = make_context(cfg)
 
   def rhs(replica_id, conservative, helper_vars):
     """d(conservative)/dt over the interior cells (:69-268)."""

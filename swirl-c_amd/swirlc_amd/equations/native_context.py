@@ -7,6 +7,7 @@ methods launch the HIP kernels on the current torch stream.
 """
 
 import collections
+import copy
 import ctypes
 import hashlib
 import numbers
@@ -60,7 +61,9 @@ class NativeContext:
     """Validates cfg and builds the native parameters; `create=False` stops
     before the native context is made (the parameters are reused by a batch)."""
     self._ctx = None
-    self._pending_planes = []  # (sc_bc_face, host planes) uploaded at creation
+    # (struct, pointer field, host planes) uploaded at creation: per-layer
+    # DIRICHLET halo planes and plane-valued face / flux boundary values.
+    self._pending_planes = []
     validate_models(cfg)
     scheme = _get(cfg, 'interpolation_scheme')
     if scheme not in _INTERP:
@@ -87,7 +90,9 @@ class NativeContext:
       raise ValueError(
           f'conservative_variable_names must start with {list(types.BASE_CONSERVATIVE)}')
     if len(self.names) > 7:
-      raise NotImplementedError('at most two transported scalars are supported')
+      raise NotImplementedError(
+          'one native context holds at most two transported scalars (make_context'
+          ' splits more of them over several contexts)')
     # The reference rebuilds face conserved states from the primitives listed
     # in primitive_variable_names (compressible_navier_stokes.py:131-144), so a
     # scalar whose primitive is not listed has no face state there.
@@ -175,12 +180,13 @@ class NativeContext:
     NativeContext.created += 1
 
   def upload_planes(self):
-    """Copies the per-layer DIRICHLET planes to the device and points the BC
-    table at them (kept alive by the context)."""
-    for face, planes in self._pending_planes:
+    """Copies the per-layer DIRICHLET planes and the plane-valued boundary
+    values to the device and points the C tables at them (kept alive by the
+    context)."""
+    for obj, field, planes in self._pending_planes:
       dev = torch.from_numpy(planes).to(self.device).contiguous()
       self._keep.append(dev)
-      face.layer_planes = dev.data_ptr()
+      setattr(obj, field, dev.data_ptr())
     self._pending_planes = []
 
   def cache_key(self):
@@ -196,11 +202,14 @@ class NativeContext:
       for d in range(3):
         for s in range(2):
           params.bc[f][d][s].layer_planes = None
+    for k in range(N.SC_MAX_BOUNDARY_VALUES):
+      params.boundary_values[k].plane = None
     digest.update(bytes(params))
     for arr in self._keep:
       if isinstance(arr, np.ndarray):
         digest.update(arr.tobytes())
-    for _, planes in self._pending_planes:
+    for _, field, planes in self._pending_planes:
+      digest.update(field.encode())
       digest.update(planes.tobytes())
     digest.update(str(self.device).encode())
     return digest.digest()
@@ -210,7 +219,8 @@ class NativeContext:
   @staticmethod
   def _dirichlet_along(spec, dim, side):
     """boundary/boundary.py:186-227: (var -> value) of the DIRICHLET entries
-    on one boundary; `None` types are skipped."""
+    on one boundary; `None` types are skipped.  A value is a float, or the
+    value given (a plane, see _layer_plane)."""
     out = {}
     for var, d in spec.items():
       t, val = d[dim][side]
@@ -220,9 +230,7 @@ class NativeContext:
         raise AssertionError(
             f'Unsupported interface (face or flux) BC type {t}. Only "DIRICHLET"'
             ' or `None` is supported.')
-      if not isinstance(val, (int, float, np.floating, np.integer)):
-        raise NotImplementedError('face and flux boundary values must be scalars')
-      out[var] = float(val)
+      out[var] = float(val) if isinstance(val, numbers.Number) else val
     return out
 
   def _fill_boundary_values(self, params, bc):
@@ -278,7 +286,14 @@ class NativeContext:
     params.n_boundary_values = len(entries)
     for k, (kind, d, side, var, val) in enumerate(entries):
       e = params.boundary_values[k]
-      e.kind, e.dim, e.side, e.var, e.value = kind, d, side, var, val
+      e.kind, e.dim, e.side, e.var = kind, d, side, var
+      if isinstance(val, float):
+        e.value = val
+      else:
+        # A plane of values over the face (tensor_scatter_1d_update_global
+        # with a tensor update): uploaded with the context.
+        e.value = 0.0
+        self._pending_planes.append((e, 'plane', self._layer_plane(val, d)))
 
   def _plane_shape(self, d):
     nz, nx, ny = self.shape
@@ -328,7 +343,7 @@ class NativeContext:
             out.layer_values[i] = float(v)
         else:
           planes = np.ascontiguousarray(np.stack([self._layer_plane(v, d) for v in val]))
-          self._pending_planes.append((out, planes))
+          self._pending_planes.append((out, 'layer_planes', planes))
       else:
         raise NotImplementedError(f'unsupported DIRICHLET value of type {type(val)}')
 
@@ -464,16 +479,169 @@ class NativeContext:
         N.stream_handle(stream)), 'sc_rk_stage_vjp')
 
 
+def _part_cfg(cfg, keep, drop):
+  """cfg restricted to the conserved fields `keep`: the scalars in `drop`
+  (and their primitives) leave the variable lists and the face / flux value
+  tables."""
+  src = cfg._simulation_cfg if hasattr(cfg, '_simulation_cfg') else cfg
+  out = copy.deepcopy(dict(src))
+  gone = set(drop) | {types.conservative_to_primitive_name(n) for n in drop}
+  out['conservative_variable_names'] = list(keep)
+  prims = out.get('primitive_variable_names')
+  if prims is not None:
+    out['primitive_variable_names'] = [v for v in prims if v not in gone]
+  bc = dict(out.get('bc') or {})
+  if bc.get('cell_faces'):
+    bc['cell_faces'] = {v: s for v, s in bc['cell_faces'].items() if v not in gone}
+  if bc.get('intercell_fluxes'):
+    bc['intercell_fluxes'] = {t: {v: s for v, s in spec.items() if v not in gone}
+                              for t, spec in bc['intercell_fluxes'].items()}
+  out['bc'] = bc
+  return type(cfg)(out) if hasattr(cfg, '_simulation_cfg') else out
+
+
+class ChunkedContext:
+  """The native context of a state with more than two transported scalars.
+
+  Scalars are passive: their fluxes depend on the base fields (rho, momentum,
+  energy) and on the scalar itself only (compressible_navier_stokes.py:419-
+  427, hllc.py:182-192, hll.py).  The scalars are split in groups of two;
+  part c is a NativeContext over the five base fields and group c.  Every
+  call runs each part over its fields; the parts after the first send their
+  copies of the base-field outputs to scratch fields of their own, and run
+  before the first part, so an output that aliases u0 is written last.  The
+  results equal those of one context over all fields (each field's output
+  depends on the base fields and itself only).  Cost: one RHS evaluation per
+  part.  No reference counterpart (the reference carries any number of
+  scalars in one pass).
+  """
+
+  def __init__(self, cfg, device=None, create=True):
+    names = list(_get(cfg, 'conservative_variable_names'))
+    if names[:5] != list(types.BASE_CONSERVATIVE):
+      raise ValueError(
+          f'conservative_variable_names must start with {list(types.BASE_CONSERVATIVE)}')
+    scalars = names[5:]
+    self.groups = [scalars[i:i + 2] for i in range(0, len(scalars), 2)]
+    self.parts = [
+        NativeContext(_part_cfg(cfg, names[:5] + g, [s for s in scalars if s not in g]),
+                      device=device, create=create) for g in self.groups]
+    p0 = self.parts[0]
+    self.names = names
+    self.nf = len(names)
+    self.hw, self.n, self.shape, self.h = p0.hw, p0.n, p0.shape, p0.h
+    self.prim_bc_mask = 0
+    for c, part in enumerate(self.parts):
+      for k in range(part.nf):
+        if (part.prim_bc_mask >> k) & 1:
+          self.prim_bc_mask |= 1 << (k if k < 5 else 5 + 2 * c + (k - 5))
+    self._sink = None
+    if create:
+      self._make_sink()
+
+  def _make_sink(self):
+    if self._sink is None and len(self.parts) > 1:
+      self._sink = [torch.empty(self.shape, dtype=torch.float32, device=self.device)
+                    for _ in range(5)]
+
+  @property
+  def device(self):
+    return self.parts[0].device
+
+  @device.setter
+  def device(self, dev):
+    for part in self.parts:
+      part.device = dev
+
+  @property
+  def _pending_planes(self):
+    return [p for part in self.parts for p in part._pending_planes]
+
+  fields = NativeContext.fields
+
+  def create(self):
+    for part in self.parts:
+      part.create()
+    self._make_sink()
+
+  def cache_key(self):
+    digest = hashlib.sha256(b'chunked')
+    for part in self.parts:
+      digest.update(part.cache_key())
+    return digest.digest()
+
+  def close(self):
+    for part in self.parts:
+      part.close()
+
+  def _split(self, fields, c, sink=False):
+    """Part c's share of a field list (base fields replaced by the scratch
+    fields when `sink`)."""
+    if fields is None:
+      return None
+    base = list(fields[:5])
+    if sink and c > 0:
+      base = self._sink
+    return base + list(fields[5 + 2 * c:7 + 2 * c])
+
+  def _order(self):
+    return range(len(self.parts) - 1, -1, -1)
+
+  def fill_halos(self, q, stream=None):
+    for c in self._order():
+      self.parts[c].fill_halos(self._split(q, c), stream)
+
+  def rhs(self, q, r, stream=None):
+    for c in self._order():
+      self.parts[c].rhs(self._split(q, c), self._split(r, c, sink=True), stream)
+
+  def rk_stage(self, form, c0, c1, dt_dev, u0, uk, out, stream=None):
+    for c in self._order():
+      self.parts[c].rk_stage(form, c0, c1, dt_dev, self._split(u0, c), self._split(uk, c),
+                             self._split(out, c, sink=True), stream)
+
+  def max_wave_speed(self, q, out_dev, cfl=0.0, dt_dev=None, stream=None):
+    self.parts[0].max_wave_speed(self._split(q, 0), out_dev, cfl, dt_dev, stream)
+
+  def hllc_branch_codes(self, q, dim, codes, stream=None):
+    self.parts[0].hllc_branch_codes(self._split(q, 0), dim, codes, stream)
+
+  def set_reference_state(self, p_ref, rho_ref):
+    for part in self.parts:
+      part.set_reference_state(p_ref, rho_ref)
+
+  def get_tuning(self):
+    return self.parts[0].get_tuning()
+
+  def set_tuning(self, tuning):
+    for part in self.parts:
+      part.set_tuning(tuning)
+
+  def _unsupported(self, *args, **kwargs):
+    raise NotImplementedError('single-kernel timing and the adjoint take at most two'
+                              ' transported scalars')
+
+  sweep_only = sweep_only_timed = enable_adjoint = rhs_vjp = rk_stage_vjp = _unsupported
+
+
+def make_context(cfg, device=None, create=True):
+  """The native context for cfg: a NativeContext, or a ChunkedContext when
+  the state carries more than two transported scalars."""
+  if len(_get(cfg, 'conservative_variable_names') or []) > 7:
+    return ChunkedContext(cfg, device=device, create=create)
+  return NativeContext(cfg, device=device, create=create)
+
+
 _CACHE_SIZE = 8
 _cache = collections.OrderedDict()
 
 
 def cached_context(cfg, device=None):
-  """A NativeContext for cfg, reused across calls with an equal configuration
-  (grid, schemes, BC tables and values, device): the stand-alone seams call
-  this so repeated calls make no device allocations.  The least recently used
-  of more than 8 contexts is released."""
-  ctx = NativeContext(cfg, device=device, create=False)
+  """A native context (make_context) for cfg, reused across calls with an
+  equal configuration (grid, schemes, BC tables and values, device): the
+  stand-alone seams call this so repeated calls make no device allocations.
+  The least recently used of more than 8 contexts is released."""
+  ctx = make_context(cfg, device=device, create=False)
   if ctx.device.type == 'cuda' and ctx.device.index is None:
     ctx.device = torch.device('cuda', torch.cuda.current_device())
   key = ctx.cache_key()

@@ -69,14 +69,21 @@ def numerical_flux(scheme, conservative_neg, conservative_pos, primitive_neg, pr
     for t in (pdn, pdp):
       _check_field(t, ctx, 'p_dyn')
   out = {n: torch.empty(ctx.shape, dtype=torch.float32, device=ctx.device) for n in names}
-  N.check(N.lib().sc_riemann_flux(
-      ctx._ctx, _DIMS[dim],
-      _ptrs([conservative_neg[n] for n in names], N.SC_MAX_FIELDS, ctx, 'conservative_neg'),
-      _ptrs([conservative_pos[n] for n in names], N.SC_MAX_FIELDS, ctx, 'conservative_pos'),
-      _ptrs([primitive_neg[n] for n in prim_names], N.SC_MAX_FIELDS + 2, ctx, 'primitive_neg'),
-      _ptrs([primitive_pos[n] for n in prim_names], N.SC_MAX_FIELDS + 2, ctx, 'primitive_pos'),
-      ctypes.c_void_p(pdn.data_ptr()) if pdn is not None else None,
-      ctypes.c_void_p(pdp.data_ptr()) if pdp is not None else None,
-      _ptrs([out[n] for n in names], N.SC_MAX_FIELDS, ctx, 'flux'), N.stream_handle()),
-          'sc_riemann_flux')
+  # More than two scalars: one call per part of the context (five base
+  # fields and up to two scalars), the base fluxes kept from the first.
+  for c, part in enumerate(getattr(ctx, 'parts', [ctx])):
+    pn = part.names
+    pp = list(_BASE_PRIM) + [types.conservative_to_primitive_name(n) for n in pn[5:]]
+    dst = dict(out) if c == 0 else {**{n: torch.empty_like(out[n]) for n in pn[:5]},
+                                    **{n: out[n] for n in pn[5:]}}
+    N.check(N.lib().sc_riemann_flux(
+        part._ctx, _DIMS[dim],
+        _ptrs([conservative_neg[n] for n in pn], N.SC_MAX_FIELDS, ctx, 'conservative_neg'),
+        _ptrs([conservative_pos[n] for n in pn], N.SC_MAX_FIELDS, ctx, 'conservative_pos'),
+        _ptrs([primitive_neg[n] for n in pp], N.SC_MAX_FIELDS + 2, ctx, 'primitive_neg'),
+        _ptrs([primitive_pos[n] for n in pp], N.SC_MAX_FIELDS + 2, ctx, 'primitive_pos'),
+        ctypes.c_void_p(pdn.data_ptr()) if pdn is not None else None,
+        ctypes.c_void_p(pdp.data_ptr()) if pdp is not None else None,
+        _ptrs([dst[n] for n in pn], N.SC_MAX_FIELDS, ctx, 'flux'), N.stream_handle()),
+            'sc_riemann_flux')
   return out

@@ -99,6 +99,51 @@ def test_face_values_match_oracle(scheme, flux):
   _check(cfg, 32, f'faces {scheme}/{flux}')
 
 
+def _planes(spec, n, h, seed):
+  """spec with every DIRICHLET value v replaced by a plane over the face,
+  v * (1 + 0.1 U(-1, 1)) + 0.05 U(-1, 1), [A][B] over the full extents of the
+  two other axes: x faces [nz+2h][ny+2h], y faces [nz+2h][nx+2h], z faces
+  [nx+2h][ny+2h] (the slice shape of a [z][x][y] field)."""
+  rng = np.random.default_rng(seed)
+  nx, ny, nz = (m + 2 * h for m in n)
+  shape = {'x': (nz, ny), 'y': (nz, nx), 'z': (nx, ny)}
+  out = {}
+  for var, dims in spec.items():
+    out[var] = {}
+    for dim, sides in dims.items():
+      out[var][dim] = {}
+      for side, entry in sides.items():
+        if entry[0] is None:
+          out[var][dim][side] = entry
+          continue
+        v = float(entry[1])
+        plane = (v * (1 + 0.1 * rng.uniform(-1, 1, shape[dim]))
+                 + 0.05 * rng.uniform(-1, 1, shape[dim]))
+        out[var][dim][side] = (entry[0], plane.astype(np.float32))
+  return out
+
+
+def _plane_fluxes(n, h, seed):
+  return {t: _planes(s, n, h, seed + k) for k, (t, s) in enumerate(_flux_spec().items())}
+
+
+@pytest.mark.parametrize('scheme,flux,visc', [('WENO_5', 'HLLC', True), ('MUSCL', 'HLL', False)])
+def test_plane_flux_values_match_oracle(scheme, flux, visc):
+  n = (12, 10, 9)
+  cfg = _cfg(n, scheme, flux, visc, fluxes=_plane_fluxes(n, 3, 40))
+  _check(cfg, 33, f'plane fluxes {scheme}/{flux}/visc={visc}')
+
+
+@pytest.mark.parametrize('scheme,flux', [('WENO_5', 'HLLC'), ('WENO_3', 'HLL')])
+def test_plane_face_values_match_oracle(scheme, flux):
+  """Face planes of a conserved (rho_u, rho_e) and primitive (v, p) value,
+  mixed with scalar flux values."""
+  n = (12, 10, 9)
+  faces = _planes(_face_spec(), n, 3, 50)
+  cfg = _cfg(n, scheme, flux, False, faces=faces, fluxes=_flux_spec())
+  _check(cfg, 34, f'plane faces {scheme}/{flux}')
+
+
 def test_face_value_errors_follow_the_reference():
   faces = _face_spec()
   with pytest.raises(NotImplementedError):

@@ -113,11 +113,13 @@ def test_exchange_halos_matches_oracle_bitwise(n, h, case):
   np.testing.assert_array_equal(t.cpu().numpy(), u)  # input unchanged
 
 
+@pytest.mark.parametrize('planes', [False, True])
 @pytest.mark.parametrize('flux_type', ['convective', 'diffusive', 'total', 'none'])
-def test_update_fluxes_matches_oracle_bitwise(flux_type):
+def test_update_fluxes_matches_oracle_bitwise(flux_type, planes):
   import test_gpu_boundary_values as BV
   n, h = (12, 10, 9), 3
-  cfg = BV._cfg(n, 'WENO_5', 'HLLC', True, fluxes=BV._flux_spec())
+  spec = BV._plane_fluxes(n, h, 70) if planes else BV._flux_spec()
+  cfg = BV._cfg(n, 'WENO_5', 'HLLC', True, fluxes=spec)
   ocfg = oracle_cfg(cfg)
   rng = np.random.default_rng(11)
   shape = (n[2] + 2 * h, n[0] + 2 * h, n[1] + 2 * h)
@@ -133,17 +135,19 @@ def test_update_fluxes_matches_oracle_bitwise(flux_type):
       np.testing.assert_array_equal(g[d][v].cpu().numpy(), fl[d][v])  # input unchanged
 
 
-@pytest.mark.parametrize('case', ['no_rho', 'rho', 'rho_then_p'])
+@pytest.mark.parametrize('case', ['no_rho', 'rho', 'rho_then_p', 'planes', 'rho_planes'])
 def test_update_faces_matches_oracle_bitwise(case):
   import test_gpu_boundary_values as BV
   import test_gpu_riemann as RI
   n, h = (12, 10, 9), 3
   spec = dict(BV._face_spec())
-  if case != 'no_rho':
+  if case not in ('no_rho', 'planes'):
     spec['rho'] = {'x': {0: (BV.D, 1.1), 1: BV.NONE}, 'y': {0: BV.NONE, 1: BV.NONE},
                    'z': {0: BV.NONE, 1: (BV.D, 0.9)}}
-  if case == 'rho':
+  if case in ('rho', 'rho_planes'):
     del spec['p']
+  if case.endswith('planes'):
+    spec = BV._planes(spec, n, h, 71)
   cfg = BV._cfg(n, 'WENO_5', 'HLLC', False, faces=spec)
   ocfg = oracle_cfg(cfg)
   st = random_state(n, h, seed=12)

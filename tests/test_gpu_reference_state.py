@@ -26,12 +26,18 @@ NAMES = ('rho', 'rho_u', 'rho_v', 'rho_w', 'rho_e')
 TOL = 1e-5
 
 
-def _cfg(n, scheme, flux, visc):
+def _cfg(n, scheme, flux, visc, nscalar=0):
   cfg = grid_cfg(n, 3, (1.0, 1.0, 0.05))
+  names = list(NAMES) + [f'rho_y{s}' for s in range(nscalar)]
   wall = {'rho': 1.0, 'rho_u': 0.0, 'rho_v': 0.0, 'rho_w': 0.0, 'rho_e': 6.0}
+  wall.update({f'rho_y{s}': 0.3 + 0.2 * s for s in range(nscalar)})
   cells = {k: {'x': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
                'y': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
-               'z': {0: (BC.DIRICHLET, wall[k]), 1: (BC.NEUMANN, 0.0)}} for k in NAMES}
+               'z': {0: (BC.DIRICHLET, wall[k]), 1: (BC.NEUMANN, 0.0)}} for k in names}
+  if nscalar:
+    cfg.update({'conservative_variable_names': names,
+                'primitive_variable_names': ['rho', 'u', 'v', 'w', 'e'] +
+                                            [f'y{s}' for s in range(nscalar)]})
   cfg.update({'interpolation_scheme': scheme, 'numeric_flux_scheme': flux,
               'include_diffusion': visc, 'transport_parameters': {'nu': 2e-3, 'pr': 0.7},
               'dt': 2e-3, 'g': {'x': 0.0, 'y': 0.0, 'z': -1.0},
@@ -39,12 +45,12 @@ def _cfg(n, scheme, flux, visc):
   return cfg
 
 
-def _setup(n, scheme, flux, visc, seed, frac):
+def _setup(n, scheme, flux, visc, seed, frac, nscalar=0):
   h = 3
-  cfg = _cfg(n, scheme, flux, visc)
+  cfg = _cfg(n, scheme, flux, visc, nscalar)
   ocfg = oracle_cfg(cfg)
   pe = O.gravitational_potential(ocfg, np.float64)
-  st = random_state(n, h, seed=seed, jump=False, pe=pe)
+  st = random_state(n, h, seed=seed, jump=False, pe=pe, nscalar=nscalar)
   prim = O.conservative_to_primitive({k: v.astype(np.float64) for k, v in st.items()})
   p = O.pressure(prim, pe)
   # Reference fields: a fraction of the smooth part of the state (the
@@ -56,7 +62,7 @@ def _setup(n, scheme, flux, visc, seed, frac):
 
 def _compare(got, ref, h, tag):
   errs = {}
-  for k in NAMES:
+  for k in ref:
     a, b = interior(got[k], h), interior(ref[k], h)
     assert np.array_equal(np.isnan(a), np.isnan(b)), k
     fin = ~np.isnan(b)
@@ -95,3 +101,22 @@ def test_rk3_step_with_reference_state_matches_oracle():
                       gh, c, None, fn)
   torch.cuda.synchronize()
   _compare({k: v.cpu().numpy() for k, v in out.items()}, ref, h, 'rk3')
+
+
+@pytest.mark.parametrize('nscalar,scheme,flux,visc', [(1, 'WENO_5', 'HLLC', True),
+                                                      (2, 'MUSCL', 'HLL', False),
+                                                      (2, 'WENO_3', 'HLLC', False)])
+def test_reference_state_with_scalars_matches_oracle(nscalar, scheme, flux, visc):
+  """Transported scalars next to p_dyn: the HLLC star states carry the
+  scalars with the dynamic-pressure contact speed (hllc.py:109-120,182-192)."""
+  n, h = (12, 10, 9), 3
+  cfg, ocfg, st, helper = _setup(n, scheme, flux, visc, 43, 0.8, nscalar)
+  with np.errstate(all='ignore'):
+    ref = O.rhs(st, helper, ocfg)
+  assert len(ref) == 5 + nscalar
+  fn = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
+  gh = {k: torch.from_numpy(v).cuda() for k, v in helper.items()}
+  out = fn(0, {k: torch.from_numpy(v.copy()).cuda() for k, v in st.items()}, gh)
+  torch.cuda.synchronize()
+  _compare({k: v.cpu().numpy() for k, v in out.items()}, ref, h,
+           f'{nscalar} scalars {scheme}/{flux}')

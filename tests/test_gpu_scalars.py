@@ -103,3 +103,78 @@ def test_rk3_step_with_scalar_matches_oracle(nscalar):
   errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
   print(f'[scalar] rk3 ns={nscalar}: ' + ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
   assert max(errs.values()) <= TOL, errs
+
+
+MANY = [(3, 'WENO_5', 'HLLC', True, 'periodic'), (5, 'MUSCL', 'HLL', True, 'bounded'),
+        (4, 'WENO_3', 'HLLC', False, 'bounded')]
+
+
+@pytest.mark.parametrize('nscalar,scheme,flux,visc,bcs', MANY)
+def test_rhs_with_more_than_two_scalars_matches_oracle(nscalar, scheme, flux, visc, bcs):
+  """Scalars beyond two run in further parts of a chunked context (five base
+  fields plus two scalars each); every field matches the oracle."""
+  n, h = (13, 11, 9), 3
+  names = _names(nscalar)
+  cfg = _cfg(n, scheme, flux, visc, bcs, nscalar)
+  st = random_state(n, h, seed=14, nscalar=nscalar)
+  ref = O.rhs(st, {}, oracle_cfg(cfg))
+  fn = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
+  assert len(fn.native_context.parts) == (nscalar + 1) // 2
+  out = fn(0, _gpu(st), {})
+  torch.cuda.synchronize()
+  assert sorted(out) == sorted(names)
+  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
+  print(f'[scalar] {scheme}/{flux}/visc={visc}/{bcs}/ns={nscalar}: ' +
+        ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
+  assert max(errs.values()) <= TOL, errs
+
+
+def test_boundary_values_on_scalars_of_later_parts_match_oracle():
+  """Flux and face values given on scalars that sit in the second and third
+  parts (rho_y3 convective flux plane, y4 face value) and on the base fields."""
+  n, h, nscalar = (12, 10, 9), 3, 5
+  D, NONE = BC.DIRICHLET, (None, None)
+  cfg = _cfg(n, 'WENO_5', 'HLLC', False, 'bounded', nscalar)
+  rng = np.random.default_rng(5)
+  plane = rng.uniform(-0.2, 0.2, (n[2] + 2 * h, n[1] + 2 * h)).astype(np.float32)
+  cfg['bc']['intercell_fluxes'] = {'convective': {
+      'rho_y3': {'x': {0: (D, plane), 1: (D, 0.05)}, 'y': {0: NONE, 1: NONE},
+                 'z': {0: NONE, 1: NONE}},
+      'rho': {'x': {0: NONE, 1: NONE}, 'y': {0: NONE, 1: NONE},
+              'z': {0: (D, 0.1), 1: NONE}}}}
+  cfg['bc']['cell_faces'] = {
+      'y4': {'x': {0: NONE, 1: NONE}, 'y': {0: NONE, 1: NONE}, 'z': {0: (D, 0.7), 1: NONE}},
+      'rho_y0': {'x': {0: NONE, 1: (D, 0.2)}, 'y': {0: NONE, 1: NONE},
+                 'z': {0: NONE, 1: NONE}}}
+  st = random_state(n, h, seed=15, nscalar=nscalar)
+  ref = O.rhs(st, {}, oracle_cfg(cfg))
+  fn = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg)))
+  out = fn(0, _gpu(st), {})
+  torch.cuda.synchronize()
+  names = _names(nscalar)
+  errs = {k: rel_linf(interior(out[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
+  print('[scalar] boundary values, ns=5: ' +
+        ' '.join(f'{k}={v:.1e}' for k, v in errs.items()), flush=True)
+  assert max(errs.values()) <= TOL, errs
+
+
+def test_rk3_steps_with_five_scalars_in_place_match_oracle():
+  """Solver steps (graph-replayed, the last stage writing over u0) with five
+  scalars, against oracle SSP-RK3 steps."""
+  from swirlc_amd.core.solver import Solver
+  n, h, nscalar = (13, 11, 9), 3, 5
+  names = _names(nscalar)
+  cfg = _cfg(n, 'WENO_5', 'HLLC', True, 'periodic', nscalar)
+  st = random_state(n, h, seed=16, nscalar=nscalar)
+  ocfg = oracle_cfg(cfg)
+  ref = st
+  for _ in range(2):
+    ref = O.rk3(ref, {}, ocfg, dt=np.float32(2e-3))
+  solver = Solver(SwirlCParameters(copy.deepcopy(cfg)), _gpu(st), use_graph=True)
+  solver.step(2)
+  torch.cuda.synchronize()
+  got = dict(zip(names, solver.u))
+  errs = {k: rel_linf(interior(got[k].cpu().numpy(), h), interior(ref[k], h)) for k in names}
+  print('[scalar] solver rk3 x2, ns=5: ' + ' '.join(f'{k}={v:.1e}' for k, v in errs.items()),
+        flush=True)
+  assert max(errs.values()) <= 2 * TOL, errs

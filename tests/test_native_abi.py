@@ -179,3 +179,21 @@ def test_python_config_validation_errors():
     NativeContext(SwirlCParameters({
         'conservative_variable_names': base + ['rho_a', 'rho_b', 'rho_c'],
         'primitive_variable_names': ['rho', 'u', 'v', 'w', 'e', 'a', 'b', 'c']}))
+
+
+def test_plane_values_change_the_context():
+  """Contexts built from equal configurations but different face planes are
+  different (the plane bytes are part of the cached context's key)."""
+  import test_gpu_boundary_values as BV
+  from swirlc_amd.core.parameter import SwirlCParameters
+  from swirlc_amd.equations.native_context import NativeContext
+  n = (8, 8, 8)
+  f1 = BV._planes(BV._face_spec(), n, 3, 60)
+  f2 = BV._planes(BV._face_spec(), n, 3, 61)
+  k1 = NativeContext(SwirlCParameters(BV._cfg(n, 'WENO_5', 'HLLC', False, faces=f1)),
+                     create=False).cache_key()
+  k1b = NativeContext(SwirlCParameters(BV._cfg(n, 'WENO_5', 'HLLC', False, faces=f1)),
+                      create=False).cache_key()
+  k2 = NativeContext(SwirlCParameters(BV._cfg(n, 'WENO_5', 'HLLC', False, faces=f2)),
+                     create=False).cache_key()
+  assert k1 == k1b and k1 != k2
