@@ -458,13 +458,22 @@ def parallel_rk3_step(pool, state, ocfg, dt, slabs, shm_in, shm_out, shape):
 
 
 def _cpu_model():
+  """The host CPU's model name with its family / model / stepping numbers and
+  clock from /proc/cpuinfo (a virtualised host may report a bare name)."""
+  info = {}
   try:
     for line in open('/proc/cpuinfo'):
-      if line.startswith('model name'):
-        return line.split(':', 1)[1].strip()
+      if ':' in line:
+        k, v = line.split(':', 1)
+        info.setdefault(k.strip(), v.strip())
+      elif info:
+        break
   except OSError:
-    pass
-  return ''
+    return ''
+  name = info.get('model name', '')
+  extra = [f'{k} {info[k]}' for k in ('cpu family', 'model', 'stepping', 'cpu MHz')
+           if k in info]
+  return f'{name} ({", ".join(extra)})' if extra else name
 
 
 def cpu_baseline(n, single_n, workers):

@@ -21,6 +21,17 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
     (boundary/boundary_test.py:61-341).
   * EOS, conversions, physical flux, HLL, HLLC, RHS, source, viscous, RK:
     the reference's closed-form tests (see the test files' docstrings).
+  * the p_ref / rho_ref helper path (update_helper_variables, the p_dyn swap
+    of hllc_flux, d_rho in the gravity source): a quiescent hydrostatic
+    column with p_ref = p, rho_ref = rho gives RHS == 0 (HLLC) and zero
+    momentum RHS (HLL); an isolated contact keeps its exact HLLC flux with
+    p_dyn; each fails on a planted wrong substitution
+    (tests/test_oracle_pins.py).
+  * max_wave_speed (no reference counterpart): closed forms max |u| + c on
+    uniform states with one extreme cell, with and without gravity.
+  * edge and corner halo cells: golden vectors of the prototype notebook's
+    sequential x, y, z fill (demo/swirl_c_prototype.ipynb cell 2,
+    tests/golden/notebook_halo.npz).
   * weno / muscl: swirl_lm is not vendored (pyproject.toml:31-32) and cannot
     be imported here, so these follow the in-repo prototype notebook
     (demo/swirl_c_prototype.ipynb cell 4, `weno5` and `muscl`), shifted from

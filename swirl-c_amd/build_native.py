@@ -32,6 +32,9 @@ CFLAGS = [
     '-ffp-contract=on',
     '-Wno-unused-command-line-argument',
 ]
+# Extra compiler flags (e.g. -D definitions of the kernels' compile-time
+# knobs), whitespace separated.
+CFLAGS += os.environ.get('SWIRLC_EXTRA_CFLAGS', '').split()
 
 
 def sources():

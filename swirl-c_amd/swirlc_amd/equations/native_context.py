@@ -310,7 +310,7 @@ class NativeContext:
       full = [self.shape[0], self.shape[1], self.shape[2]]
       full[[1, 2, 0][d]] = 1
       arr = np.broadcast_to(arr, full).reshape(a, b)
-    return np.ascontiguousarray(np.broadcast_to(arr, (a, b)), dtype=np.float32)
+    return np.array(np.broadcast_to(arr, (a, b)), dtype=np.float32)  # a writable copy
 
   def _fill_dim_bc(self, faces, dim_spec, d):
     types_ = [dim_spec[s][0] for s in (0, 1)]
