@@ -306,7 +306,8 @@ def bench_adjoint(n, steps, every):
   (fixed dt = the CFL-0.5 step of the initial state), with respect to all
   five initial conserved fields; RK states checkpointed every `every` steps
   and recomputed.  Reports forward and backward (adjoint incl. recompute)
-  times and their ratio."""
+  times and their ratio, and the same for the rollout that keeps every
+  stage state (`full_tape`: nothing recomputed)."""
   import numpy as np
   import torch
   from swirlc_amd import configs
@@ -348,13 +349,43 @@ def bench_adjoint(n, steps, every):
   J.backward()
   torch.cuda.synchronize()
   t_bwd = time.perf_counter() - t0
+  J_val = float(J.detach())
   finite = all(bool(torch.isfinite(t.grad).all()) for t in x)
   gnorm = float(sum((t.grad.double() ** 2).sum() for t in x) ** 0.5)
+  grads = [t.grad for t in x]
+  del out, J, x
+  # The same rollout keeping every stage state (every=0: no recomputation in
+  # the backward pass); run twice, the second timed (the first allocates the
+  # tape, which the allocator then keeps).
+  tape = None
+  try:
+    for rep in range(2):
+      xt = [t.clone().requires_grad_(True) for t in u0]
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      out = A.rollout(nctx, dt, steps, *xt, every=0)
+      torch.cuda.synchronize()
+      tf = time.perf_counter() - t0
+      Jt = A.kinetic_density_loss(out, rho_t, h)
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      Jt.backward()
+      torch.cuda.synchronize()
+      tb = time.perf_counter() - t0
+      del out
+    same = all(torch.equal(a.grad, b) for a, b in zip(xt, grads))
+    tape = {'forward_s': round(tf, 4), 'adjoint_s': round(tb, 4),
+            'adjoint_over_forward': round(tb / tf, 3), 'gradient_bitwise_equal': same}
+    del xt, Jt
+  except MemoryError as e:
+    tape = {'skipped': str(e)}
+  torch.cuda.empty_cache()
   return {'workload': f'kelvin_helmholtz_{n}^3_viscous_ssprk3_{steps}steps_ckpt{every}',
-          'dt': float(dt[0]), 'loss': float(J.detach()), 'grad_norm': gnorm, 'finite': finite,
+          'dt': float(dt[0]), 'loss': float(J_val), 'grad_norm': gnorm, 'finite': finite,
           'forward_s': round(t_fwd, 4), 'adjoint_s': round(t_bwd, 4),
           'adjoint_over_forward': round(t_bwd / t_fwd, 3),
           'forward_Mcell_updates_per_s': round(n ** 3 * steps / t_fwd / 1e6, 1),
+          'full_tape': tape,
           'tuning': {k: adj_tuning[k] for k in ('adj_seg', 'adj_by', 'adj_by_y')}}
 
 

@@ -366,12 +366,21 @@ sc_status sc_rk_stage_vjp(sc_ctx* ctx, int form, float c0, float c1,
  * per-instance time steps read from a device array, so a whole RK step of
  * the ensemble is a fixed launch sequence that can be captured in a HIP graph.
  * Instance i's field f occupies elements [offset, offset + (nx+2h)(ny+2h)(nz+2h))
- * of arena f, laid out like a single-grid field.  Restrictions: 5 fields, no
- * gravitational potential, no primitive-variable BCs, no per-layer planes. */
+ * of arena f, laid out like a single-grid field.  5 to 7 fields (up to two
+ * transported scalars); gravity (each instance's potential from its own
+ * coordinates), primitive-variable BCs and scalar face / flux boundary
+ * values are supported; per-layer Dirichlet planes and plane-valued face /
+ * flux values are not (a plane has the shape of one grid). */
 typedef struct sc_instance {
   int nx, ny, nz;
   double dx, dy, dz;
   long long offset;
+  /* Host cell-centre coordinates including halo cells (nx + 2h, ny + 2h,
+   * nz + 2h entries, as sc_grid's), read at creation; required when the
+   * parameters carry a gravity vector, else ignored. */
+  const float* x_full;
+  const float* y_full;
+  const float* z_full;
 } sc_instance;
 
 typedef struct sc_batch sc_batch;

@@ -1022,7 +1022,8 @@ sc_status sc_rk_stage_vjp(sc_ctx* c, int form, float c0, float c1, const float* 
 namespace sc {
 template <int SCHEME>
 hipError_t launch_batch_sweeps(const BatchArgs& b, const BatchTiles& tiles, int flux,
-                               bool visc, const SweepTuning& tune, hipStream_t stream);
+                               bool visc, int ns, bool full, const SweepTuning& tune,
+                               hipStream_t stream);
 }
 
 struct sc_batch {
@@ -1043,6 +1044,11 @@ struct sc_batch {
   unsigned* speed_bits = nullptr;
   float* r_arena[SC_MAX_FIELDS] = {};
   float* dz_arena[SC_MAX_FIELDS] = {};
+  float* pe_dev = nullptr;                 // potential tables of every instance
+  sc::BoundaryBc* bbc_dev = nullptr;       // boundary face / flux values
+  sc::BatchHaloTile* ptiles_dev = nullptr; // primitive-BC finish tiles
+  int nptiles = 0;
+  bool full = false;                       // gravity, primitive BCs or boundary values
   sc::HaloArgs halo_cfg;
   sc::SweepArgs proto;
   sc::SweepTuning tune;
@@ -1132,6 +1138,9 @@ sc_status sc_batch_destroy(sc_batch* b) {
   }
   hipFree(b->stiles_dev);
   hipFree(b->hmin_dev);
+  hipFree(b->pe_dev);
+  hipFree(b->bbc_dev);
+  hipFree(b->ptiles_dev);
   hipFree(b->speed_bits);
   for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->r_arena[f]);
   for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->dz_arena[f]);
@@ -1145,14 +1154,24 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     return fail(SC_ERR_INVALID, "sc_batch_create: null argument or no instances");
   *out = nullptr;
   const sc_params& p = *params;
-  if (p.n_fields != 5)
-    return fail(SC_ERR_UNSUPPORTED, "batched ensembles support the 5 base fields only");
-  if (gravity_direction(p.g) != -1)
-    return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support a gravity vector");
-  if (p.prim_bc_mask != 0)
-    return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support primitive-variable BCs");
-  if (p.n_boundary_values != 0)
-    return fail(SC_ERR_UNSUPPORTED, "batched ensembles do not support face or flux boundary values");
+  if (p.n_fields < 5 || p.n_fields > SC_MAX_FIELDS)
+    return fail(SC_ERR_UNSUPPORTED, "n_fields must be 5, 6 or 7 (at most two transported scalars)");
+  const int nf = p.n_fields;
+  const int gdir = gravity_direction(p.g);
+  if (p.gravity_source && gdir == -1)
+    return fail(SC_ERR_INVALID, "Gravity direction must be specified if the gravity source"
+                                " function is to be used.");
+  for (int k = 0; k < p.n_boundary_values && k < SC_MAX_BOUNDARY_VALUES; ++k)
+    if (p.boundary_values[k].plane != nullptr)
+      return fail(SC_ERR_UNSUPPORTED,
+                  "batched ensembles take scalar face / flux boundary values only (a plane"
+                  " has the shape of one grid)");
+  sc::BoundaryBc bbc[3][2];
+  int bbc_dims = 0;
+  {
+    const sc_status bst = build_boundary_tables(p, bbc, &bbc_dims);
+    if (bst != SC_OK) return bst;
+  }
   // Validate the shared configuration with a throwaway single-grid check of
   // every instance shape (schemes, halo width, BC types).
   long long end = 0;
@@ -1177,8 +1196,11 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     if (g.nx >= (1 << 20) || (long long)(g.nx + 2 * halo) * (g.ny + 2 * halo) >= (1LL << 30))
       return fail(SC_ERR_UNSUPPORTED, "batched instance too large: nx must be below 2^20 and "
                                       "(nx + 2 * halo_width) * (ny + 2 * halo_width) below 2^30");
+    if (gdir != -1 && (g.x_full == nullptr || g.y_full == nullptr || g.z_full == nullptr))
+      return fail(SC_ERR_INVALID, "a gravity vector needs every instance's x_full, y_full and"
+                                  " z_full coordinates");
     const int nn[3] = {g.nx, g.ny, g.nz};
-    for (int f = 0; f < 5; ++f)
+    for (int f = 0; f < nf; ++f)
       for (int d = 0; d < 3; ++d)
         for (int s = 0; s < 2; ++s) {
           const sc_bc_face& bf = p.bc[f][d][s];
@@ -1194,7 +1216,7 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     const long long cells = (long long)(g.nx + 2 * halo) * (g.ny + 2 * halo) * (g.nz + 2 * halo);
     const long long NN[3] = {g.nx + 2LL * halo, g.ny + 2LL * halo, g.nz + 2LL * halo};
     for (int d = 0; d < 3; ++d)
-      if (sc::halo_dim_threads(5, halo, NN, d) > 0x7fffffffLL)
+      if (sc::halo_dim_threads(nf, halo, NN, d) > 0x7fffffffLL)
         return fail(SC_ERR_UNSUPPORTED, "batched instance too large (halo cells of a dim >= 2^31)");
     if (g.offset < end)
       return fail(SC_ERR_INVALID, "instance offsets must be increasing and non-overlapping");
@@ -1204,14 +1226,19 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   sc_batch* b = new sc_batch();
   b->params = p;
   b->h = halo;
+  b->nf = nf;
   b->n_inst = n;
+  b->full = gdir != -1 || p.prim_bc_mask != 0 || bbc_dims != 0;
   b->total = end;
   b->inst_host.assign(inst, inst + n);
 
   std::vector<sc::BatchInst> bi(n);
   std::vector<sc::BatchHaloInst> hi(n);
   std::vector<float> hmin(n);
-  std::vector<sc::BatchHaloTile> ht[3], stl;
+  std::vector<sc::BatchHaloTile> ht[3], stl, ptl;
+  // Potential tables (gravity): per instance pe_x over NX, pe_y over NY,
+  // pe_z over NZ, tabulated as in sc_ctx_create (generic.py:312-319).
+  std::vector<float> pe_host;
   for (int i = 0; i < n; ++i) {
     const sc_instance& g = inst[i];
     const double hh[3] = {g.dx, g.dy, g.dz};
@@ -1220,6 +1247,17 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     e.ny = g.ny;
     e.nz = g.nz;
     e.off = g.offset;
+    e.pe_off = -1;
+    if (gdir != -1) {
+      e.pe_off = (long long)pe_host.size();
+      const float* coord[3] = {g.x_full, g.y_full, g.z_full};
+      const int NN3[3] = {g.nx + 2 * halo, g.ny + 2 * halo, g.nz + 2 * halo};
+      for (int d = 0; d < 3; ++d) {
+        const double gv = 9.81 * p.g[d];
+        const float coef = d == 0 ? (float)(-gv) : (float)gv;
+        for (int c = 0; c < NN3[d]; ++c) pe_host.push_back(coef * coord[d][c]);
+      }
+    }
     for (int d = 0; d < 3; ++d) {
       e.h[d] = (float)hh[d];
       e.inv_h[d] = 1.0f / (float)hh[d];
@@ -1229,14 +1267,19 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     hi[i].n[1] = g.ny;
     hi[i].n[2] = g.nz;
     hi[i].off = g.offset;
+    hi[i].pe_off = e.pe_off;
     float hm = e.h[0];
     if (e.h[1] < hm) hm = e.h[1];
     if (e.h[2] < hm) hm = e.h[2];
     hmin[i] = hm;
     const long long N[3] = {g.nx + 2LL * halo, g.ny + 2LL * halo, g.nz + 2LL * halo};
     for (int d = 0; d < 3; ++d) {
-      const long long threads = sc::halo_dim_threads(5, halo, N, d);
+      const long long threads = sc::halo_dim_threads(nf, halo, N, d);
       for (long long k = 0; k < (threads + 255) / 256; ++k) ht[d].push_back({i, (int)k});
+    }
+    if (p.prim_bc_mask != 0) {
+      const long long padded = N[0] * N[1] * N[2];
+      for (long long k = 0; k < (padded + 255) / 256; ++k) ptl.push_back({i, (int)k});
     }
     const long long cells = (long long)g.nx * g.ny * g.nz;
     for (long long k = 0; k < (cells + sc::kSpeedChunk - 1) / sc::kSpeedChunk; ++k)
@@ -1252,9 +1295,18 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   }
   if (st == SC_OK) st = upload(stl, &b->stiles_dev);
   b->nstiles = (int)stl.size();
+  if (st == SC_OK) st = upload(ptl, &b->ptiles_dev);
+  b->nptiles = (int)ptl.size();
+  if (st == SC_OK) st = upload(pe_host, &b->pe_dev);
+  if (st == SC_OK && bbc_dims != 0) {
+    st = hip_check(hipMalloc((void**)&b->bbc_dev, sizeof(bbc)), "hipMalloc boundary values");
+    if (st == SC_OK)
+      st = hip_check(hipMemcpy(b->bbc_dev, bbc, sizeof(bbc), hipMemcpyHostToDevice),
+                     "hipMemcpy boundary values");
+  }
   if (st == SC_OK)
     st = hip_check(hipMalloc((void**)&b->speed_bits, sizeof(unsigned) * n), "hipMalloc speed");
-  for (int f = 0; f < 5 && st == SC_OK; ++f) {
+  for (int f = 0; f < nf && st == SC_OK; ++f) {
     st = hip_check(hipMalloc((void**)&b->r_arena[f], sizeof(float) * b->total), "hipMalloc R");
     if (st == SC_OK)
       st = hip_check(hipMemset(b->r_arena[f], 0, sizeof(float) * b->total), "hipMemset R");
@@ -1270,9 +1322,10 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   // Shared halo configuration (BC tables).
   sc::HaloArgs& ha = b->halo_cfg;
   memset(&ha, 0, sizeof(ha));
-  ha.nf = 5;
+  ha.nf = nf;
   ha.h = halo;
-  for (int f = 0; f < 5; ++f)
+  ha.prim_mask = p.prim_bc_mask;
+  for (int f = 0; f < nf; ++f)
     for (int d = 0; d < 3; ++d)
       for (int s = 0; s < 2; ++s) {
         const sc_bc_face& bf = p.bc[f][d][s];
@@ -1287,9 +1340,14 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   sc::SweepArgs& a = b->proto;
   memset(&a, 0, sizeof(a));
   a.h = halo;
-  a.has_pe = 0;
-  a.grav_src = 0;
-  a.prim_bc_mask = 0;
+  a.has_pe = 0;  // per instance (tile_args)
+  a.grav_src = p.gravity_source ? 1 : 0;
+  a.grav_mask = 0;
+  if (p.gravity_source) a.grav_mask = gdir == 3 ? 7 : (1 << gdir);
+  for (int d = 0; d < 3; ++d) a.gdir[d] = (float)p.g[d];
+  a.prim_bc_mask = p.prim_bc_mask;
+  a.bbc = b->bbc_dev;
+  a.bbc_dims = bbc_dims;
   a.vc.nu = (float)p.nu;
   a.vc.kfac = (float)(1.0 / p.pr) * sc::kCp;
   *out = b;
@@ -1298,7 +1356,7 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
 
 sc_status sc_batch_fill_halos(sc_batch* b, float* const* q, void* stream) {
   if (b == nullptr || q == nullptr) return fail(SC_ERR_INVALID, "sc_batch_fill_halos: null argument");
-  for (int f = 0; f < 5; ++f)
+  for (int f = 0; f < b->nf; ++f)
     if (q[f] == nullptr) return fail(SC_ERR_INVALID, "sc_batch_fill_halos: null arena");
   for (int d = 0; d < 3; ++d) {
     sc_status st = hip_check(
@@ -1307,6 +1365,11 @@ sc_status sc_batch_fill_halos(sc_batch* b, float* const* q, void* stream) {
         "batch halo kernel");
     if (st != SC_OK) return st;
   }
+  if (b->halo_cfg.prim_mask != 0)
+    return hip_check(sc::launch_batch_halo_prim_finish(b->halo_cfg, q, b->hinst_dev,
+                                                       b->ptiles_dev, b->nptiles,
+                                                       grid_for(b->nptiles), (hipStream_t)stream),
+                     "batch halo prim kernel");
   return SC_OK;
 }
 
@@ -1319,7 +1382,7 @@ sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const flo
     return fail(SC_ERR_UNSUPPORTED, "sc_batch_rk_stage: form must be SC_RK_EULER or SC_RK_BLEND");
   if (form == SC_RK_BLEND && u0 == nullptr)
     return fail(SC_ERR_INVALID, "sc_batch_rk_stage: u0 required");
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < b->nf; ++f) {
     if (uk[f] == nullptr || out[f] == nullptr || (form == SC_RK_BLEND && u0[f] == nullptr))
       return fail(SC_ERR_INVALID, "sc_batch_rk_stage: null arena");
     if (out[f] == uk[f]) return fail(SC_ERR_INVALID, "sc_batch_rk_stage: out must not alias uk");
@@ -1332,8 +1395,9 @@ sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const flo
   ba.proto.rk_form = form;
   ba.proto.c0 = c0;
   ba.proto.c1 = c1;
-  ba.nf = 5;
-  for (int f = 0; f < 5; ++f) {
+  ba.nf = b->nf;
+  ba.pe = b->pe_dev;
+  for (int f = 0; f < b->nf; ++f) {
     ba.q.p[f] = uk[f];
     ba.u0.p[f] = form == SC_RK_BLEND ? u0[f] : nullptr;
     ba.r.p[f] = b->r_arena[f];
@@ -1346,12 +1410,13 @@ sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const flo
   const int flux = b->params.numeric_flux;
   const bool visc = b->params.include_diffusion != 0;
   hipStream_t s = (hipStream_t)stream;
+  const int ns = b->nf - 5;
   if (b->params.interpolation == SC_WENO_5) {
-    e = sc::launch_batch_sweeps<sc::kWeno5>(ba, b->tiles, flux, visc, b->tune, s);
+    e = sc::launch_batch_sweeps<sc::kWeno5>(ba, b->tiles, flux, visc, ns, b->full, b->tune, s);
   } else if (b->params.interpolation == SC_WENO_3) {
-    e = sc::launch_batch_sweeps<sc::kWeno3>(ba, b->tiles, flux, visc, b->tune, s);
+    e = sc::launch_batch_sweeps<sc::kWeno3>(ba, b->tiles, flux, visc, ns, b->full, b->tune, s);
   } else {
-    e = sc::launch_batch_sweeps<sc::kMuscl>(ba, b->tiles, flux, visc, b->tune, s);
+    e = sc::launch_batch_sweeps<sc::kMuscl>(ba, b->tiles, flux, visc, ns, b->full, b->tune, s);
   }
   return hip_check(e, "batch sweep kernels");
 }
@@ -1365,7 +1430,7 @@ sc_status sc_batch_max_wave_speed(sc_batch* b, const float* const* q, float cfl,
   return hip_check(sc::launch_batch_max_speed(qb, b->hinst_dev, b->h, b->stiles_dev, b->nstiles,
                                               grid_for(b->nstiles), b->n_inst,
                                               (unsigned*)speed_dev, cfl, b->hmin_dev, dt_dev,
-                                              (hipStream_t)stream),
+                                              b->pe_dev, (hipStream_t)stream),
                    "batch max speed kernel");
 }
 

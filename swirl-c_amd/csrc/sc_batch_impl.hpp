@@ -18,6 +18,7 @@ namespace sc {
 struct BatchInst {
   int nx, ny, nz;
   long long off;  // element offset of the instance in every arena
+  long long pe_off;  // its potential tables in BatchArgs::pe (x, y, z), or -1
   float inv_h[3];
   float h[3];
   float inv_two_h[3];
@@ -39,6 +40,7 @@ struct BatchArgs {
   const BatchTile* tiles;
   int ntiles;
   const float* dt;  // per-instance time steps
+  const float* pe;  // potential tables of every instance (gravity), or null
 };
 
 __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int dir) {
@@ -69,28 +71,39 @@ __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int d
     a.vc.inv_two_h[d] = in.inv_two_h[d];
   }
   a.dt = b.dt != nullptr ? b.dt + id : nullptr;
+  if (b.pe != nullptr && in.pe_off >= 0) {
+    a.has_pe = 1;
+    a.pe_x = b.pe + in.pe_off;
+    a.pe_y = a.pe_x + a.NX;
+    a.pe_z = a.pe_y + a.NY;
+  } else {
+    a.has_pe = 0;
+  }
   return a;
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE>
+// FULL: the build with gravity, primitive-variable BCs and boundary face /
+// flux values (decided at run time from the shared configuration); the other
+// build leaves that code out.
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool FULL>
 __global__ void __launch_bounds__(256) batch_march_kernel(const BatchArgs b) {
   for (int t = (int)blockIdx.x; t < b.ntiles; t += (int)gridDim.x) {
     const BatchTile tl = b.tiles[t];
     const SweepArgs a = tile_args(b, tl.inst, DIR);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, false, false, true>(a, ti);
+    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, false, FULL, !FULL>(a, ti);
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool FULL>
 __global__ void __launch_bounds__(256) batch_y_kernel(const BatchArgs b) {
   for (int t = (int)blockIdx.x; t < b.ntiles; t += (int)gridDim.x) {
     const BatchTile tl = b.tiles[t];
     const SweepArgs a = tile_args(b, tl.inst, 1);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_y_body<SCHEME, FLUX, VISC, NS, false, false, true>(a, ti, 1);
+    sweep_y_body<SCHEME, FLUX, VISC, NS, false, FULL, !FULL>(a, ti, 1);
   }
 }
 
@@ -101,7 +114,7 @@ struct BatchTiles {
   int grid[3];  // blocks launched per direction (one per tile)
 };
 
-template <int SCHEME, int FLUX, bool VISC, int NS>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool FULL>
 hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
                                  const SweepTuning& tune, hipStream_t stream) {
   {
@@ -109,7 +122,7 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
     bx.proto.seg_len = tune.seg_x;
     bx.tiles = tiles.t[0];
     bx.ntiles = tiles.n[0];
-    hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0>),
+    hipLaunchKernelGGL((batch_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, FULL>),
                        dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
   }
   {
@@ -117,33 +130,49 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
     bz.proto.seg_len = tune.seg_z;
     bz.tiles = tiles.t[2];
     bz.ntiles = tiles.n[2];
-    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 2>),
+    hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, FULL>),
                        dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
   }
   {
     BatchArgs by = b;
     by.tiles = tiles.t[1];
     by.ntiles = tiles.n[1];
-    hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS>), dim3(tiles.grid[1]),
+    hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS, FULL>), dim3(tiles.grid[1]),
                        dim3(64, tune.by_y), 0, stream, by);
   }
   return hipGetLastError();
 }
 
+template <int SCHEME, int FLUX, bool VISC>
+hipError_t launch_batch_variant(const BatchArgs& b, const BatchTiles& tiles, int ns, bool full,
+                                const SweepTuning& tune, hipStream_t stream) {
+  if (ns == 0)
+    return full ? launch_batch_sweeps_t<SCHEME, FLUX, VISC, 0, true>(b, tiles, tune, stream)
+                : launch_batch_sweeps_t<SCHEME, FLUX, VISC, 0, false>(b, tiles, tune, stream);
+  if (ns == 1)
+    return full ? launch_batch_sweeps_t<SCHEME, FLUX, VISC, 1, true>(b, tiles, tune, stream)
+                : launch_batch_sweeps_t<SCHEME, FLUX, VISC, 1, false>(b, tiles, tune, stream);
+  return full ? launch_batch_sweeps_t<SCHEME, FLUX, VISC, 2, true>(b, tiles, tune, stream)
+              : launch_batch_sweeps_t<SCHEME, FLUX, VISC, 2, false>(b, tiles, tune, stream);
+}
+
+// ns: transported scalars (0..2); full: the configuration has gravity,
+// primitive-variable BCs or boundary face / flux values.
 template <int SCHEME>
 hipError_t launch_batch_sweeps(const BatchArgs& b, const BatchTiles& tiles, int flux,
-                               bool visc, const SweepTuning& tune, hipStream_t stream);
+                               bool visc, int ns, bool full, const SweepTuning& tune,
+                               hipStream_t stream);
 
-#define SC_INSTANTIATE_BATCH(SCHEME)                                                     \
-  template <>                                                                            \
-  hipError_t launch_batch_sweeps<SCHEME>(const BatchArgs& b, const BatchTiles& tiles,    \
-                                         int flux, bool visc, const SweepTuning& tune,   \
-                                         hipStream_t stream) {                           \
-    if (flux == kHllc)                                                                   \
-      return visc ? launch_batch_sweeps_t<SCHEME, kHllc, true, 0>(b, tiles, tune, stream) \
-                  : launch_batch_sweeps_t<SCHEME, kHllc, false, 0>(b, tiles, tune, stream); \
-    return visc ? launch_batch_sweeps_t<SCHEME, kHll, true, 0>(b, tiles, tune, stream)     \
-                : launch_batch_sweeps_t<SCHEME, kHll, false, 0>(b, tiles, tune, stream);   \
+#define SC_INSTANTIATE_BATCH(SCHEME)                                                      \
+  template <>                                                                             \
+  hipError_t launch_batch_sweeps<SCHEME>(const BatchArgs& b, const BatchTiles& tiles,     \
+                                         int flux, bool visc, int ns, bool full,          \
+                                         const SweepTuning& tune, hipStream_t stream) {   \
+    if (flux == kHllc)                                                                    \
+      return visc ? launch_batch_variant<SCHEME, kHllc, true>(b, tiles, ns, full, tune, stream) \
+                  : launch_batch_variant<SCHEME, kHllc, false>(b, tiles, ns, full, tune, stream); \
+    return visc ? launch_batch_variant<SCHEME, kHll, true>(b, tiles, ns, full, tune, stream)     \
+                : launch_batch_variant<SCHEME, kHll, false>(b, tiles, ns, full, tune, stream);   \
   }
 
 }  // namespace sc

@@ -319,7 +319,8 @@ __global__ void batch_speed_init_kernel(unsigned* bits, int n) {
 
 __global__ void __launch_bounds__(256)
 batch_max_speed_kernel(CFieldPtrs q_base, const BatchHaloInst* inst, int h,
-                       const BatchHaloTile* tiles, int ntiles, unsigned* bits) {
+                       const BatchHaloTile* tiles, int ntiles, unsigned* bits,
+                       const float* pe_arena) {
   __shared__ float part[4];
   for (int tt = (int)blockIdx.x; tt < ntiles; tt += (int)gridDim.x) {
     const BatchHaloTile tl = tiles[tt];
@@ -337,7 +338,12 @@ batch_max_speed_kernel(CFieldPtrs q_base, const BatchHaloInst* inst, int h,
       float q[5], w[7];
 #pragma unroll
       for (int f = 0; f < 5; ++f) q[f] = q_base.p[f][idx];
-      cons_to_prim<0>(q, 0.0f, w);
+      float pe = 0.0f;
+      if (pe_arena != nullptr && in.pe_off >= 0) {
+        const float* px = pe_arena + in.pe_off;
+        pe = (px[i] - px[NX + j]) - px[NX + NY + k];
+      }
+      cons_to_prim<0>(q, pe, w);
       const float c = sqrtf(kGamma * w[kP] / w[kRho]);
       const float s = fmaxf(fmaxf(fabsf(w[kU]) + c, fabsf(w[kV]) + c), fabsf(w[kW]) + c);
       m = fmaxf(m, s);
@@ -415,16 +421,51 @@ hipError_t launch_batch_halo_dim(const HaloArgs& cfg, float* const* q_base,
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256)
+batch_halo_prim_finish_kernel(const HaloArgs a, CFieldPtrs q_base, const BatchHaloInst* inst,
+                              const BatchHaloTile* tiles, int ntiles) {
+  for (int tt = (int)blockIdx.x; tt < ntiles; tt += (int)gridDim.x) {
+    const BatchHaloTile tl = tiles[tt];
+    const BatchHaloInst in = inst[tl.inst];
+    const int h = a.h;
+    const long long NX = in.n[0] + 2 * h, NY = in.n[1] + 2 * h, NZ = in.n[2] + 2 * h;
+    const long long t = (long long)tl.blk * blockDim.x + threadIdx.x;
+    if (t >= NX * NY * NZ) continue;
+    const int y = (int)(t % NY);
+    const int x = (int)((t / NY) % NX);
+    const int z = (int)(t / (NY * NX));
+    const bool halo = x < h || x >= h + in.n[0] || y < h || y >= h + in.n[1] ||
+                      z < h || z >= h + in.n[2];
+    if (!halo) continue;
+    const long long idx = in.off + t;
+    const float r = q_base.p[0][idx];
+    for (int f = 1; f < a.nf; ++f) {
+      if ((a.prim_mask >> f) & 1) q_base.p[f][idx] = r * q_base.p[f][idx];
+    }
+  }
+}
+
+hipError_t launch_batch_halo_prim_finish(const HaloArgs& cfg, float* const* q_base,
+                                         const BatchHaloInst* inst, const BatchHaloTile* tiles,
+                                         int ntiles, int grid, hipStream_t stream) {
+  CFieldPtrs qb;
+  for (int f = 0; f < kMaxFields; ++f) qb.p[f] = f < cfg.nf ? q_base[f] : nullptr;
+  hipLaunchKernelGGL(batch_halo_prim_finish_kernel, dim3(grid), 256, 0, stream, cfg, qb, inst,
+                     tiles, ntiles);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_max_speed(float* const* q_base, const BatchHaloInst* inst, int h,
                                   const BatchHaloTile* tiles, int ntiles, int grid,
                                   int n_inst, unsigned* bits, float cfl,
-                                  const float* hmin, float* dt, hipStream_t stream) {
+                                  const float* hmin, float* dt, const float* pe,
+                                  hipStream_t stream) {
   CFieldPtrs qb;
   for (int f = 0; f < kMaxFields; ++f) qb.p[f] = f < 5 ? q_base[f] : nullptr;
   hipLaunchKernelGGL(batch_speed_init_kernel, dim3((n_inst + 255) / 256), 256, 0, stream,
                      bits, n_inst);
   hipLaunchKernelGGL(batch_max_speed_kernel, dim3(grid), 256, 0, stream, qb, inst, h, tiles,
-                     ntiles, bits);
+                     ntiles, bits, pe);
   if (dt != nullptr) {
     hipLaunchKernelGGL(batch_cfl_dt_kernel, dim3((n_inst + 255) / 256), 256, 0, stream, bits,
                        n_inst, cfl, hmin, dt);

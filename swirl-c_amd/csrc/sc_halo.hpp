@@ -54,6 +54,9 @@ hipError_t launch_halo_periodic(const HaloArgs& a, hipStream_t stream);
 struct BatchHaloInst {
   int n[3];
   long long off;
+  // Offset of the instance's potential tables in the batch's potential
+  // arena (pe_x over NX, then pe_y over NY, then pe_z over NZ), or -1.
+  long long pe_off;
 };
 struct BatchHaloTile {
   int inst, blk;
@@ -65,10 +68,18 @@ hipError_t launch_batch_halo_dim(const HaloArgs& cfg, float* const* q_base,
 // Batched wave speed: tiles (instance, chunk of kSpeedChunk interior cells);
 // bits[i] = max speed of instance i, then dt[i] = cfl * hmin[i] / max.
 constexpr int kSpeedChunk = 4096;
+// pe: the batch's potential arena (instances with pe_off >= 0), or null.
 hipError_t launch_batch_max_speed(float* const* q_base, const BatchHaloInst* inst, int h,
                                   const BatchHaloTile* tiles, int ntiles, int grid,
                                   int n_inst, unsigned* bits, float cfl,
-                                  const float* hmin, float* dt, hipStream_t stream);
+                                  const float* hmin, float* dt, const float* pe,
+                                  hipStream_t stream);
+// Halo cells of fields with a primitive BC hold the primitive after the
+// per-dim fills; multiply them by the halo density, instance by instance
+// (tiles: one per 256 padded cells of an instance).
+hipError_t launch_batch_halo_prim_finish(const HaloArgs& cfg, float* const* q_base,
+                                         const BatchHaloInst* inst, const BatchHaloTile* tiles,
+                                         int ntiles, int grid, hipStream_t stream);
 
 struct SpeedArgs {
   const float* q[5];

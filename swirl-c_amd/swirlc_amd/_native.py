@@ -92,6 +92,8 @@ class ScInstance(ctypes.Structure):
       ('nx', ctypes.c_int), ('ny', ctypes.c_int), ('nz', ctypes.c_int),
       ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('dz', ctypes.c_double),
       ('offset', ctypes.c_longlong),
+      ('x_full', ctypes.POINTER(ctypes.c_float)), ('y_full', ctypes.POINTER(ctypes.c_float)),
+      ('z_full', ctypes.POINTER(ctypes.c_float)),
   ]
 
 

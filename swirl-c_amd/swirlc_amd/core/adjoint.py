@@ -13,7 +13,10 @@ its backward pass recomputes each k-step segment from the segment's
 checkpoint, storing the three stage states of every step of the segment, and
 then runs the stage adjoints in reverse.  Peak memory is about
 (steps / k + 3 k) states instead of 3 * steps; the recomputation costs one
-forward rollout.  dt is a constant of the rollout (no derivative through a
+forward rollout.  With k = 0 the forward pass keeps the three stage states
+of every step in one device allocation (3 * steps states: 29 GB for 200
+steps of a 128^3 grid with halos) and the backward pass recomputes
+nothing.  dt is a constant of the rollout (no derivative through a
 CFL time step), as cfg.dt is in the reference.
 
 Fields are full-size [nz+2h, nx+2h, ny+2h] fp32 GPU tensors; gradients are
@@ -89,8 +92,14 @@ class RolloutFunction(torch.autograd.Function):
 
   @staticmethod
   def forward(ctx, nctx, dt_dev, steps, every, *u0):
-    if steps < 1 or every < 1:
-      raise ValueError('steps and the checkpoint interval must be positive')
+    if steps < 1 or every < 0:
+      raise ValueError('steps must be positive and the checkpoint interval non-negative')
+    ctx.nctx = nctx
+    ctx.dt = dt_dev
+    ctx.steps = steps
+    ctx.every = every
+    if every == 0:
+      return _tape_forward(ctx, nctx, dt_dev, steps, u0)
     u = [t.detach().clone() for t in u0]
     s1 = [torch.empty_like(t) for t in u]
     s2 = [torch.empty_like(t) for t in u]
@@ -99,16 +108,14 @@ class RolloutFunction(torch.autograd.Function):
       rk3_forward(nctx, dt_dev, u, s1, s2, u)
       if (n + 1) % every == 0 and n + 1 < steps:
         ckpts[n + 1] = [t.clone() for t in u]
-    ctx.nctx = nctx
-    ctx.dt = dt_dev
-    ctx.steps = steps
-    ctx.every = every
     ctx.ckpts = ckpts
     return tuple(u)
 
   @staticmethod
   def backward(ctx, *grads):
     nctx, dt_dev, steps, every = ctx.nctx, ctx.dt, ctx.steps, ctx.every
+    if every == 0:
+      return _tape_backward(ctx, grads)
     base = ctx.ckpts[0]
     lam = _grads(nctx, grads, base)
     lam0 = [torch.zeros_like(t) for t in base]
@@ -133,6 +140,57 @@ class RolloutFunction(torch.autograd.Function):
     return (None, None, None, None, *lam)
 
 
+class _Tape:
+  """Stage states u_n, s1, s2 of every step of a rollout, in one device
+  allocation; each field starts on a 256-byte boundary."""
+
+  def __init__(self, steps, like):
+    self.nf = len(like)
+    self.shape = like[0].shape
+    self.numel = like[0].numel()
+    self.stride = (self.numel + 63) // 64 * 64
+    dev = like[0].device
+    need = 3 * steps * self.nf * self.stride * 4
+    free, _ = torch.cuda.mem_get_info(dev)
+    if need > free:
+      raise MemoryError(f'rollout with every=0 keeps {need / 2**30:.1f} GiB of stage states;'
+                        f' {free / 2**30:.1f} GiB are free (use checkpoints, every > 0)')
+    self.flat = torch.empty(3 * steps * self.nf * self.stride, dtype=torch.float32, device=dev)
+
+  def state(self, n, m):
+    """Fields of state m (0 u_n, 1 s1, 2 s2) of step n."""
+    out = []
+    for f in range(self.nf):
+      o = ((n * 3 + m) * self.nf + f) * self.stride
+      out.append(self.flat[o:o + self.numel].view(self.shape))
+    return out
+
+
+def _tape_forward(ctx, nctx, dt_dev, steps, u0):
+  """Forward rollout keeping u_n, s1, s2 of every step."""
+  tape = _Tape(steps, u0)
+  for d, t in zip(tape.state(0, 0), u0):
+    d.copy_(t.detach())
+  out = [torch.empty_like(t) for t in u0]
+  for n in range(steps):
+    nxt = tape.state(n + 1, 0) if n + 1 < steps else out
+    rk3_forward(nctx, dt_dev, tape.state(n, 0), tape.state(n, 1), tape.state(n, 2), nxt)
+  ctx.tape = tape
+  return tuple(out)
+
+
+def _tape_backward(ctx, grads):
+  nctx, dt_dev, tape = ctx.nctx, ctx.dt, ctx.tape
+  base = tape.state(0, 0)
+  lam = _grads(nctx, grads, base)
+  lam0 = [torch.zeros_like(t) for t in base]
+  for n in reversed(range(ctx.steps)):
+    rk3_backward(nctx, dt_dev, tape.state(n, 0), tape.state(n, 1), tape.state(n, 2), lam,
+                 lam0)
+  ctx.tape = None
+  return (None, None, None, None, *lam)
+
+
 def rhs(nctx, *q):
   """Differentiable RHS of the conserved fields (list order of nctx.names)."""
   return RhsFunction.apply(nctx, *q)
@@ -140,7 +198,8 @@ def rhs(nctx, *q):
 
 def rollout(nctx, dt, steps, *u0, every=20):
   """Differentiable `steps`-step SSP-RK3 rollout with fixed dt, checkpointing
-  the state every `every` steps."""
+  the state every `every` steps (0: keep every stage state, no
+  recomputation)."""
   dt_dev = _dt_tensor(dt, u0[0].device)
   return RolloutFunction.apply(nctx, dt_dev, int(steps), int(every), *u0)
 

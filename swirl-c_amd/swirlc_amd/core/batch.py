@@ -1,8 +1,8 @@
 """Batched ensembles: many independent grids of different sizes, one launch per phase.
 
 `BatchSolver` advances N simulations that share one configuration (schemes,
-BC types, transport) but have their own grid sizes, spacings, initial states
-and CFL time steps.  Each field of all instances lives in one packed arena;
+BC types and values, transport, gravity, transported scalars) but have their
+own grid sizes, spacings, initial states and CFL time steps.  Each field of all instances lives in one packed arena;
 every phase of an RK stage (halo fill per dim, x / y / z sweep, wave speed)
 is one launch over a tile list covering all instances, and the whole RK3 step
 is captured once in a HIP graph and replayed.  No reference counterpart:
@@ -11,10 +11,12 @@ the reference runs one simulation per TPU job.
 
 import ctypes
 
+import numpy as np
 import torch
 
 from swirlc_amd import _native as N
-from swirlc_amd.core.parameter import SwirlCParameters
+from swirlc_amd.common import utils
+from swirlc_amd.core.parameter import SwirlCParameters, _get_cell_center_coordinates
 from swirlc_amd.equations.native_context import NativeContext
 
 _C11, _C21 = 0.75, 1.0 / 3.0
@@ -27,8 +29,12 @@ class BatchSolver:
   Args:
     cfg: configuration shared by every instance (a SwirlCParameters or dict);
       its grid keys are ignored.
-    instances: list of (grid, state) where grid = (nx, ny, nz, dx, dy, dz) and
-      state maps the conserved names to arrays shaped [nz+2h, nx+2h, ny+2h].
+    instances: list of (grid, state) where grid = (nx, ny, nz, dx, dy, dz) or
+      (nx, ny, nz, dx, dy, dz, x, y, z) and state maps the conserved names to
+      arrays shaped [nz+2h, nx+2h, ny+2h].  x, y, z are the instance's
+      cell-centre coordinates (used by gravity); by default the centres of
+      [0, n * d] along each axis, as a single-grid configuration of lx = nx *
+      dx would have them.
     cfl: Courant number of the per-instance time step (device computed), or
       None to use cfg.dt for every instance.
   """
@@ -43,22 +49,33 @@ class BatchSolver:
     probe_cfg = SwirlCParameters({**cfg._simulation_cfg, 'core_nx': nx0, 'core_ny': ny0,
                                   'core_nz': nz0})
     probe = NativeContext(probe_cfg, device=device, create=False)
-    if probe.nf != 5:
-      raise NotImplementedError('batched ensembles support the 5 base fields only')
     if probe._pending_planes:
-      raise NotImplementedError('batched ensembles do not support per-layer DIRICHLET planes')
+      raise NotImplementedError('batched ensembles do not support per-layer DIRICHLET planes'
+                                ' or plane-valued face / flux values')
     self.names = probe.names
+    self.nf = probe.nf
     h = self.h = probe.hw
     self.n = len(instances)
     descs = (N.ScInstance * self.n)()
     self.shapes, self.offsets = [], []
+    self._coords = []  # host coordinate arrays referenced by descs
     off = 0
     for k, (grid, _) in enumerate(instances):
-      nx, ny, nz, dx, dy, dz = grid
+      nx, ny, nz, dx, dy, dz = grid[:6]
       d = descs[k]
       d.nx, d.ny, d.nz = nx, ny, nz
       d.dx, d.dy, d.dz = float(dx), float(dy), float(dz)
       d.offset = off
+      if probe.gravity_dir != -1:
+        if len(grid) >= 9:
+          axes = grid[6:9]
+        else:
+          axes = [_get_cell_center_coordinates(0.0, sp * m, m)
+                  for m, sp in ((nx, dx), (ny, dy), (nz, dz))]
+        for attr, c in zip(('x_full', 'y_full', 'z_full'), axes):
+          arr = np.ascontiguousarray(utils.mesh_full(c, h), dtype=np.float32)
+          self._coords.append(arr)
+          setattr(d, attr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
       shape = (nz + 2 * h, nx + 2 * h, ny + 2 * h)
       self.shapes.append(shape)
       self.offsets.append(off)
@@ -66,7 +83,7 @@ class BatchSolver:
       off += (cells + _ALIGN - 1) // _ALIGN * _ALIGN
     self.total = off
     dev = torch.device(device)
-    self.u = [torch.zeros(self.total, dtype=torch.float32, device=dev) for _ in range(5)]
+    self.u = [torch.zeros(self.total, dtype=torch.float32, device=dev) for _ in range(self.nf)]
     self.s1 = [torch.zeros_like(t) for t in self.u]
     self.s2 = [torch.zeros_like(t) for t in self.u]
     for k, (_, state) in enumerate(instances):

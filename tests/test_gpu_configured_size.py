@@ -6,7 +6,8 @@
   must equal single-grid steps bit for bit, per-instance dt included.
 * Differentiation: the viscous Kelvin-Helmholtz setup at 128^3 rolled out
   200 SSP-RK3 steps with checkpoints every 20 steps: the gradient does not
-  depend on the checkpoint interval (bit for bit) and repeats exactly.  The
+  depend on the checkpoint interval (20, 50, or every stage state kept; bit
+  for bit) and repeats exactly.  The
   gradient's agreement with FP64 central finite differences of the oracle is
   checked on a KH-like state with w != 0 (so z faces leave the HLLC
   contact-speed snap of hllc.py:339-341) along 16 directions from
@@ -87,16 +88,17 @@ def test_kh128_checkpointed_rollout_is_interval_invariant_and_deterministic():
   J20, g20 = grad(20)
   J50, g50 = grad(50)
   J20b, g20b = grad(20)
-  assert J20 == J50 == J20b
+  J0, g0 = grad(0)  # every stage state kept, nothing recomputed
+  assert J20 == J50 == J20b == J0
   sl = (slice(h, -h),) * 3
-  for a, b, c in zip(g20, g50, g20b):
-    assert torch.equal(a, b) and torch.equal(a, c)
+  for a, b, c, d in zip(g20, g50, g20b, g0):
+    assert torch.equal(a, b) and torch.equal(a, c) and torch.equal(a, d)
     assert bool(torch.isfinite(a).all())
     halo = a.clone()
     halo[sl] = 0.0
     assert not bool(halo.any())
-  print(f'[adjoint] J = {J20:.6e}, gradient identical for checkpoint intervals 20 and 50',
-        flush=True)
+  print(f'[adjoint] J = {J20:.6e}, gradient identical for checkpoint intervals 20, 50 and'
+        ' for the full tape', flush=True)
 
 
 def _kh_3d(n, h=3):
