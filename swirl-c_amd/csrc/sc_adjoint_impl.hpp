@@ -68,6 +68,63 @@ __device__ __forceinline__ void add_lG(const AdjArgs& A, long long idx, const fl
   }
 }
 
+// Boundary face / flux values (the forward's face_flux at a face with a
+// BoundaryBc) in reverse: a flux value fixes its component of the
+// convective, diffusive or total flux, so that component's adjoint stops
+// there; a face value fixes a face-state component (primitive: a constant;
+// conserved: value / rho_face, whose derivative reaches the face density).
+__device__ __forceinline__ void flux_bc_adj(const BoundaryBc& b, const float* lF, float* lFc,
+                                            float* lFd) {
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    const bool tot = (b.flux_mask >> (16 + f)) & 1;
+    lFc[f] = (tot || ((b.flux_mask >> f) & 1)) ? 0.0f : lF[f];
+    lFd[f] = (tot || ((b.flux_mask >> (8 + f)) & 1)) ? 0.0f : lF[f];
+  }
+}
+
+// lW: adjoint of the face state after apply_face_bc; on return, of the
+// state before it (W, the state before it: its density is never replaced).
+__device__ __forceinline__ void face_bc_adj(const BoundaryBc& b, long long pidx,
+                                            const float* W, float* lW) {
+  for (int e = b.n_face - 1; e >= 0; --e) {
+    const FaceEntry fe = b.face[e];
+#pragma unroll
+    for (int k = 1; k < 7; ++k) {
+      if (k != fe.var) continue;
+      if (fe.kind == 1 && W[kRho] != 0.0f) {
+        const float val = fe.plane != nullptr ? fe.plane[pidx] : fe.val;
+        lW[kRho] -= lW[k] * (val / (W[kRho] * W[kRho]));
+      }
+      lW[k] = 0.0f;
+    }
+  }
+}
+
+// Riemann adjoint of one face with the boundary values of bb (or none):
+// L, R the reconstructed face states before the face values, lFc the
+// convective flux adjoint; lL, lR receive the state adjoints.
+template <int FLUX, int D>
+__device__ __forceinline__ void face_riemann_adj(const BoundaryBc* bb, long long pidx,
+                                                 const float* L, const float* R, float pe,
+                                                 const float* lFc, float* lL, float* lR) {
+  if (bb == nullptr) {
+    riemann_adj<FLUX, D>(L, R, pe, lFc, lL, lR);
+    return;
+  }
+  float Lm[7], Rm[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    Lm[v] = L[v];
+    Rm[v] = R[v];
+  }
+  apply_face_bc<0>(*bb, Lm, pidx);
+  apply_face_bc<0>(*bb, Rm, pidx);
+  riemann_adj<FLUX, D>(Lm, Rm, pe, lFc, lL, lR);
+  face_bc_adj(*bb, pidx, L, lL);
+  face_bc_adj(*bb, pidx, R, lR);
+}
+
 // ---------------------------------------------------------------------------
 // Marching adjoint pass along x (DIR 0) or z (DIR 2).  Block (64, BY): lane ->
 // y, threadIdx.y -> the other transverse index, blockIdx.z -> a run of
@@ -75,7 +132,8 @@ __device__ __forceinline__ void add_lG(const AdjArgs& A, long long idx, const fl
 // a face stencil of the direction reaches).  Iteration c handles face c (the
 // face between cells c-1 and c) and the reconstruction adjoint of cell c-1.
 // ---------------------------------------------------------------------------
-template <int DIR, int SCHEME, int FLUX, bool VISC>
+// BB: the build with boundary face / flux values.
+template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB>
 __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
@@ -171,14 +229,23 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       float lF[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) lF[f] = (gC[f] - gPrev[f]) * a.inv_hdir;
-      riemann_adj<FLUX, DIR>(fpPrev, fmC, pe_at(a, kk(c), ii(c), j), lF, lL, lR);
+      const int side = c == a.h ? 0 : (c == a.h + n ? 1 : -1);
+      const BoundaryBc* bb =
+          (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
+      float lFc[5], lFd[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) lFc[f] = lFd[f] = lF[f];
+      if (bb != nullptr) flux_bc_adj(*bb, lF, lFc, lFd);
+      // Boundary plane cell of this face: x faces [z][y], z faces [x][y].
+      face_riemann_adj<FLUX, DIR>(bb, (long long)o * a.NY + j, fpPrev, fmC,
+                                  pe_at(a, kk(c), ii(c), j), lFc, lL, lR);
       if constexpr (VISC) {
         transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
         // Cells c-1 and c sit at window positions RAD and RAD+1.
         float lA[NV], lB[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) lA[v] = lB[v] = 0.0f;
-        viscous_adj<DIR>(W[RAD], W[RAD + 1], tPrev, tCur, a.vc, lF, lA, lB, ltPrev, ltCur);
+        viscous_adj<DIR>(W[RAD], W[RAD + 1], tPrev, tCur, a.vc, lFd, lA, lB, ltPrev, ltCur);
         const int sa = slot(RAD), sb = slot(RAD + 1);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -252,11 +319,11 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
   }
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC>
+template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB>
 __global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_march_body<DIR, SCHEME, FLUX, VISC>(A, ti);
+  adj_march_body<DIR, SCHEME, FLUX, VISC, BB>(A, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,7 +352,7 @@ __host__ __device__ inline int y_adj_windows(int nx, int ny, int h, int rad, int
   return (int)((span + outa - 1) / outa);
 }
 
-template <int SCHEME, int FLUX, bool VISC>
+template <int SCHEME, int FLUX, bool VISC, bool BB>
 __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = YAdjGeom<SCHEME>::RAD;
   constexpr int KA = YAdjGeom<SCHEME>::KA;
@@ -359,8 +426,16 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     float lF[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) lF[f] = (gC[f] - gP[f]) * a.inv_hdir;
-    riemann_adj<FLUX, 1>(L, fm, pe, lF, lL, lR);
-    if constexpr (VISC) viscous_adj<1>(Am, W0, tPrev, tCur, a.vc, lF, lA, lB, lta, ltb);
+    const int side = yc == a.h ? 0 : (yc == a.h + a.ny ? 1 : -1);
+    const BoundaryBc* bb =
+        (BB && ((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
+    float lFc[5], lFd[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) lFc[f] = lFd[f] = lF[f];
+    if (bb != nullptr) flux_bc_adj(*bb, lF, lFc, lFd);
+    // Boundary plane cell of this face: y faces [z][x].
+    face_riemann_adj<FLUX, 1>(bb, (long long)k * a.NX + i, L, fm, pe, lFc, lL, lR);
+    if constexpr (VISC) viscous_adj<1>(Am, W0, tPrev, tCur, a.vc, lFd, lA, lB, lta, ltb);
   }
   // Reconstruction adjoint of the own cell: left state of face y+1 (lane + 1)
   // and right state of face y.
@@ -399,16 +474,16 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC>
+template <int SCHEME, int FLUX, bool VISC, bool BB>
 __global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_y_body<SCHEME, FLUX, VISC>(A, ti);
+  adj_y_body<SCHEME, FLUX, VISC, BB>(A, ti);
 }
 
 
 // Launches the three adjoint passes (x, y, z) for one scheme.
-template <int SCHEME, int FLUX, bool VISC>
+template <int SCHEME, int FLUX, bool VISC, bool BB>
 hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& tune,
                                hipStream_t stream) {
   constexpr int RAD = Radius<SCHEME>::value;
@@ -424,23 +499,32 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
     ax.s.inv_hdir = 1.0f / h[0];
     const int ext = a.nx + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nz + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC>), grid, blk_m, lds, stream, ax);
+    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, BB>), grid, blk_m, lds, stream,
+                       ax);
   }
   {
     AdjArgs ay = A;
     ay.s.inv_hdir = 1.0f / h[1];
     const int nwin = y_adj_windows(a.nx, a.ny, a.h, RAD, YAdjGeom<SCHEME>::OUTA);
     const dim3 grid((nwin + by_y - 1) / by_y, 1, a.nz);
-    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC>), grid, blk_y, 0, stream, ay);
+    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, BB>), grid, blk_y, 0, stream, ay);
   }
   {
     AdjArgs az = A;
     az.s.inv_hdir = 1.0f / h[2];
     const int ext = a.nz + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nx + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC>), grid, blk_m, lds, stream, az);
+    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, BB>), grid, blk_m, lds, stream,
+                       az);
   }
   return hipGetLastError();
+}
+
+template <int SCHEME, int FLUX, bool VISC>
+hipError_t launch_adj_bb(const AdjArgs& A, const float h[3], const SweepTuning& tune,
+                         hipStream_t stream) {
+  return A.s.bbc_dims != 0 ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, true>(A, h, tune, stream)
+                           : launch_adj_sweeps_t<SCHEME, FLUX, VISC, false>(A, h, tune, stream);
 }
 
 template <int SCHEME>
@@ -453,10 +537,10 @@ hipError_t launch_adj_sweeps(const AdjArgs& A, const float h[3], int flux, bool 
                                        bool visc, const SweepTuning& tune,                \
                                        hipStream_t stream) {                              \
     if (flux == kHllc)                                                                    \
-      return visc ? launch_adj_sweeps_t<SCHEME, kHllc, true>(A, h, tune, stream)          \
-                  : launch_adj_sweeps_t<SCHEME, kHllc, false>(A, h, tune, stream);        \
-    return visc ? launch_adj_sweeps_t<SCHEME, kHll, true>(A, h, tune, stream)             \
-                : launch_adj_sweeps_t<SCHEME, kHll, false>(A, h, tune, stream);           \
+      return visc ? launch_adj_bb<SCHEME, kHllc, true>(A, h, tune, stream)                \
+                  : launch_adj_bb<SCHEME, kHllc, false>(A, h, tune, stream);              \
+    return visc ? launch_adj_bb<SCHEME, kHll, true>(A, h, tune, stream)                   \
+                : launch_adj_bb<SCHEME, kHll, false>(A, h, tune, stream);                 \
   }
 
 // Non-templated kernels of the adjoint (sc_adjoint.hip).

@@ -869,8 +869,6 @@ sc_status sc_ctx_enable_adjoint(sc_ctx* c) {
     return fail(SC_ERR_UNSUPPORTED, "the adjoint supports the 5 base conserved fields only");
   if (c->params.prim_bc_mask != 0)
     return fail(SC_ERR_UNSUPPORTED, "the adjoint does not support primitive-variable BCs");
-  if (c->bbc_dims != 0)
-    return fail(SC_ERR_UNSUPPORTED, "the adjoint does not support face or flux boundary values");
   if (c->adj_lq[0] != nullptr) return SC_OK;
   sc_status st = SC_OK;
   const size_t bytes = sizeof(float) * c->cells_full;

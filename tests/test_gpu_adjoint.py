@@ -153,6 +153,58 @@ def test_rhs_vjp_matches_fp64_finite_differences(scheme, flux, visc, bcs, gravit
   assert max(errs) <= FD_TOL
 
 
+BV_CASES = [
+    ('WENO_5', 'HLLC', True, False, False),
+    ('WENO_3', 'HLLC', False, True, False),
+    ('MUSCL', 'HLL', False, True, True),
+    ('WENO_5', 'HLL', True, False, True),
+]
+
+
+@pytest.mark.parametrize('scheme,flux,visc,faces,planes', BV_CASES)
+def test_rhs_vjp_with_boundary_values_matches_fp64_finite_differences(scheme, flux, visc,
+                                                                      faces, planes):
+  """Face and flux Dirichlet values (boundary.py:89-461): a fixed flux
+  component passes no adjoint, a fixed primitive face value neither, and a
+  fixed conserved face value q_f / rho_f reaches the face density; values
+  as scalars and as planes."""
+  import test_gpu_boundary_values as BV
+  n, h = (12, 10, 9), 3
+  cfg = _cfg(n, scheme, flux, visc, 'mixed')
+  fluxes = BV._plane_fluxes(n, h, 80) if planes else BV._flux_spec()
+  cfg['bc']['intercell_fluxes'] = fluxes
+  if faces:
+    cfg['bc']['cell_faces'] = BV._planes(BV._face_spec(), n, h, 81) if planes else \
+        BV._face_spec()
+  st = random_state(n, h, seed=4, jump=False)
+  sl = (slice(h, -h),) * 3
+  rng = np.random.default_rng(12)
+  g = {k: np.zeros_like(st[k]) for k in NAMES}
+  for k in NAMES:
+    g[k][sl] = rng.standard_normal(g[k][sl].shape).astype(np.float32)
+  ctx = _ctx(cfg)
+  q = _gpu(st)
+  lq = [torch.zeros_like(t) for t in q]
+  print(f'[adjoint] rhs vjp with boundary values {scheme} {flux} visc={visc} faces={faces}'
+        f' planes={planes}', flush=True)
+  ctx.rhs_vjp(q, _gpu(g), lq)
+  torch.cuda.synchronize()
+  grad = [t.cpu().numpy() for t in lq]
+  ocfg = oracle_cfg(cfg)
+
+  def J(s):
+    r = O.rhs(s, {}, ocfg)
+    return sum(float(np.sum(g[k][sl].astype(np.float64) * r[k][sl])) for k in NAMES)
+
+  errs = []
+  for v in _directions(st, h, 4, seed=22):
+    fd = _fd(J, st, v)
+    ad = _dot(grad, v)
+    errs.append(abs(ad - fd) / abs(fd))
+  print(f'[adjoint]   rel errors {["%.1e" % e for e in errs]}', flush=True)
+  assert max(errs) <= FD_TOL
+
+
 def _kh_small(n, nu=1e-2):
   cfg = _cfg(n, 'WENO_5', 'HLLC', True, 'periodic', nu=nu)
   h = 3
