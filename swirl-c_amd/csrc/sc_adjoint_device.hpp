@@ -233,8 +233,9 @@ __device__ __forceinline__ void recon_adj<kMuscl>(const float* a, float lfp, flo
 // lF: adjoint of the 5 face fluxes; accumulates into lL, lR.
 // ---------------------------------------------------------------------------
 
-// Adjoint of physical_flux<D, 0>.
-template <int D>
+// Adjoint of physical_flux<D, 0, PD> (PD: the aligned momentum flux takes the
+// dynamic pressure s[7], the energy flux keeps p).
+template <int D, bool PD = false>
 __device__ __forceinline__ void physical_flux_adj(const float* s, const float* lF, float* ls) {
   const float rho = s[kRho], vn = s[kU + D], ir = frcp(rho);
   const float m = rho * vn;
@@ -244,7 +245,12 @@ __device__ __forceinline__ void physical_flux_adj(const float* s, const float* l
     lm += lF[1 + k] * s[kU + k];
     ls[kU + k] += lF[1 + k] * m;
   }
-  ls[kP] += lF[1 + D] + lF[4] * m * ir;
+  if constexpr (PD) {
+    ls[7] += lF[1 + D];
+    ls[kP] += lF[4] * m * ir;
+  } else {
+    ls[kP] += lF[1 + D] + lF[4] * m * ir;
+  }
   ls[kE] += lF[4] * m;
   ls[kRho] += -lF[4] * m * s[kP] * ir * ir;
   ls[kRho] += lm * vn;
@@ -379,9 +385,36 @@ __device__ __forceinline__ void hllc_star_adj(const float* S, float sk, float sm
   lS[kU + D] += lvn;
 }
 
-template <int D>
-__device__ __forceinline__ void hllc_adj(const float* L, const float* R, float pe,
-                                         const float* lF, float* lL, float* lR) {
+// PD: L0, R0 carry the dynamic pressure at index 7; the wave speeds, the
+// contact speed and the star state use it in place of p (hllc.py:109-120),
+// the physical flux both (physical_flux_adj<D, true>).
+template <int D, bool PD = false>
+__device__ __forceinline__ void hllc_adj(const float* L0, const float* R0, float pe,
+                                         const float* lF, float* lL0, float* lR0) {
+  constexpr int NV = PD ? 8 : 7;
+  float Lp[7], Rp[7];
+  if constexpr (PD) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      Lp[v] = L0[v];
+      Rp[v] = R0[v];
+    }
+    Lp[kP] = L0[7];
+    Rp[kP] = R0[7];
+  }
+  const float* L = PD ? Lp : L0;
+  const float* R = PD ? Rp : R0;
+  // Adjoints with respect to L, R (with PD: p stands for p_dyn); merged into
+  // lL0, lR0 at the end.
+  float lLp[7], lRp[7];
+  float* lL = lL0;
+  float* lR = lR0;
+  if constexpr (PD) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) lLp[v] = lRp[v] = 0.0f;
+    lL = lLp;
+    lR = lRp;
+  }
   float sl, sr;
   roe_speeds<D>(L, R, pe, sl, sr);
   const float vl = L[kU + D], vr = R[kU + D];
@@ -404,19 +437,47 @@ __device__ __forceinline__ void hllc_adj(const float* L, const float* R, float p
   // One side's flux (L for branches 0, 1, R for 2, 3), selected per lane as
   // in the forward, so the lanes of a wave share one adjoint evaluation.
   const bool left = br <= 1;
-  float S[7], lS[7];
-#pragma unroll
-  for (int v = 0; v < 7; ++v) {
-    S[v] = left ? L[v] : R[v];
-    lS[v] = 0.0f;
-  }
   float lsk = 0.0f, lsm = 0.0f;
-  physical_flux_adj<D>(S, lF, lS);
-  if (br == 1 || br == 2) hllc_star_adj<D>(S, left ? sl : sr, sm, lF, lS, lsk, lsm);
+  if constexpr (PD) {
+    // Physical flux of the side's own state (p and p_dyn), star correction
+    // of the state with p_dyn for p.
+    float S0[NV], Ss[7], lS0[NV], lSs[7];
 #pragma unroll
-  for (int v = 0; v < 7; ++v) {
-    lL[v] += left ? lS[v] : 0.0f;
-    lR[v] += left ? 0.0f : lS[v];
+    for (int v = 0; v < NV; ++v) {
+      S0[v] = left ? L0[v] : R0[v];
+      lS0[v] = 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      Ss[v] = left ? L[v] : R[v];
+      lSs[v] = 0.0f;
+    }
+    physical_flux_adj<D, true>(S0, lF, lS0);
+    if (br == 1 || br == 2) hllc_star_adj<D>(Ss, left ? sl : sr, sm, lF, lSs, lsk, lsm);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      lL0[v] += left ? lS0[v] : 0.0f;
+      lR0[v] += left ? 0.0f : lS0[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      lL[v] += left ? lSs[v] : 0.0f;
+      lR[v] += left ? 0.0f : lSs[v];
+    }
+  } else {
+    float S[7], lS[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      S[v] = left ? L[v] : R[v];
+      lS[v] = 0.0f;
+    }
+    physical_flux_adj<D>(S, lF, lS);
+    if (br == 1 || br == 2) hllc_star_adj<D>(S, left ? sl : sr, sm, lF, lS, lsk, lsm);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      lL[v] += left ? lS[v] : 0.0f;
+      lR[v] += left ? 0.0f : lS[v];
+    }
   }
   float lsl = left ? lsk : 0.0f, lsr = left ? 0.0f : lsk;
   if (!snapped && den != 0.0f && lsm != 0.0f) {
@@ -439,9 +500,19 @@ __device__ __forceinline__ void hllc_adj(const float* L, const float* R, float p
     lR[kU + D] += lden * rr;
   }
   roe_speeds_adj<D>(L, R, pe, lsl, lsr, lL, lR);
+  if constexpr (PD) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      const int dst = v == kP ? 7 : v;
+      lL0[dst] += lLp[v];
+      lR0[dst] += lRp[v];
+    }
+  }
 }
 
-template <int D>
+// PD: the physical fluxes take p_dyn (index 7) in the aligned momentum flux;
+// the wave speeds keep p (hll.py:107-115).
+template <int D, bool PD = false>
 __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe,
                                         const float* lF, float* lL, float* lR) {
   float sl, sr;
@@ -454,8 +525,8 @@ __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe
   const float n3 = sr * fabsf(sl) - sl * fabsf(sr);
   const float t3 = n3 * (0.5f * rds);
   float fl[5], fr[5], ul[5], ur[5];
-  physical_flux<D, 0>(L, fl);
-  physical_flux<D, 0>(R, fr);
+  physical_flux<D, 0, PD>(L, fl);
+  physical_flux<D, 0, PD>(R, fr);
   ul[0] = L[kRho];
   ur[0] = R[kRho];
 #pragma unroll
@@ -483,8 +554,8 @@ __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe
       lL[k] += lul * L[kRho];
     }
   }
-  physical_flux_adj<D>(R, lfr, lR);
-  physical_flux_adj<D>(L, lfl, lL);
+  physical_flux_adj<D, PD>(R, lfr, lR);
+  physical_flux_adj<D, PD>(L, lfl, lL);
   lt1 -= lt2;
   float lsl = 0.0f, lsr = 0.0f, lrds = lt1 * n1 + lt3 * 0.5f * n3;
   if (sr < 0.0f) lsr += lt1 * rds;
@@ -500,13 +571,13 @@ __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe
   roe_speeds_adj<D>(L, R, pe, lsl, lsr, lL, lR);
 }
 
-template <int FLUX, int D>
+template <int FLUX, int D, bool PD = false>
 __device__ __forceinline__ void riemann_adj(const float* L, const float* R, float pe,
                                             const float* lF, float* lL, float* lR) {
   if constexpr (FLUX == kHllc) {
-    hllc_adj<D>(L, R, pe, lF, lL, lR);
+    hllc_adj<D, PD>(L, R, pe, lF, lL, lR);
   } else {
-    hll_adj<D>(L, R, pe, lF, lL, lR);
+    hll_adj<D, PD>(L, R, pe, lF, lL, lR);
   }
 }
 

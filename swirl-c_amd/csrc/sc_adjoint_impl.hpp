@@ -39,12 +39,24 @@ struct AdjArgs {
 };
 
 // Conserved adjoint of one cell from its primitive adjoint (and the gravity
-// source adjoint of the x pass), accumulated into lq.
+// source adjoint of the x pass), accumulated into lq.  PD: lw[7] is the
+// adjoint of the cell's dynamic pressure p - p_ref (a constant 0 outside
+// the interior along the pass's direction).  The gravity source takes
+// rho - rho_ref with a reference state: the same derivative.
+template <bool PD = false>
 __device__ __forceinline__ void retire_cell(const AdjArgs& A, long long idx, const float* W,
                                             float pe, const float* lw, bool src,
                                             bool interior) {
   float lq[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  cons_to_prim_adj(W, pe, lw, lq);
+  if constexpr (PD) {
+    float l7[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) l7[v] = lw[v];
+    if (interior) l7[kP] += lw[7];
+    cons_to_prim_adj(W, pe, l7, lq);
+  } else {
+    cons_to_prim_adj(W, pe, lw, lq);
+  }
   const SweepArgs& a = A.s;
   if (src && interior && a.grav_src) {
     // R[1+d] += (rho * G) * g_d on the selected momenta.
@@ -104,23 +116,24 @@ __device__ __forceinline__ void face_bc_adj(const BoundaryBc& b, long long pidx,
 // Riemann adjoint of one face with the boundary values of bb (or none):
 // L, R the reconstructed face states before the face values, lFc the
 // convective flux adjoint; lL, lR receive the state adjoints.
-template <int FLUX, int D>
+template <int FLUX, int D, bool PD>
 __device__ __forceinline__ void face_riemann_adj(const BoundaryBc* bb, long long pidx,
                                                  const float* L, const float* R, float pe,
                                                  const float* lFc, float* lL, float* lR) {
   if (bb == nullptr) {
-    riemann_adj<FLUX, D>(L, R, pe, lFc, lL, lR);
+    riemann_adj<FLUX, D, PD>(L, R, pe, lFc, lL, lR);
     return;
   }
-  float Lm[7], Rm[7];
+  constexpr int NV = PD ? 8 : 7;
+  float Lm[NV], Rm[NV];
 #pragma unroll
-  for (int v = 0; v < 7; ++v) {
+  for (int v = 0; v < NV; ++v) {
     Lm[v] = L[v];
     Rm[v] = R[v];
   }
   apply_face_bc<0>(*bb, Lm, pidx);
   apply_face_bc<0>(*bb, Rm, pidx);
-  riemann_adj<FLUX, D>(Lm, Rm, pe, lFc, lL, lR);
+  riemann_adj<FLUX, D, PD>(Lm, Rm, pe, lFc, lL, lR);
   face_bc_adj(*bb, pidx, L, lL);
   face_bc_adj(*bb, pidx, R, lR);
 }
@@ -132,13 +145,14 @@ __device__ __forceinline__ void face_riemann_adj(const BoundaryBc* bb, long long
 // a face stencil of the direction reaches).  Iteration c handles face c (the
 // face between cells c-1 and c) and the reconstruction adjoint of cell c-1.
 // ---------------------------------------------------------------------------
-// BB: the build with boundary face / flux values.
-template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB>
+// BB: the build with boundary face / flux values; PD: with a reference state
+// (the dynamic pressure is an eighth reconstructed variable).
+template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
 __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
   constexpr int NW = WN + 1;  // window: cells c-1-RAD .. c+RAD
-  constexpr int NV = 7;
+  constexpr int NV = PD ? 8 : 7;
   const SweepArgs& a = A.s;
   const int j = a.h + ti.bx * 64 + ti.tx;
   const int o = a.h + ti.by * ti.by_dim + ti.ty;
@@ -178,7 +192,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
   for (int w = 0; w < WN; ++w) {
     const int c = c_start - 1 - RAD + w;
-    load_prim<0>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
+    load_prim<0, PD>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
                  W[w]);
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc(w, v) = 0.0f;
@@ -205,7 +219,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
   for (int c = c_start; c <= c_last; ++c) {
     {
       const int cn = min(c + RAD, NM - 1);
-      load_prim<0>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
+      load_prim<0, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
                    pe_at(a, kk(cn), ii(cn), j), W[NW - 1]);
     }
     const bool face = c <= f_hi;
@@ -237,7 +251,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       for (int f = 0; f < 5; ++f) lFc[f] = lFd[f] = lF[f];
       if (bb != nullptr) flux_bc_adj(*bb, lF, lFc, lFd);
       // Boundary plane cell of this face: x faces [z][y], z faces [x][y].
-      face_riemann_adj<FLUX, DIR>(bb, (long long)o * a.NY + j, fpPrev, fmC,
+      face_riemann_adj<FLUX, DIR, PD>(bb, (long long)o * a.NY + j, fpPrev, fmC,
                                   pe_at(a, kk(c), ii(c), j), lFc, lL, lR);
       if constexpr (VISC) {
         transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
@@ -282,7 +296,8 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
         acc(s0, v) = 0.0f;
       }
       if (owned(x))
-        retire_cell(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), l0, DIR == 0,
+        retire_cell<PD>(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), l0,
+                        DIR == 0,
                     interior(x));
       s0 = s0 + 1 < WN ? s0 + 1 : 0;
     }
@@ -314,16 +329,17 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
     for (int v = 0; v < NV; ++v) l0[v] = acc(slot(w), v);
     if (owned(x))
-      retire_cell(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), l0, DIR == 0,
+      retire_cell<PD>(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), l0,
+                      DIR == 0,
                   interior(x));
   }
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB>
+template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
 __global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_march_body<DIR, SCHEME, FLUX, VISC, BB>(A, ti);
+  adj_march_body<DIR, SCHEME, FLUX, VISC, BB, PD>(A, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,13 +368,13 @@ __host__ __device__ inline int y_adj_windows(int nx, int ny, int h, int rad, int
   return (int)((span + outa - 1) / outa);
 }
 
-template <int SCHEME, int FLUX, bool VISC, bool BB>
+template <int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
 __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = YAdjGeom<SCHEME>::RAD;
   constexpr int KA = YAdjGeom<SCHEME>::KA;
   constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
   constexpr int WN = 2 * RAD + 1;
-  constexpr int NV = 7;
+  constexpr int NV = PD ? 8 : 7;
   const SweepArgs& a = A.s;
   const int lane = ti.tx;
   const int win = ti.bx * ti.by_dim + ti.ty;
@@ -391,7 +407,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   const float pe = pe_at(a, k, i, yc);
 
   float W0[NV];
-  load_prim<0>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
+  load_prim<0, PD>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
   float S[NV][WN], fp[NV], fm[NV], Am[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -434,7 +450,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     for (int f = 0; f < 5; ++f) lFc[f] = lFd[f] = lF[f];
     if (bb != nullptr) flux_bc_adj(*bb, lF, lFc, lFd);
     // Boundary plane cell of this face: y faces [z][x].
-    face_riemann_adj<FLUX, 1>(bb, (long long)k * a.NX + i, L, fm, pe, lFc, lL, lR);
+    face_riemann_adj<FLUX, 1, PD>(bb, (long long)k * a.NX + i, L, fm, pe, lFc, lL, lR);
     if constexpr (VISC) viscous_adj<1>(Am, W0, tPrev, tCur, a.vc, lFd, lA, lB, lta, ltb);
   }
   // Reconstruction adjoint of the own cell: left state of face y+1 (lane + 1)
@@ -467,23 +483,23 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     lt[t][1] = ltb[t][1] + wave_shl1(lta[t][1]);
   }
   if (out_lane) {
-    retire_cell(A, idx, W0, pe, tot, false, inner);
+    retire_cell<PD>(A, idx, W0, pe, tot, false, inner);
     if constexpr (VISC) {
       if (rcell) add_lG<1>(A, idx, lt);
     }
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC, bool BB>
+template <int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
 __global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_y_body<SCHEME, FLUX, VISC, BB>(A, ti);
+  adj_y_body<SCHEME, FLUX, VISC, BB, PD>(A, ti);
 }
 
 
 // Launches the three adjoint passes (x, y, z) for one scheme.
-template <int SCHEME, int FLUX, bool VISC, bool BB>
+template <int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
 hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& tune,
                                hipStream_t stream) {
   constexpr int RAD = Radius<SCHEME>::value;
@@ -492,14 +508,14 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
   A.s.seg_len = seg;
   const dim3 blk_m(64, by);
   const dim3 blk_y(64, by_y);
-  // LDS ring of the marching passes: (2*RAD+1) cells x 7 primitives per thread.
-  const size_t lds = sizeof(float) * (2 * RAD + 1) * 7 * 64 * by;
+  // LDS ring of the marching passes: (2*RAD+1) cells x NV primitives per thread.
+  const size_t lds = sizeof(float) * (2 * RAD + 1) * (PD ? 8 : 7) * 64 * by;
   {
     AdjArgs ax = A;
     ax.s.inv_hdir = 1.0f / h[0];
     const int ext = a.nx + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nz + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, BB>), grid, blk_m, lds, stream,
+    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, BB, PD>), grid, blk_m, lds, stream,
                        ax);
   }
   {
@@ -507,14 +523,14 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
     ay.s.inv_hdir = 1.0f / h[1];
     const int nwin = y_adj_windows(a.nx, a.ny, a.h, RAD, YAdjGeom<SCHEME>::OUTA);
     const dim3 grid((nwin + by_y - 1) / by_y, 1, a.nz);
-    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, BB>), grid, blk_y, 0, stream, ay);
+    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, BB, PD>), grid, blk_y, 0, stream, ay);
   }
   {
     AdjArgs az = A;
     az.s.inv_hdir = 1.0f / h[2];
     const int ext = a.nz + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nx + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, BB>), grid, blk_m, lds, stream,
+    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, BB, PD>), grid, blk_m, lds, stream,
                        az);
   }
   return hipGetLastError();
@@ -523,8 +539,12 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
 template <int SCHEME, int FLUX, bool VISC>
 hipError_t launch_adj_bb(const AdjArgs& A, const float h[3], const SweepTuning& tune,
                          hipStream_t stream) {
-  return A.s.bbc_dims != 0 ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, true>(A, h, tune, stream)
-                           : launch_adj_sweeps_t<SCHEME, FLUX, VISC, false>(A, h, tune, stream);
+  const bool bb = A.s.bbc_dims != 0, pd = A.s.p_ref != nullptr;
+  if (pd)
+    return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, true, true>(A, h, tune, stream)
+              : launch_adj_sweeps_t<SCHEME, FLUX, VISC, false, true>(A, h, tune, stream);
+  return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, true, false>(A, h, tune, stream)
+            : launch_adj_sweeps_t<SCHEME, FLUX, VISC, false, false>(A, h, tune, stream);
 }
 
 template <int SCHEME>

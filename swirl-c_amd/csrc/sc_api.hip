@@ -944,8 +944,6 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
 sc_status check_adjoint_ready(sc_ctx* c, const char* who) {
   if (c->adj_lq[0] == nullptr)
     return fail(SC_ERR_INVALID, std::string(who) + ": call sc_ctx_enable_adjoint first");
-  if (c->sweep.p_ref != nullptr)
-    return fail(SC_ERR_UNSUPPORTED, std::string(who) + ": no adjoint with a reference state");
   return SC_OK;
 }
 

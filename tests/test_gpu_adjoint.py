@@ -205,6 +205,50 @@ def test_rhs_vjp_with_boundary_values_matches_fp64_finite_differences(scheme, fl
   assert max(errs) <= FD_TOL
 
 
+@pytest.mark.parametrize('scheme,flux,visc,frac', [('WENO_5', 'HLLC', True, 0.5),
+                                                   ('MUSCL', 'HLL', True, 0.9),
+                                                   ('WENO_3', 'HLLC', False, 0.7)])
+def test_rhs_vjp_with_reference_state_matches_fp64_finite_differences(scheme, flux, visc,
+                                                                      frac):
+  """Hydrostatic reference state (helper_vars p_ref, rho_ref;
+  compressible_navier_stokes.py:273-335): p_dyn = p - p_ref reconstructed and
+  used in the momentum flux and, for HLLC, in the wave speeds and star
+  states (hllc.py:109-120); rho - rho_ref in the gravity source.  p_ref and
+  rho_ref are constants of the derivative."""
+  import test_gpu_reference_state as RS
+  n, h = (12, 10, 9), 3
+  cfg, ocfg, st, helper = RS._setup(n, scheme, flux, visc, 44, frac)
+  cfg['transport_parameters'] = {'nu': 1e-2, 'pr': 0.7}
+  ocfg = oracle_cfg(cfg)
+  sl = (slice(h, -h),) * 3
+  rng = np.random.default_rng(13)
+  g = {k: np.zeros_like(st[k]) for k in NAMES}
+  for k in NAMES:
+    g[k][sl] = rng.standard_normal(g[k][sl].shape).astype(np.float32)
+  ctx = _ctx(cfg)
+  ctx.set_reference_state(*(torch.from_numpy(helper[k]).cuda() for k in ('p_ref', 'rho_ref')))
+  q = _gpu(st)
+  lq = [torch.zeros_like(t) for t in q]
+  print(f'[adjoint] rhs vjp with reference state {scheme} {flux} visc={visc} frac={frac}',
+        flush=True)
+  ctx.rhs_vjp(q, _gpu(g), lq)
+  torch.cuda.synchronize()
+  grad = [t.cpu().numpy() for t in lq]
+  hv = {k: v.astype(np.float64) for k, v in helper.items()}
+
+  def J(s):
+    r = O.rhs(s, hv, ocfg)
+    return sum(float(np.sum(g[k][sl].astype(np.float64) * r[k][sl])) for k in NAMES)
+
+  errs = []
+  for v in _directions(st, h, 4, seed=23):
+    fd = _fd(J, st, v)
+    ad = _dot(grad, v)
+    errs.append(abs(ad - fd) / abs(fd))
+  print(f'[adjoint]   rel errors {["%.1e" % e for e in errs]}', flush=True)
+  assert max(errs) <= FD_TOL
+
+
 def _kh_small(n, nu=1e-2):
   cfg = _cfg(n, 'WENO_5', 'HLLC', True, 'periodic', nu=nu)
   h = 3
