@@ -335,8 +335,8 @@ sc_status sc_ctx_set_tuning(sc_ctx* ctx, const sc_tuning* tuning);
  * branch, wave-speed min/max and MUSCL limiter selections replayed; WENO
  * smoothness-indicator and weight derivatives included).  dt is a constant
  * (no derivative through a CFL time step).  Boundary face / flux values and
- * a reference state (sc_ctx_set_reference_state) are differentiated too;
- * restrictions: 5 fields, no primitive-variable BCs.  p_ref and rho_ref are
+ * a reference state (sc_ctx_set_reference_state) and primitive-variable BCs
+ * are differentiated too; restriction: 5 fields.  p_ref and rho_ref are
  * constants of the derivative.  Cotangents ("lam") are full-size field arrays of
  * which only interior cells are read; outputs have zero halo cells. */
 

@@ -12,6 +12,10 @@ constexpr int kZeroMax = 16;
 struct PtrList {
   float* p[kZeroMax];
 };
+// The stage state's fields (read-only).
+struct StatePtrs {
+  const float* p[kMaxFields];
+};
 
 __global__ void __launch_bounds__(256) adj_zero_kernel(PtrList l, int np, long long n) {
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -58,46 +62,102 @@ __global__ void __launch_bounds__(256) adj_visc_gather_kernel(const AdjArgs A) {
   A.lq.p[0][X] += lrho * inv;
 }
 
-// Reverse of halo_dim_body (sc_halo.hip) for one (field, plane cell): the
-// forward writes of the h layers are undone in reverse order; a copy
-// dst = src becomes l_src += l_dst, l_dst = 0; a DIRICHLET write drops the
-// halo adjoint; NEUMANN ghosts all derive from the edge cell.
-__global__ void __launch_bounds__(256) halo_adj_dim_kernel(const HaloArgs a, int d) {
+// Reverse of halo_dim_body (sc_halo.hip) for one plane cell, all fields (one
+// thread, so the density adjoints that primitive-BC fields add are not
+// raced): the forward writes of the h layers are undone in reverse order; a
+// copy dst = src becomes l_src += l_dst, l_dst = 0; a DIRICHLET write drops
+// the halo adjoint; NEUMANN ghosts all derive from the edge cell.  For a
+// field with its BC on the primitive the halo adjoints are primitive
+// adjoints (halo_adj_prim_kernel runs first); where the forward read a
+// source cell's primitive dnn(q, rho) (a cell not in the halo of an earlier
+// dim), the adjoint reaching it is converted back to q and rho (state: the
+// filled stage state).
+__global__ void __launch_bounds__(256)
+halo_adj_dim_kernel(const HaloArgs a, StatePtrs state, int d) {
   const int d_a = d == 0 ? 2 : (d == 1 ? 2 : 0);
   const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
   const int A = a.N[d_a], B = a.N[d_b];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)a.nf * A * B) return;
-  const int f = (int)(t / ((long long)A * B));
-  const int rem = (int)(t % ((long long)A * B));
-  const int ia = rem / B, ib = rem % B;
-  float* v = a.q[f];
+  if (t >= (long long)A * B) return;
+  const int ia = (int)(t / B), ib = (int)(t % B);
   const long long base = (long long)ia * a.st[d_a] + (long long)ib * a.st[d_b];
   const long long s = a.st[d];
   const int h = a.h, N = a.N[d], n = a.n[d];
-  auto at = [&](int c) -> float& { return v[base + (long long)c * s]; };
-  const BcFaceDev& lo = a.bc[f][d][0];
-  const BcFaceDev& hi = a.bc[f][d][1];
-  if (lo.type == kBcPeriodic || hi.type == kBcPeriodic) {
-    for (int i = h - 1; i >= 0; --i) {
-      at(h + i) += at(N - h + i);
-      at(N - h + i) = 0.0f;
-    }
-    for (int i = h - 1; i >= 0; --i) {
-      at(n + i) += at(i);
-      at(i) = 0.0f;
-    }
-    return;
+  // Sources of this line lie in the halo of an earlier dim (x before y
+  // before z) and then hold primitives already.
+  bool earlier_halo = false;
+  if (d >= 1) {
+    const int cx = d_a == 0 ? ia : ib;
+    earlier_halo |= (cx < h || cx >= h + a.n[0]);
   }
-  for (int side = 1; side >= 0; --side) {
-    const BcFaceDev& bc = side == 0 ? lo : hi;
-    float sum = 0.0f;
-    for (int i = 0; i < h; ++i) {
-      const int c = side == 0 ? h - 1 - i : N - h + i;
-      sum += at(c);
-      at(c) = 0.0f;
+  if (d == 2) earlier_halo |= (ib < h || ib >= h + a.n[1]);
+  for (int f = 0; f < a.nf; ++f) {
+    float* v = a.q[f];
+    const bool conv = ((a.prim_mask >> f) & 1) && !earlier_halo;
+    auto at = [&](int c) -> float& { return v[base + (long long)c * s]; };
+    // l_src += l (a primitive adjoint when conv: dnn(q, rho) at the source).
+    auto add_src = [&](int c, float l) {
+      const long long idx = base + (long long)c * s;
+      if (!conv) {
+        v[idx] += l;
+        return;
+      }
+      const float rho = state.p[0][idx];
+      if (rho == 0.0f) return;
+      const float ir = 1.0f / rho;
+      v[idx] += l * ir;
+      a.q[0][idx] -= l * state.p[f][idx] * ir * ir;
+    };
+    const BcFaceDev& lo = a.bc[f][d][0];
+    const BcFaceDev& hi = a.bc[f][d][1];
+    if (lo.type == kBcNone) continue;
+    if (lo.type == kBcPeriodic || hi.type == kBcPeriodic) {
+      for (int i = h - 1; i >= 0; --i) {
+        const float l = at(N - h + i);
+        at(N - h + i) = 0.0f;
+        add_src(h + i, l);
+      }
+      for (int i = h - 1; i >= 0; --i) {
+        const float l = at(i);
+        at(i) = 0.0f;
+        add_src(n + i, l);
+      }
+      continue;
     }
-    if (bc.type == kBcNeumann) at(side == 0 ? h : N - h - 1) += sum;
+    for (int side = 1; side >= 0; --side) {
+      const BcFaceDev& bc = side == 0 ? lo : hi;
+      float sum = 0.0f;
+      for (int i = 0; i < h; ++i) {
+        const int c = side == 0 ? h - 1 - i : N - h + i;
+        sum += at(c);
+        at(c) = 0.0f;
+      }
+      if (bc.type == kBcNeumann) add_src(side == 0 ? h : N - h - 1, sum);
+    }
+  }
+}
+
+// Reverse of halo_prim_finish_kernel: halo cells of primitive-BC fields hold
+// q = rho_h * prim_h; their adjoint becomes the primitive adjoint
+// l * rho_h, and l * prim_h joins the halo density's adjoint.
+__global__ void __launch_bounds__(256) halo_adj_prim_kernel(const HaloArgs a, StatePtrs state) {
+  const long long total = (long long)a.N[0] * a.N[1] * a.N[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int y = (int)(t % a.N[1]);
+  const int x = (int)((t / a.N[1]) % a.N[0]);
+  const int z = (int)(t / ((long long)a.N[1] * a.N[0]));
+  const int h = a.h;
+  const bool halo = x < h || x >= h + a.n[0] || y < h || y >= h + a.n[1] ||
+                    z < h || z >= h + a.n[2];
+  if (!halo) return;
+  const float r = state.p[0][t];
+  for (int f = 1; f < a.nf; ++f) {
+    if (!((a.prim_mask >> f) & 1)) continue;
+    const float l = a.q[f][t];
+    const float prim = r == 0.0f ? 0.0f : state.p[f][t] / r;
+    a.q[0][t] += l * prim;
+    a.q[f][t] = l * r;
   }
 }
 
@@ -160,11 +220,23 @@ hipError_t launch_adj_visc_gather(const AdjArgs& A, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_halo_adj_dim(const HaloArgs& a, int d, hipStream_t stream) {
+hipError_t launch_halo_adj_dim(const HaloArgs& a, const float* const* state, int d,
+                               hipStream_t stream) {
   const int d_a = d == 0 ? 2 : (d == 1 ? 2 : 0);
   const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
-  const long long n = (long long)a.nf * a.N[d_a] * a.N[d_b];
-  hipLaunchKernelGGL(halo_adj_dim_kernel, blocks_for(n, 1ll << 30), 256, 0, stream, a, d);
+  const long long n = (long long)a.N[d_a] * a.N[d_b];
+  StatePtrs st;
+  for (int f = 0; f < kMaxFields; ++f) st.p[f] = f < a.nf ? state[f] : nullptr;
+  hipLaunchKernelGGL(halo_adj_dim_kernel, blocks_for(n, 1ll << 30), 256, 0, stream, a, st, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_halo_adj_prim(const HaloArgs& a, const float* const* state,
+                                hipStream_t stream) {
+  const long long n = (long long)a.N[0] * a.N[1] * a.N[2];
+  StatePtrs st;
+  for (int f = 0; f < kMaxFields; ++f) st.p[f] = f < a.nf ? state[f] : nullptr;
+  hipLaunchKernelGGL(halo_adj_prim_kernel, blocks_for(n, 1ll << 30), 256, 0, stream, a, st);
   return hipGetLastError();
 }
 

@@ -566,8 +566,13 @@ hipError_t launch_adj_sweeps(const AdjArgs& A, const float h[3], int flux, bool 
 // Non-templated kernels of the adjoint (sc_adjoint.hip).
 hipError_t launch_adj_zero(float* const* p, int np, long long n, hipStream_t stream);
 hipError_t launch_adj_visc_gather(const AdjArgs& A, hipStream_t stream);
-// Transpose of the halo fill along d (lq in a.q); call for d = 2, 1, 0.
-hipError_t launch_halo_adj_dim(const HaloArgs& a, int d, hipStream_t stream);
+// Transpose of the halo fill along d (lq in a.q, the filled stage state in
+// state); call launch_halo_adj_prim first when a.prim_mask is set, then for
+// d = 2, 1, 0.
+hipError_t launch_halo_adj_dim(const HaloArgs& a, const float* const* state, int d,
+                               hipStream_t stream);
+hipError_t launch_halo_adj_prim(const HaloArgs& a, const float* const* state,
+                                hipStream_t stream);
 // Grid extents for the element-wise adjoint kernels.
 struct AdjGeom {
   int n[3], N[3], h;

@@ -867,8 +867,6 @@ sc_status sc_ctx_enable_adjoint(sc_ctx* c) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_enable_adjoint: null context");
   if (c->nf != 5)
     return fail(SC_ERR_UNSUPPORTED, "the adjoint supports the 5 base conserved fields only");
-  if (c->params.prim_bc_mask != 0)
-    return fail(SC_ERR_UNSUPPORTED, "the adjoint does not support primitive-variable BCs");
   if (c->adj_lq[0] != nullptr) return SC_OK;
   sc_status st = SC_OK;
   const size_t bytes = sizeof(float) * c->cells_full;
@@ -934,8 +932,12 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
   }
   sc::HaloArgs ha = c->halo;
   for (int f = 0; f < 5; ++f) ha.q[f] = lq[f];
+  if (ha.prim_mask != 0) {
+    st = hip_check(sc::launch_halo_adj_prim(ha, q, s), "adjoint halo prim kernel");
+    if (st != SC_OK) return st;
+  }
   for (int d = 2; d >= 0; --d) {
-    st = hip_check(sc::launch_halo_adj_dim(ha, d, s), "adjoint halo kernel");
+    st = hip_check(sc::launch_halo_adj_dim(ha, q, d, s), "adjoint halo kernel");
     if (st != SC_OK) return st;
   }
   return SC_OK;

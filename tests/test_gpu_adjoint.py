@@ -47,6 +47,19 @@ def _bcs(kind):
         'y': {0: (BC.DIRICHLET, [wall[k], wall[k] * 1.01, wall[k] * 1.02]),
               1: (BC.NEUMANN, 0.0)},
         'z': {0: (BC.NEUMANN, 0.0), 1: (BC.DIRICHLET, wall[k])}} for k in NAMES}}
+  if kind == 'prim_mixed':
+    # rho_u and rho_e with their BCs on the primitives u and e
+    # (boundary.py:516-531), the others on the conserved fields.
+    wall = {'rho': 1.1, 'u': 0.18, 'rho_v': 0.0, 'rho_w': 0.0, 'e': 5.5}
+    return {'cell_averages': {k: {
+        'x': {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)},
+        'y': {0: (BC.DIRICHLET, [wall[k], wall[k] * 1.01, wall[k] * 1.02]),
+              1: (BC.NEUMANN, 0.01)},
+        'z': {0: (BC.NEUMANN, -0.02), 1: (BC.DIRICHLET, wall[k])}} for k in wall}}
+  if kind == 'prim_neumann':
+    return {'cell_averages': {
+        k: {d: {0: (BC.NEUMANN, 0.01 * i), 1: (BC.NEUMANN, -0.01 * i)} for d in 'xyz'}
+        for i, k in enumerate(('rho', 'u', 'v', 'rho_w', 'e'))}}
   raise ValueError(kind)
 
 
@@ -113,6 +126,9 @@ RHS_CASES = [
     ('WENO_3', 'HLLC', True, 'mixed', False),
     ('MUSCL', 'HLL', True, 'mixed', True),
     ('MUSCL', 'HLLC', False, 'neumann', False),
+    ('WENO_5', 'HLLC', True, 'prim_mixed', False),
+    ('MUSCL', 'HLL', False, 'prim_neumann', True),
+    ('WENO_3', 'HLL', True, 'prim_neumann', False),
 ]
 
 
