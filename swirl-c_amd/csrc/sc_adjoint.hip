@@ -62,9 +62,19 @@ __global__ void __launch_bounds__(256) adj_visc_gather_kernel(const AdjArgs A) {
   A.lq.p[0][X] += lrho * inv;
 }
 
-// Reverse of halo_dim_body (sc_halo.hip) for one plane cell, all fields (one
-// thread, so the density adjoints that primitive-BC fields add are not
-// raced): the forward writes of the h layers are undone in reverse order; a
+// Number of thread units per plane cell of the transposed fill: unit 0 takes
+// the density and every field with its BC on the primitive (those add to the
+// density's adjoint, so one thread does them in a fixed order), each other
+// field a unit of its own.
+__host__ __device__ inline int halo_adj_units(const HaloArgs& a) {
+  int u = 1;
+  for (int f = 1; f < a.nf; ++f)
+    if (!((a.prim_mask >> f) & 1)) ++u;
+  return u;
+}
+
+// Reverse of halo_dim_body (sc_halo.hip) for one plane cell and one unit of
+// fields (halo_adj_units): the forward writes of the h layers are undone in reverse order; a
 // copy dst = src becomes l_src += l_dst, l_dst = 0; a DIRICHLET write drops
 // the halo adjoint; NEUMANN ghosts all derive from the edge cell.  For a
 // field with its BC on the primitive the halo adjoints are primitive
@@ -78,8 +88,23 @@ halo_adj_dim_kernel(const HaloArgs a, StatePtrs state, int d) {
   const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
   const int A = a.N[d_a], B = a.N[d_b];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)A * B) return;
-  const int ia = (int)(t / B), ib = (int)(t % B);
+  const int units = halo_adj_units(a);
+  if (t >= (long long)units * A * B) return;
+  const int unit = (int)(t / ((long long)A * B));
+  const int rem = (int)(t % ((long long)A * B));
+  const int ia = rem / B, ib = rem % B;
+  // Unit > 0: its single (non-primitive-BC) field.
+  int only = -1;
+  if (unit > 0) {
+    int u = 0;
+    for (int f = 1; f < a.nf; ++f) {
+      if ((a.prim_mask >> f) & 1) continue;
+      if (++u == unit) {
+        only = f;
+        break;
+      }
+    }
+  }
   const long long base = (long long)ia * a.st[d_a] + (long long)ib * a.st[d_b];
   const long long s = a.st[d];
   const int h = a.h, N = a.N[d], n = a.n[d];
@@ -92,8 +117,10 @@ halo_adj_dim_kernel(const HaloArgs a, StatePtrs state, int d) {
   }
   if (d == 2) earlier_halo |= (ib < h || ib >= h + a.n[1]);
   for (int f = 0; f < a.nf; ++f) {
+    const bool prim_f = (a.prim_mask >> f) & 1;
+    if (unit == 0 ? (f != 0 && !prim_f) : f != only) continue;
     float* v = a.q[f];
-    const bool conv = ((a.prim_mask >> f) & 1) && !earlier_halo;
+    const bool conv = prim_f && !earlier_halo;
     auto at = [&](int c) -> float& { return v[base + (long long)c * s]; };
     // l_src += l (a primitive adjoint when conv: dnn(q, rho) at the source).
     auto add_src = [&](int c, float l) {
@@ -224,7 +251,7 @@ hipError_t launch_halo_adj_dim(const HaloArgs& a, const float* const* state, int
                                hipStream_t stream) {
   const int d_a = d == 0 ? 2 : (d == 1 ? 2 : 0);
   const int d_b = d == 0 ? 1 : (d == 1 ? 0 : 1);
-  const long long n = (long long)a.N[d_a] * a.N[d_b];
+  const long long n = (long long)halo_adj_units(a) * a.N[d_a] * a.N[d_b];
   StatePtrs st;
   for (int f = 0; f < kMaxFields; ++f) st.p[f] = f < a.nf ? state[f] : nullptr;
   hipLaunchKernelGGL(halo_adj_dim_kernel, blocks_for(n, 1ll << 30), 256, 0, stream, a, st, d);
