@@ -336,8 +336,9 @@ sc_status sc_ctx_set_tuning(sc_ctx* ctx, const sc_tuning* tuning);
  * smoothness-indicator and weight derivatives included).  dt is a constant
  * (no derivative through a CFL time step).  Boundary face / flux values and
  * a reference state (sc_ctx_set_reference_state) and primitive-variable BCs
- * are differentiated too; restriction: 5 fields.  p_ref and rho_ref are
- * constants of the derivative.  Cotangents ("lam") are full-size field arrays of
+ * are differentiated too, and one or two transported scalars (without
+ * boundary values or a reference state).  p_ref and rho_ref are constants
+ * of the derivative.  Cotangents ("lam") are full-size field arrays of
  * which only interior cells are read; outputs have zero halo cells. */
 
 /* Allocates the context's adjoint work arrays (14 full-size fp32 fields);

@@ -10,9 +10,11 @@
 
 namespace sc {
 
-// Adjoint of cons_to_prim<0> (common/utils.py:145-183, generic.py:27-88):
-// w = (rho, u, v, w, e, p, h) from q = (rho, rho u, rho v, rho w, rho e).
-// Takes the forward primitives W (the velocities and energy are q * (1/rho)).
+// Adjoint of cons_to_prim<NS> (common/utils.py:145-183, generic.py:27-88):
+// w = (rho, u, v, w, e, p, h, y_s..) from q = (rho, rho u, rho v, rho w,
+// rho e, rho y_s..).  Takes the forward primitives W (the velocities, energy
+// and scalars are q * (1/rho)).
+template <int NS = 0>
 __device__ __forceinline__ void cons_to_prim_adj(const float* W, float pe, const float* lw,
                                                  float* lq) {
   const float rho = W[kRho], inv = frcp(rho);
@@ -41,6 +43,15 @@ __device__ __forceinline__ void cons_to_prim_adj(const float* W, float pe, const
   lq[3] += lww * inv;
   lq[4] += le * inv;
   linv += (lu * u + lv * v + lww * w + le * e) * rho;
+  if constexpr (NS > 0) {
+    float ls = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      lq[5 + s] += lw[7 + s] * inv;
+      ls += lw[7 + s] * W[7 + s];
+    }
+    linv += ls * rho;
+  }
   lrho += -linv * inv * inv;
   lq[0] += lrho;
 }
@@ -233,9 +244,10 @@ __device__ __forceinline__ void recon_adj<kMuscl>(const float* a, float lfp, flo
 // lF: adjoint of the 5 face fluxes; accumulates into lL, lR.
 // ---------------------------------------------------------------------------
 
-// Adjoint of physical_flux<D, 0, PD> (PD: the aligned momentum flux takes the
-// dynamic pressure s[7], the energy flux keeps p).
-template <int D, bool PD = false>
+// Adjoint of physical_flux<D, NS, PD> (PD: the aligned momentum flux takes
+// the dynamic pressure s[7 + NS], the energy flux keeps p; scalar s is
+// advected as m * y_s).
+template <int D, int NS = 0, bool PD = false>
 __device__ __forceinline__ void physical_flux_adj(const float* s, const float* lF, float* ls) {
   const float rho = s[kRho], vn = s[kU + D], ir = frcp(rho);
   const float m = rho * vn;
@@ -245,8 +257,13 @@ __device__ __forceinline__ void physical_flux_adj(const float* s, const float* l
     lm += lF[1 + k] * s[kU + k];
     ls[kU + k] += lF[1 + k] * m;
   }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    lm += lF[5 + k] * s[7 + k];
+    ls[7 + k] += lF[5 + k] * m;
+  }
   if constexpr (PD) {
-    ls[7] += lF[1 + D];
+    ls[7 + NS] += lF[1 + D];
     ls[kP] += lF[4] * m * ir;
   } else {
     ls[kP] += lF[1 + D] + lF[4] * m * ir;
@@ -328,7 +345,7 @@ __device__ __forceinline__ void roe_speeds_adj(const float* L, const float* R, f
 
 // Adjoint of hllc_star<D, 0> (flux f enters additively: its adjoint is lfs
 // itself and is handled by the caller).
-template <int D>
+template <int D, int NS = 0>
 __device__ __forceinline__ void hllc_star_adj(const float* S, float sk, float sm,
                                               const float* lfs, float* lS, float& lsk,
                                               float& lsm) {
@@ -356,6 +373,13 @@ __device__ __forceinline__ void hllc_star_adj(const float* S, float sk, float sm
       lS[kU + k] += lfs[1 + k] * a * q;
       lq += lfs[1 + k] * a * S[kU + k];
     }
+  }
+  // Scalars: fs = f + (a * y) * q.
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    la += lfs[5 + k] * S[7 + k] * q;
+    lS[7 + k] += lfs[5 + k] * a * q;
+    lq += lfs[5 + k] * a * S[7 + k];
   }
   {
     const float A = (rho * q) * (sm - vn);
@@ -388,30 +412,32 @@ __device__ __forceinline__ void hllc_star_adj(const float* S, float sk, float sm
 // PD: L0, R0 carry the dynamic pressure at index 7; the wave speeds, the
 // contact speed and the star state use it in place of p (hllc.py:109-120),
 // the physical flux both (physical_flux_adj<D, true>).
-template <int D, bool PD = false>
+template <int D, int NS = 0, bool PD = false>
 __device__ __forceinline__ void hllc_adj(const float* L0, const float* R0, float pe,
                                          const float* lF, float* lL0, float* lR0) {
-  constexpr int NV = PD ? 8 : 7;
-  float Lp[7], Rp[7];
+  constexpr int NB = NVar<NS>::value;  // primitives without p_dyn
+  constexpr int NV = NB + (PD ? 1 : 0);
+  constexpr int KD = 7 + NS;           // p_dyn's index
+  float Lp[NB], Rp[NB];
   if constexpr (PD) {
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
+    for (int v = 0; v < NB; ++v) {
       Lp[v] = L0[v];
       Rp[v] = R0[v];
     }
-    Lp[kP] = L0[7];
-    Rp[kP] = R0[7];
+    Lp[kP] = L0[KD];
+    Rp[kP] = R0[KD];
   }
   const float* L = PD ? Lp : L0;
   const float* R = PD ? Rp : R0;
   // Adjoints with respect to L, R (with PD: p stands for p_dyn); merged into
   // lL0, lR0 at the end.
-  float lLp[7], lRp[7];
+  float lLp[NB], lRp[NB];
   float* lL = lL0;
   float* lR = lR0;
   if constexpr (PD) {
 #pragma unroll
-    for (int v = 0; v < 7; ++v) lLp[v] = lRp[v] = 0.0f;
+    for (int v = 0; v < NB; ++v) lLp[v] = lRp[v] = 0.0f;
     lL = lLp;
     lR = lRp;
   }
@@ -441,40 +467,41 @@ __device__ __forceinline__ void hllc_adj(const float* L0, const float* R0, float
   if constexpr (PD) {
     // Physical flux of the side's own state (p and p_dyn), star correction
     // of the state with p_dyn for p.
-    float S0[NV], Ss[7], lS0[NV], lSs[7];
+    float S0[NV], Ss[NB], lS0[NV], lSs[NB];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       S0[v] = left ? L0[v] : R0[v];
       lS0[v] = 0.0f;
     }
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
+    for (int v = 0; v < NB; ++v) {
       Ss[v] = left ? L[v] : R[v];
       lSs[v] = 0.0f;
     }
-    physical_flux_adj<D, true>(S0, lF, lS0);
-    if (br == 1 || br == 2) hllc_star_adj<D>(Ss, left ? sl : sr, sm, lF, lSs, lsk, lsm);
+    physical_flux_adj<D, NS, true>(S0, lF, lS0);
+    if (br == 1 || br == 2)
+      hllc_star_adj<D, NS>(Ss, left ? sl : sr, sm, lF, lSs, lsk, lsm);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       lL0[v] += left ? lS0[v] : 0.0f;
       lR0[v] += left ? 0.0f : lS0[v];
     }
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
+    for (int v = 0; v < NB; ++v) {
       lL[v] += left ? lSs[v] : 0.0f;
       lR[v] += left ? 0.0f : lSs[v];
     }
   } else {
-    float S[7], lS[7];
+    float S[NB], lS[NB];
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
+    for (int v = 0; v < NB; ++v) {
       S[v] = left ? L[v] : R[v];
       lS[v] = 0.0f;
     }
-    physical_flux_adj<D>(S, lF, lS);
-    if (br == 1 || br == 2) hllc_star_adj<D>(S, left ? sl : sr, sm, lF, lS, lsk, lsm);
+    physical_flux_adj<D, NS>(S, lF, lS);
+    if (br == 1 || br == 2) hllc_star_adj<D, NS>(S, left ? sl : sr, sm, lF, lS, lsk, lsm);
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
+    for (int v = 0; v < NB; ++v) {
       lL[v] += left ? lS[v] : 0.0f;
       lR[v] += left ? 0.0f : lS[v];
     }
@@ -502,19 +529,20 @@ __device__ __forceinline__ void hllc_adj(const float* L0, const float* R0, float
   roe_speeds_adj<D>(L, R, pe, lsl, lsr, lL, lR);
   if constexpr (PD) {
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
-      const int dst = v == kP ? 7 : v;
+    for (int v = 0; v < NB; ++v) {
+      const int dst = v == kP ? KD : v;
       lL0[dst] += lLp[v];
       lR0[dst] += lRp[v];
     }
   }
 }
 
-// PD: the physical fluxes take p_dyn (index 7) in the aligned momentum flux;
-// the wave speeds keep p (hll.py:107-115).
-template <int D, bool PD = false>
+// PD: the physical fluxes take p_dyn (index 7 + NS) in the aligned momentum
+// flux; the wave speeds keep p (hll.py:107-115).
+template <int D, int NS = 0, bool PD = false>
 __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe,
                                         const float* lF, float* lL, float* lR) {
+  constexpr int NF = 5 + NS;
   float sl, sr;
   roe_speeds<D>(L, R, pe, sl, sr);
   const float ds = sr - sl;
@@ -524,38 +552,41 @@ __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe
   const float t2 = 1.0f - t1;
   const float n3 = sr * fabsf(sl) - sl * fabsf(sr);
   const float t3 = n3 * (0.5f * rds);
-  float fl[5], fr[5], ul[5], ur[5];
-  physical_flux<D, 0, PD>(L, fl);
-  physical_flux<D, 0, PD>(R, fr);
+  float fl[NF], fr[NF], ul[NF], ur[NF];
+  physical_flux<D, NS, PD>(L, fl);
+  physical_flux<D, NS, PD>(R, fr);
+  // Conserved face states: u_0 = rho, u_k = rho * phi_k (phi at primitive
+  // index k for k = 1..4, 7 + s for scalar s).
+  auto pidx = [](int k) { return k < 5 ? k : k + 2; };
   ul[0] = L[kRho];
   ur[0] = R[kRho];
 #pragma unroll
-  for (int k = 1; k < 5; ++k) {
-    ul[k] = L[kRho] * L[k];
-    ur[k] = R[kRho] * R[k];
+  for (int k = 1; k < NF; ++k) {
+    ul[k] = L[kRho] * L[pidx(k)];
+    ur[k] = R[kRho] * R[pidx(k)];
   }
-  float lt1 = 0.0f, lt2 = 0.0f, lt3 = 0.0f, lfl[5], lfr[5];
+  float lt1 = 0.0f, lt2 = 0.0f, lt3 = 0.0f, lfl[NF], lfr[NF];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < NF; ++k) {
     lt1 += lF[k] * fr[k];
     lt2 += lF[k] * fl[k];
     lt3 -= lF[k] * (ur[k] - ul[k]);
     lfr[k] = lF[k] * t1;
     lfl[k] = lF[k] * t2;
-    // conserved face states: u_0 = rho, u_k = rho * phi_k
     const float lur = -lF[k] * t3, lul = lF[k] * t3;
     if (k == 0) {
       lR[kRho] += lur;
       lL[kRho] += lul;
     } else {
-      lR[kRho] += lur * R[k];
-      lR[k] += lur * R[kRho];
-      lL[kRho] += lul * L[k];
-      lL[k] += lul * L[kRho];
+      const int pk = pidx(k);
+      lR[kRho] += lur * R[pk];
+      lR[pk] += lur * R[kRho];
+      lL[kRho] += lul * L[pk];
+      lL[pk] += lul * L[kRho];
     }
   }
-  physical_flux_adj<D, PD>(R, lfr, lR);
-  physical_flux_adj<D, PD>(L, lfl, lL);
+  physical_flux_adj<D, NS, PD>(R, lfr, lR);
+  physical_flux_adj<D, NS, PD>(L, lfl, lL);
   lt1 -= lt2;
   float lsl = 0.0f, lsr = 0.0f, lrds = lt1 * n1 + lt3 * 0.5f * n3;
   if (sr < 0.0f) lsr += lt1 * rds;
@@ -571,13 +602,13 @@ __device__ __forceinline__ void hll_adj(const float* L, const float* R, float pe
   roe_speeds_adj<D>(L, R, pe, lsl, lsr, lL, lR);
 }
 
-template <int FLUX, int D, bool PD = false>
+template <int FLUX, int D, int NS = 0, bool PD = false>
 __device__ __forceinline__ void riemann_adj(const float* L, const float* R, float pe,
                                             const float* lF, float* lL, float* lR) {
   if constexpr (FLUX == kHllc) {
-    hllc_adj<D, PD>(L, R, pe, lF, lL, lR);
+    hllc_adj<D, NS, PD>(L, R, pe, lF, lL, lR);
   } else {
-    hll_adj<D, PD>(L, R, pe, lF, lL, lR);
+    hll_adj<D, NS, PD>(L, R, pe, lF, lL, lR);
   }
 }
 

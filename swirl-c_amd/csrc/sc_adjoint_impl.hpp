@@ -43,19 +43,22 @@ struct AdjArgs {
 // adjoint of the cell's dynamic pressure p - p_ref (a constant 0 outside
 // the interior along the pass's direction).  The gravity source takes
 // rho - rho_ref with a reference state: the same derivative.
-template <bool PD = false>
+template <int NS = 0, bool PD = false>
 __device__ __forceinline__ void retire_cell(const AdjArgs& A, long long idx, const float* W,
                                             float pe, const float* lw, bool src,
                                             bool interior) {
-  float lq[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if constexpr (PD) {
-    float l7[7];
+  constexpr int NF = 5 + NS, NB = NVar<NS>::value;
+  float lq[NF];
 #pragma unroll
-    for (int v = 0; v < 7; ++v) l7[v] = lw[v];
-    if (interior) l7[kP] += lw[7];
-    cons_to_prim_adj(W, pe, l7, lq);
+  for (int f = 0; f < NF; ++f) lq[f] = 0.0f;
+  if constexpr (PD) {
+    float lb[NB];
+#pragma unroll
+    for (int v = 0; v < NB; ++v) lb[v] = lw[v];
+    if (interior) lb[kP] += lw[NB];
+    cons_to_prim_adj<NS>(W, pe, lb, lq);
   } else {
-    cons_to_prim_adj(W, pe, lw, lq);
+    cons_to_prim_adj<NS>(W, pe, lw, lq);
   }
   const SweepArgs& a = A.s;
   if (src && interior && a.grav_src) {
@@ -65,7 +68,7 @@ __device__ __forceinline__ void retire_cell(const AdjArgs& A, long long idx, con
       if (a.grav_mask & (1 << d)) lq[0] += A.g.p[1 + d][idx] * (kG * a.gdir[d]);
   }
 #pragma unroll
-  for (int f = 0; f < 5; ++f) A.lq.p[f][idx] += lq[f];
+  for (int f = 0; f < NF; ++f) A.lq.p[f][idx] += lq[f];
 }
 
 template <int D>
@@ -85,10 +88,11 @@ __device__ __forceinline__ void add_lG(const AdjArgs& A, long long idx, const fl
 // convective, diffusive or total flux, so that component's adjoint stops
 // there; a face value fixes a face-state component (primitive: a constant;
 // conserved: value / rho_face, whose derivative reaches the face density).
+template <int NF>
 __device__ __forceinline__ void flux_bc_adj(const BoundaryBc& b, const float* lF, float* lFc,
                                             float* lFd) {
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < NF; ++f) {
     const bool tot = (b.flux_mask >> (16 + f)) & 1;
     lFc[f] = (tot || ((b.flux_mask >> f) & 1)) ? 0.0f : lF[f];
     lFd[f] = (tot || ((b.flux_mask >> (8 + f)) & 1)) ? 0.0f : lF[f];
@@ -97,12 +101,13 @@ __device__ __forceinline__ void flux_bc_adj(const BoundaryBc& b, const float* lF
 
 // lW: adjoint of the face state after apply_face_bc; on return, of the
 // state before it (W, the state before it: its density is never replaced).
+template <int NS>
 __device__ __forceinline__ void face_bc_adj(const BoundaryBc& b, long long pidx,
                                             const float* W, float* lW) {
   for (int e = b.n_face - 1; e >= 0; --e) {
     const FaceEntry fe = b.face[e];
 #pragma unroll
-    for (int k = 1; k < 7; ++k) {
+    for (int k = 1; k < NVar<NS>::value; ++k) {
       if (k != fe.var) continue;
       if (fe.kind == 1 && W[kRho] != 0.0f) {
         const float val = fe.plane != nullptr ? fe.plane[pidx] : fe.val;
@@ -116,26 +121,26 @@ __device__ __forceinline__ void face_bc_adj(const BoundaryBc& b, long long pidx,
 // Riemann adjoint of one face with the boundary values of bb (or none):
 // L, R the reconstructed face states before the face values, lFc the
 // convective flux adjoint; lL, lR receive the state adjoints.
-template <int FLUX, int D, bool PD>
+template <int FLUX, int D, int NS, bool PD>
 __device__ __forceinline__ void face_riemann_adj(const BoundaryBc* bb, long long pidx,
                                                  const float* L, const float* R, float pe,
                                                  const float* lFc, float* lL, float* lR) {
   if (bb == nullptr) {
-    riemann_adj<FLUX, D, PD>(L, R, pe, lFc, lL, lR);
+    riemann_adj<FLUX, D, NS, PD>(L, R, pe, lFc, lL, lR);
     return;
   }
-  constexpr int NV = PD ? 8 : 7;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   float Lm[NV], Rm[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     Lm[v] = L[v];
     Rm[v] = R[v];
   }
-  apply_face_bc<0>(*bb, Lm, pidx);
-  apply_face_bc<0>(*bb, Rm, pidx);
-  riemann_adj<FLUX, D, PD>(Lm, Rm, pe, lFc, lL, lR);
-  face_bc_adj(*bb, pidx, L, lL);
-  face_bc_adj(*bb, pidx, R, lR);
+  apply_face_bc<NS>(*bb, Lm, pidx);
+  apply_face_bc<NS>(*bb, Rm, pidx);
+  riemann_adj<FLUX, D, NS, PD>(Lm, Rm, pe, lFc, lL, lR);
+  face_bc_adj<NS>(*bb, pidx, L, lL);
+  face_bc_adj<NS>(*bb, pidx, R, lR);
 }
 
 // ---------------------------------------------------------------------------
@@ -147,12 +152,13 @@ __device__ __forceinline__ void face_riemann_adj(const BoundaryBc* bb, long long
 // ---------------------------------------------------------------------------
 // BB: the build with boundary face / flux values; PD: with a reference state
 // (the dynamic pressure is an eighth reconstructed variable).
-template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
 __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
   constexpr int NW = WN + 1;  // window: cells c-1-RAD .. c+RAD
-  constexpr int NV = PD ? 8 : 7;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
+  constexpr int NF = 5 + NS;
   const SweepArgs& a = A.s;
   const int j = a.h + ti.bx * 64 + ti.tx;
   const int o = a.h + ti.by * ti.by_dim + ti.ty;
@@ -178,7 +184,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
     const bool in = interior(c);
     const long long idx = base + (long long)(in ? c : a.h) * sm;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) gv[f] = in ? A.g.p[f][idx] : 0.0f;
+    for (int f = 0; f < NF; ++f) gv[f] = in ? A.g.p[f][idx] : 0.0f;
   };
 
   // Primitive adjoints of the WN cells c-1-RAD .. c-1+RAD are accumulated
@@ -192,7 +198,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
   for (int w = 0; w < WN; ++w) {
     const int c = c_start - 1 - RAD + w;
-    load_prim<0, PD>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
+    load_prim<NS, PD>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
                  W[w]);
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc(w, v) = 0.0f;
@@ -210,7 +216,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
     recon_replay<SCHEME>(s, fpPrev[v], dummy);
     lRprev[v] = 0.0f;
   }
-  float gPrev[5];
+  float gPrev[NF];
   load_g(c_start - 1, gPrev);
   float tPrev[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
   float ltPrev[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
@@ -219,7 +225,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
   for (int c = c_start; c <= c_last; ++c) {
     {
       const int cn = min(c + RAD, NM - 1);
-      load_prim<0, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
+      load_prim<NS, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
                    pe_at(a, kk(cn), ii(cn), j), W[NW - 1]);
     }
     const bool face = c <= f_hi;
@@ -232,7 +238,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       for (int w = 0; w < WN; ++w) s[w] = W[w + 1][v];
       recon_replay<SCHEME>(s, fpC[v], fmC[v]);
     }
-    float gC[5];
+    float gC[NF];
     load_g(c, gC);
     float lL[NV], lR[NV];
 #pragma unroll
@@ -240,18 +246,18 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
     float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
     float ltCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
     if (face) {
-      float lF[5];
+      float lF[NF];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) lF[f] = (gC[f] - gPrev[f]) * a.inv_hdir;
+      for (int f = 0; f < NF; ++f) lF[f] = (gC[f] - gPrev[f]) * a.inv_hdir;
       const int side = c == a.h ? 0 : (c == a.h + n ? 1 : -1);
       const BoundaryBc* bb =
           (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
-      float lFc[5], lFd[5];
+      float lFc[NF], lFd[NF];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) lFc[f] = lFd[f] = lF[f];
-      if (bb != nullptr) flux_bc_adj(*bb, lF, lFc, lFd);
+      for (int f = 0; f < NF; ++f) lFc[f] = lFd[f] = lF[f];
+      if (bb != nullptr) flux_bc_adj<NF>(*bb, lF, lFc, lFd);
       // Boundary plane cell of this face: x faces [z][y], z faces [x][y].
-      face_riemann_adj<FLUX, DIR, PD>(bb, (long long)o * a.NY + j, fpPrev, fmC,
+      face_riemann_adj<FLUX, DIR, NS, PD>(bb, (long long)o * a.NY + j, fpPrev, fmC,
                                   pe_at(a, kk(c), ii(c), j), lFc, lL, lR);
       if constexpr (VISC) {
         transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
@@ -296,7 +302,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
         acc(s0, v) = 0.0f;
       }
       if (owned(x))
-        retire_cell<PD>(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), l0,
+        retire_cell<NS, PD>(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), l0,
                         DIR == 0,
                     interior(x));
       s0 = s0 + 1 < WN ? s0 + 1 : 0;
@@ -312,7 +318,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       lRprev[v] = lR[v];
     }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) gPrev[f] = gC[f];
+    for (int f = 0; f < NF; ++f) gPrev[f] = gC[f];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       tPrev[t][0] = tCur[t][0];
@@ -329,17 +335,17 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
     for (int v = 0; v < NV; ++v) l0[v] = acc(slot(w), v);
     if (owned(x))
-      retire_cell<PD>(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), l0,
+      retire_cell<NS, PD>(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), l0,
                       DIR == 0,
                   interior(x));
   }
 }
 
-template <int DIR, int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
 __global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_march_body<DIR, SCHEME, FLUX, VISC, BB, PD>(A, ti);
+  adj_march_body<DIR, SCHEME, FLUX, VISC, NS, BB, PD>(A, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -368,13 +374,14 @@ __host__ __device__ inline int y_adj_windows(int nx, int ny, int h, int rad, int
   return (int)((span + outa - 1) / outa);
 }
 
-template <int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
 __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = YAdjGeom<SCHEME>::RAD;
   constexpr int KA = YAdjGeom<SCHEME>::KA;
   constexpr int OUTA = YAdjGeom<SCHEME>::OUTA;
   constexpr int WN = 2 * RAD + 1;
-  constexpr int NV = PD ? 8 : 7;
+  constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
+  constexpr int NF = 5 + NS;
   const SweepArgs& a = A.s;
   const int lane = ti.tx;
   const int win = ti.bx * ti.by_dim + ti.ty;
@@ -407,7 +414,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   const float pe = pe_at(a, k, i, yc);
 
   float W0[NV];
-  load_prim<0, PD>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
+  load_prim<NS, PD>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
   float S[NV][WN], fp[NV], fm[NV], Am[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -419,14 +426,14 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     Am[v] = S[v][RAD - 1];
     recon_replay<SCHEME>(S[v], fp[v], fm[v]);
   }
-  float gC[5];
+  float gC[NF];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) gC[f] = inner ? A.g.p[f][idx] : 0.0f;
-  float L[NV], gP[5];
+  for (int f = 0; f < NF; ++f) gC[f] = inner ? A.g.p[f][idx] : 0.0f;
+  float L[NV], gP[NF];
 #pragma unroll
   for (int v = 0; v < NV; ++v) L[v] = wave_shr1(fp[v]);
 #pragma unroll
-  for (int f = 0; f < 5; ++f) gP[f] = wave_shr1(gC[f]);
+  for (int f = 0; f < NF; ++f) gP[f] = wave_shr1(gC[f]);
   float lL[NV], lR[NV], lA[NV], lB[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) lL[v] = lR[v] = lA[v] = lB[v] = 0.0f;
@@ -439,18 +446,18 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     tPrev[t][1] = wave_shr1(tCur[t][1]);
   }
   if (face) {
-    float lF[5];
+    float lF[NF];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) lF[f] = (gC[f] - gP[f]) * a.inv_hdir;
+    for (int f = 0; f < NF; ++f) lF[f] = (gC[f] - gP[f]) * a.inv_hdir;
     const int side = yc == a.h ? 0 : (yc == a.h + a.ny ? 1 : -1);
     const BoundaryBc* bb =
         (BB && ((a.bbc_dims >> 1) & 1) && side >= 0) ? a.bbc + 2 + side : nullptr;
-    float lFc[5], lFd[5];
+    float lFc[NF], lFd[NF];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) lFc[f] = lFd[f] = lF[f];
-    if (bb != nullptr) flux_bc_adj(*bb, lF, lFc, lFd);
+    for (int f = 0; f < NF; ++f) lFc[f] = lFd[f] = lF[f];
+    if (bb != nullptr) flux_bc_adj<NF>(*bb, lF, lFc, lFd);
     // Boundary plane cell of this face: y faces [z][x].
-    face_riemann_adj<FLUX, 1, PD>(bb, (long long)k * a.NX + i, L, fm, pe, lFc, lL, lR);
+    face_riemann_adj<FLUX, 1, NS, PD>(bb, (long long)k * a.NX + i, L, fm, pe, lFc, lL, lR);
     if constexpr (VISC) viscous_adj<1>(Am, W0, tPrev, tCur, a.vc, lFd, lA, lB, lta, ltb);
   }
   // Reconstruction adjoint of the own cell: left state of face y+1 (lane + 1)
@@ -483,23 +490,23 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     lt[t][1] = ltb[t][1] + wave_shl1(lta[t][1]);
   }
   if (out_lane) {
-    retire_cell<PD>(A, idx, W0, pe, tot, false, inner);
+    retire_cell<NS, PD>(A, idx, W0, pe, tot, false, inner);
     if constexpr (VISC) {
       if (rcell) add_lG<1>(A, idx, lt);
     }
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
 __global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_y_body<SCHEME, FLUX, VISC, BB, PD>(A, ti);
+  adj_y_body<SCHEME, FLUX, VISC, NS, BB, PD>(A, ti);
 }
 
 
 // Launches the three adjoint passes (x, y, z) for one scheme.
-template <int SCHEME, int FLUX, bool VISC, bool BB, bool PD>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
 hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& tune,
                                hipStream_t stream) {
   constexpr int RAD = Radius<SCHEME>::value;
@@ -509,13 +516,13 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
   const dim3 blk_m(64, by);
   const dim3 blk_y(64, by_y);
   // LDS ring of the marching passes: (2*RAD+1) cells x NV primitives per thread.
-  const size_t lds = sizeof(float) * (2 * RAD + 1) * (PD ? 8 : 7) * 64 * by;
+  const size_t lds = sizeof(float) * (2 * RAD + 1) * (NVar<NS>::value + (PD ? 1 : 0)) * 64 * by;
   {
     AdjArgs ax = A;
     ax.s.inv_hdir = 1.0f / h[0];
     const int ext = a.nx + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nz + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, BB, PD>), grid, blk_m, lds, stream,
+    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_m, lds, stream,
                        ax);
   }
   {
@@ -523,44 +530,50 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
     ay.s.inv_hdir = 1.0f / h[1];
     const int nwin = y_adj_windows(a.nx, a.ny, a.h, RAD, YAdjGeom<SCHEME>::OUTA);
     const dim3 grid((nwin + by_y - 1) / by_y, 1, a.nz);
-    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, BB, PD>), grid, blk_y, 0, stream, ay);
+    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_y, 0, stream, ay);
   }
   {
     AdjArgs az = A;
     az.s.inv_hdir = 1.0f / h[2];
     const int ext = a.nz + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nx + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, BB, PD>), grid, blk_m, lds, stream,
+    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_m, lds, stream,
                        az);
   }
   return hipGetLastError();
 }
 
+// Builds: boundary values (BB) and / or a reference state (PD) for the base
+// fields; one or two transported scalars without either.
 template <int SCHEME, int FLUX, bool VISC>
-hipError_t launch_adj_bb(const AdjArgs& A, const float h[3], const SweepTuning& tune,
-                         hipStream_t stream) {
+hipError_t launch_adj_variant(const AdjArgs& A, const float h[3], int ns,
+                              const SweepTuning& tune, hipStream_t stream) {
+  if (ns == 1) return launch_adj_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(A, h, tune, stream);
+  if (ns == 2) return launch_adj_sweeps_t<SCHEME, FLUX, VISC, 2, false, false>(A, h, tune, stream);
   const bool bb = A.s.bbc_dims != 0, pd = A.s.p_ref != nullptr;
   if (pd)
-    return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, true, true>(A, h, tune, stream)
-              : launch_adj_sweeps_t<SCHEME, FLUX, VISC, false, true>(A, h, tune, stream);
-  return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, true, false>(A, h, tune, stream)
-            : launch_adj_sweeps_t<SCHEME, FLUX, VISC, false, false>(A, h, tune, stream);
+    return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, true, true>(A, h, tune, stream)
+              : launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, false, true>(A, h, tune, stream);
+  return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, true, false>(A, h, tune, stream)
+            : launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, false, false>(A, h, tune, stream);
 }
 
+// ns: transported scalars (0..2; boundary values and a reference state with
+// ns = 0 only, checked by the caller).
 template <int SCHEME>
-hipError_t launch_adj_sweeps(const AdjArgs& A, const float h[3], int flux, bool visc,
+hipError_t launch_adj_sweeps(const AdjArgs& A, const float h[3], int flux, bool visc, int ns,
                              const SweepTuning& tune, hipStream_t stream);
 
 #define SC_INSTANTIATE_ADJOINT(SCHEME)                                                    \
   template <>                                                                             \
   hipError_t launch_adj_sweeps<SCHEME>(const AdjArgs& A, const float h[3], int flux,      \
-                                       bool visc, const SweepTuning& tune,                \
+                                       bool visc, int ns, const SweepTuning& tune,        \
                                        hipStream_t stream) {                              \
     if (flux == kHllc)                                                                    \
-      return visc ? launch_adj_bb<SCHEME, kHllc, true>(A, h, tune, stream)                \
-                  : launch_adj_bb<SCHEME, kHllc, false>(A, h, tune, stream);              \
-    return visc ? launch_adj_bb<SCHEME, kHll, true>(A, h, tune, stream)                   \
-                : launch_adj_bb<SCHEME, kHll, false>(A, h, tune, stream);                 \
+      return visc ? launch_adj_variant<SCHEME, kHllc, true>(A, h, ns, tune, stream)       \
+                  : launch_adj_variant<SCHEME, kHllc, false>(A, h, ns, tune, stream);     \
+    return visc ? launch_adj_variant<SCHEME, kHll, true>(A, h, ns, tune, stream)          \
+                : launch_adj_variant<SCHEME, kHll, false>(A, h, ns, tune, stream);        \
   }
 
 // Non-templated kernels of the adjoint (sc_adjoint.hip).

@@ -39,7 +39,7 @@ struct sc_ctx {
   int bbc_dims;
   // adjoint work arrays (sc_ctx_enable_adjoint): state adjoint and the nine
   // velocity-gradient adjoints
-  float* adj_lq[5];
+  float* adj_lq[SC_MAX_FIELDS];
   float* adj_lG[9];
   sc::HaloArgs halo;  // BC tables; field pointers filled per call
   int halo_all_periodic;  // one-launch periodic fill applies
@@ -399,7 +399,7 @@ sc_status sc_ctx_destroy(sc_ctx* c) {
     if (c->dz_scratch[f]) hipFree(c->dz_scratch[f]);
   if (c->speed_bits) hipFree(c->speed_bits);
   if (c->bbc_dev) hipFree(c->bbc_dev);
-  for (int f = 0; f < 5; ++f)
+  for (int f = 0; f < SC_MAX_FIELDS; ++f)
     if (c->adj_lq[f]) hipFree(c->adj_lq[f]);
   for (int f = 0; f < 9; ++f)
     if (c->adj_lG[f]) hipFree(c->adj_lG[f]);
@@ -865,17 +865,18 @@ sc_status sc_ctx_set_tuning(sc_ctx* c, const sc_tuning* t) {
 
 sc_status sc_ctx_enable_adjoint(sc_ctx* c) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_ctx_enable_adjoint: null context");
-  if (c->nf != 5)
-    return fail(SC_ERR_UNSUPPORTED, "the adjoint supports the 5 base conserved fields only");
+  if (c->nf > 5 && c->bbc_dims != 0)
+    return fail(SC_ERR_UNSUPPORTED,
+                "the adjoint takes boundary face / flux values without transported scalars only");
   if (c->adj_lq[0] != nullptr) return SC_OK;
   sc_status st = SC_OK;
   const size_t bytes = sizeof(float) * c->cells_full;
-  for (int f = 0; f < 5 && st == SC_OK; ++f)
+  for (int f = 0; f < c->nf && st == SC_OK; ++f)
     st = hip_check(hipMalloc((void**)&c->adj_lq[f], bytes), "hipMalloc adjoint");
   for (int f = 0; f < 9 && st == SC_OK; ++f)
     st = hip_check(hipMalloc((void**)&c->adj_lG[f], bytes), "hipMalloc adjoint");
   if (st != SC_OK) {
-    for (int f = 0; f < 5; ++f) {
+    for (int f = 0; f < SC_MAX_FIELDS; ++f) {
       if (c->adj_lq[f]) hipFree(c->adj_lq[f]);
       c->adj_lq[f] = nullptr;
     }
@@ -897,9 +898,9 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
   sc_status st = sc_fill_halos(c, q, s);
   if (st != SC_OK) return st;
   const bool visc = c->params.include_diffusion != 0;
-  float* zero[14];
+  float* zero[SC_MAX_FIELDS + 9];
   int nz = 0;
-  for (int f = 0; f < 5; ++f) zero[nz++] = lq[f];
+  for (int f = 0; f < c->nf; ++f) zero[nz++] = lq[f];
   if (visc)
     for (int f = 0; f < 9; ++f) zero[nz++] = c->adj_lG[f];
   st = hip_check(sc::launch_adj_zero(zero, nz, c->cells_full, s), "adjoint clear");
@@ -907,7 +908,7 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
   sc::AdjArgs A;
   memset(&A, 0, sizeof(A));
   A.s = c->sweep;
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < c->nf; ++f) {
     A.s.q.p[f] = q[f];
     A.g.p[f] = g[f];
     A.lq.p[f] = lq[f];
@@ -917,12 +918,13 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
   hipError_t e;
   const int scheme = c->params.interpolation;
   const int flux = c->params.numeric_flux;
+  const int ns = c->nf - 5;
   if (scheme == SC_WENO_5) {
-    e = sc::launch_adj_sweeps<sc::kWeno5>(A, c->h, flux, visc, c->tune, s);
+    e = sc::launch_adj_sweeps<sc::kWeno5>(A, c->h, flux, visc, ns, c->tune, s);
   } else if (scheme == SC_WENO_3) {
-    e = sc::launch_adj_sweeps<sc::kWeno3>(A, c->h, flux, visc, c->tune, s);
+    e = sc::launch_adj_sweeps<sc::kWeno3>(A, c->h, flux, visc, ns, c->tune, s);
   } else {
-    e = sc::launch_adj_sweeps<sc::kMuscl>(A, c->h, flux, visc, c->tune, s);
+    e = sc::launch_adj_sweeps<sc::kMuscl>(A, c->h, flux, visc, ns, c->tune, s);
   }
   st = hip_check(e, "adjoint sweep kernels");
   if (st != SC_OK) return st;
@@ -931,7 +933,7 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
     if (st != SC_OK) return st;
   }
   sc::HaloArgs ha = c->halo;
-  for (int f = 0; f < 5; ++f) ha.q[f] = lq[f];
+  for (int f = 0; f < c->nf; ++f) ha.q[f] = lq[f];
   if (ha.prim_mask != 0) {
     st = hip_check(sc::launch_halo_adj_prim(ha, q, s), "adjoint halo prim kernel");
     if (st != SC_OK) return st;
@@ -946,6 +948,10 @@ sc_status run_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* 
 sc_status check_adjoint_ready(sc_ctx* c, const char* who) {
   if (c->adj_lq[0] == nullptr)
     return fail(SC_ERR_INVALID, std::string(who) + ": call sc_ctx_enable_adjoint first");
+  if (c->nf > 5 && c->sweep.p_ref != nullptr)
+    return fail(SC_ERR_UNSUPPORTED, std::string(who) +
+                                        ": the adjoint takes a reference state without"
+                                        " transported scalars only");
   return SC_OK;
 }
 
@@ -959,10 +965,10 @@ sc_status sc_rhs_vjp(sc_ctx* c, float* const* q, const float* const* g, float* c
     return fail(SC_ERR_INVALID, "sc_rhs_vjp: null argument");
   sc_status st = check_adjoint_ready(c, "sc_rhs_vjp");
   if (st != SC_OK) return st;
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < c->nf; ++f) {
     if (q[f] == nullptr || g[f] == nullptr || lq[f] == nullptr)
       return fail(SC_ERR_INVALID, "sc_rhs_vjp: null field pointer");
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < c->nf; ++k)
       if (lq[f] == q[k] || lq[f] == g[k])
         return fail(SC_ERR_INVALID, "sc_rhs_vjp: lq must not alias q or g");
   }
@@ -979,12 +985,12 @@ sc_status sc_rk_stage_vjp(sc_ctx* c, int form, float c0, float c1, const float* 
   if (st != SC_OK) return st;
   if (form != SC_RK_EULER && form != SC_RK_BLEND)
     return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: form must be SC_RK_EULER or SC_RK_BLEND");
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < c->nf; ++f) {
     if (uk[f] == nullptr || lam_out[f] == nullptr || lam_uk[f] == nullptr)
       return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: null field pointer");
     if (lam_u0 != nullptr && lam_u0[f] == nullptr)
       return fail(SC_ERR_INVALID, "sc_rk_stage_vjp: null lam_u0 field pointer");
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < c->nf; ++k) {
       if (lam_uk[f] == uk[k] || (lam_u0 != nullptr && (lam_u0[f] == uk[k] ||
                                                         lam_u0[f] == lam_out[k] ||
                                                         lam_u0[f] == lam_uk[k])))
@@ -1002,7 +1008,7 @@ sc_status sc_rk_stage_vjp(sc_ctx* c, int form, float c0, float c1, const float* 
   for (int d = 0; d < 3; ++d) geo.N[d] = c->N[d];
   const bool blend = form == SC_RK_BLEND;
   return hip_check(sc::launch_rk_adj_combine(lam_out, c->adj_lq, lam_uk,
-                                             blend ? lam_u0 : nullptr, 5, geo,
+                                             blend ? lam_u0 : nullptr, c->nf, geo,
                                              blend ? c0 : 0.0f, blend ? c1 : 1.0f, dt_dev, s),
                    "adjoint RK combine");
 }
