@@ -617,11 +617,72 @@ class ChunkedContext:
     for part in self.parts:
       part.set_tuning(tuning)
 
-  def _unsupported(self, *args, **kwargs):
-    raise NotImplementedError('single-kernel timing and the adjoint take at most two'
-                              ' transported scalars')
+  # -- Adjoint: the base fields' RHS does not depend on the scalars, and a
+  # scalar's RHS depends on the base fields and itself, so part c's
+  # vector-Jacobian product with the base cotangent set to zero (c > 0)
+  # gives exactly its scalars' contribution to the base adjoint, which is
+  # summed over the parts.
 
-  sweep_only = sweep_only_timed = enable_adjoint = rhs_vjp = rk_stage_vjp = _unsupported
+  def _adj_buffers(self):
+    if getattr(self, '_adj', None) is None:
+      mk = lambda: [torch.zeros(self.shape, dtype=torch.float32, device=self.device)
+                    for _ in range(5)]
+      self._adj = {'zero': mk(), 'sink': mk(), 'acc': mk()}
+    return self._adj
+
+  def enable_adjoint(self):
+    for part in self.parts:
+      part.enable_adjoint()
+    self._adj_buffers()
+
+  def _with_base(self, fields, c, base):
+    return list(base) + list(fields[5 + 2 * c:7 + 2 * c])
+
+  def rhs_vjp(self, q, g, lq, stream=None):
+    """lq = (d rhs / d q)^T g over all fields (see NativeContext.rhs_vjp)."""
+    self.enable_adjoint()
+    b = self._adj
+    for t in b['acc']:
+      t.zero_()
+    for c in self._order():
+      if c == 0:
+        continue
+      self.parts[c].rhs_vjp(self._split(q, c), self._with_base(g, c, b['zero']),
+                            self._with_base(lq, c, b['sink']), stream)
+      for a, s in zip(b['acc'], b['sink']):
+        a.add_(s)
+    self.parts[0].rhs_vjp(self._split(q, 0), self._split(g, 0), self._split(lq, 0), stream)
+    for t, a in zip(lq[:5], b['acc']):
+      t.add_(a)
+
+  def rk_stage_vjp(self, form, c0, c1, dt_dev, uk, lam_out, lam_uk, lam_u0=None,
+                   stream=None):
+    """Reverse of rk_stage over all fields (see NativeContext.rk_stage_vjp);
+    lam_uk may alias lam_out.  The parts after the first run first, with a
+    zero base cotangent, so lam_out's base fields are read before they are
+    overwritten."""
+    self.enable_adjoint()
+    b = self._adj
+    for t in b['acc']:
+      t.zero_()
+    for c in self._order():
+      if c == 0:
+        continue
+      self.parts[c].rk_stage_vjp(form, c0, c1, dt_dev, self._split(uk, c),
+                                 self._with_base(lam_out, c, b['zero']),
+                                 self._with_base(lam_uk, c, b['sink']),
+                                 self._split(lam_u0, c), stream)
+      for a, s in zip(b['acc'], b['sink']):
+        a.add_(s)
+    self.parts[0].rk_stage_vjp(form, c0, c1, dt_dev, self._split(uk, 0), self._split(lam_out, 0),
+                               self._split(lam_uk, 0), self._split(lam_u0, 0), stream)
+    for t, a in zip(lam_uk[:5], b['acc']):
+      t.add_(a)
+
+  def _unsupported(self, *args, **kwargs):
+    raise NotImplementedError('single-kernel timing takes at most two transported scalars')
+
+  sweep_only = sweep_only_timed = _unsupported
 
 
 def make_context(cfg, device=None, create=True):

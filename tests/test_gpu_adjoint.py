@@ -19,6 +19,7 @@ import pytest
 import torch
 
 from oracle import swirlc_oracle as O
+from swirlc_amd import _native as N
 from swirlc_amd.boundary import bc_types
 from swirlc_amd.core import adjoint as A
 from swirlc_amd.core.parameter import SwirlCParameters
@@ -270,12 +271,15 @@ def test_rhs_vjp_with_reference_state_matches_fp64_finite_differences(scheme, fl
     (2, 'WENO_5', 'HLL', False, 'bounded'),
     (2, 'WENO_3', 'HLLC', True, 'bounded'),
     (1, 'MUSCL', 'HLL', True, 'periodic'),
+    (3, 'WENO_5', 'HLLC', True, 'periodic'),
+    (4, 'WENO_3', 'HLL', False, 'bounded'),
 ])
 def test_rhs_vjp_with_scalars_matches_fp64_finite_differences(nscalar, scheme, flux, visc,
                                                                bcs):
   """Transported scalars (compressible_navier_stokes.py:419-427, hllc.py:182-192,
   hll.py): advected as m * y with their own reconstruction; 'bounded' gives
-  the first scalar its BC on the primitive."""
+  the first scalar its BC on the primitive.  Three and four scalars go
+  through the chunked context (parts of two scalars each)."""
   import test_gpu_scalars as SC
   n, h = (12, 10, 9), 3
   names = SC._names(nscalar)
@@ -328,6 +332,65 @@ def test_rhs_vjp_with_scalars_matches_fp64_finite_differences(nscalar, scheme, f
       break
   print(f'[adjoint]   rel errors {["%.1e" % e for e in errs]}, {unsettled} unsettled',
         flush=True)
+  assert len(errs) == 4
+  assert max(errs) <= FD_TOL
+
+
+def test_chunked_rk_stage_vjp_matches_fp64_finite_differences():
+  """rk_stage_vjp of a context with three scalars (two parts), BLEND form with
+  lam_uk aliasing lam_out as the rollout calls it: lam_uk = c1 (lam +
+  dt J^T lam), lam_u0 += c0 lam, against FD of the oracle's stage."""
+  import test_gpu_scalars as SC
+  n, h, nscalar = (12, 10, 9), 3, 3
+  names = SC._names(nscalar)
+  cfg = SC._cfg(n, 'WENO_5', 'HLLC', True, 'periodic', nscalar)
+  cfg['transport_parameters'] = {'nu': 1e-2, 'pr': 0.7}
+  st = random_state(n, h, seed=6, jump=False, nscalar=nscalar)
+  sl = (slice(h, -h),) * 3
+  rng = np.random.default_rng(15)
+  lam = {k: np.zeros_like(st[k]) for k in names}
+  for k in names:
+    lam[k][sl] = rng.standard_normal(lam[k][sl].shape).astype(np.float32)
+  ctx = _ctx(cfg)
+  assert len(ctx.parts) == 2
+  gpu = lambda d: [torch.from_numpy(np.ascontiguousarray(d[k])).cuda() for k in names]
+  dt, c0, c1 = 2e-3, 0.75, 0.25
+  dt_dev = torch.full((1,), dt, dtype=torch.float32, device='cuda')
+  uk = gpu(st)
+  l = gpu(lam)
+  l0 = [torch.zeros_like(t) for t in l]
+  ctx.rk_stage_vjp(N.SC_RK_BLEND, c0, c1, dt_dev, uk, l, l, l0)
+  torch.cuda.synchronize()
+  grad = [t.cpu().numpy() for t in l]
+  for t, k in zip(l0, names):
+    np.testing.assert_allclose(t.cpu().numpy()[sl], c0 * lam[k][sl], rtol=1e-6)
+  ocfg = oracle_cfg(cfg)
+
+  def J(s):
+    r = O.rhs(s, {}, ocfg)
+    return sum(float(np.sum(lam[k][sl].astype(np.float64) *
+                            (c1 * (s[k][sl] + dt * r[k][sl])))) for k in names)
+
+  def fd_at(v, eps):
+    sp = {k: st[k].astype(np.float64) + eps * v[k] for k in names}
+    sm = {k: st[k].astype(np.float64) - eps * v[k] for k in names}
+    return (J(sp) - J(sm)) / (2.0 * eps)
+
+  drng = np.random.default_rng(25)
+  errs = []
+  for _ in range(8):
+    v = {k: np.zeros(st[k].shape) for k in names}
+    for k in names:
+      v[k][sl] = drng.standard_normal(v[k][sl].shape) * np.abs(st[k][sl]).mean()
+    vals = [fd_at(v, e) for e in (1e-6, 1e-7, 1e-8)]
+    fd = next((a for a, b in zip(vals, vals[1:]) if abs(a - b) <= 1e-5 * abs(b)), None)
+    if fd is None:
+      continue
+    ad = sum(float(np.sum(grad[i].astype(np.float64) * v[k])) for i, k in enumerate(names))
+    errs.append(abs(ad - fd) / abs(fd))
+    if len(errs) == 4:
+      break
+  print(f'[adjoint]   chunked rk stage rel errors {["%.1e" % e for e in errs]}', flush=True)
   assert len(errs) == 4
   assert max(errs) <= FD_TOL
 
