@@ -69,6 +69,10 @@ class BatchSolver:
       if probe.gravity_dir != -1:
         if len(grid) >= 9:
           axes = grid[6:9]
+          for c, m, dim in zip(axes, (nx, ny, nz), 'xyz'):
+            if np.asarray(c).reshape(-1).size != m:
+              raise ValueError(f'instance {k}: {dim} coordinates have '
+                               f'{np.asarray(c).size} entries, expected {m}')
         else:
           axes = [_get_cell_center_coordinates(0.0, sp * m, m)
                   for m, sp in ((nx, dx), (ny, dy), (nz, dz))]

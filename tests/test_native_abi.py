@@ -197,3 +197,16 @@ def test_plane_values_change_the_context():
   k2 = NativeContext(SwirlCParameters(BV._cfg(n, 'WENO_5', 'HLLC', False, faces=f2)),
                      create=False).cache_key()
   assert k1 == k1b and k1 != k2
+
+
+def test_batch_rejects_coordinates_of_the_wrong_length():
+  """A gravity ensemble's instance coordinates are read by the host over the
+  instance's extents: a wrong length is refused before anything is built."""
+  import test_gpu_batch as TB
+  from swirlc_amd.core.batch import BatchSolver
+  from swirlc_amd.core.parameter import SwirlCParameters
+  shape = (8, 6, 5)
+  cfg = TB._wide_cfg(shape, True, False)
+  grid = (*shape, cfg['dx'], cfg['dy'], cfg['dz'], cfg['x'], cfg['y'][:-1], cfg['z'])
+  with pytest.raises(ValueError, match='y coordinates'):
+    BatchSolver(SwirlCParameters(cfg), [(grid, {})], cfl=None)
