@@ -29,11 +29,12 @@ for _p in (ROOT, os.path.join(ROOT, 'swirl-c_amd')):
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # Algorithmic bytes per interior cell per launch of each sweep kernel (fp32,
-# 5 conserved fields), in launch order: x reads U, writes R (40); z reads U,
-# writes its divergence Dz (40); y (last) reads U, R and Dz, in RK3 stages 2
-# and 3 also U0, and writes the new state (80 at stage 1, 100 at stages 2-3).
-# A step: 160 + 180 + 180 = 520 B per cell.
-SWEEP_BYTES = {'x': (40, 40, 40), 'z': (40, 40, 40), 'y': (80, 100, 100)}
+# 5 conserved fields), in launch order: x reads U, writes R (40); y reads U
+# and R, writes R (60); z (last) reads U and R, in RK3 stages 2 and 3 also
+# U0, and writes the new state (60 at stage 1, 80 at stages 2-3; the stage
+# state at the output cell is the U it reads).
+# A step: 120 + 180 + 220 = 520 B per cell.
+SWEEP_BYTES = {'x': (40, 40, 40), 'y': (60, 60, 60), 'z': (60, 80, 80)}
 STEP_BYTES = 520
 # The same step with a separate viscous-flux kernel (read U, read and write
 # R: 60 B per stage) would move 700 B per cell.
@@ -120,7 +121,7 @@ def parse():
 
 def time_step_kernels(solver, steps):
   """Kernel durations inside RK3 steps: a step's launch sequence (CFL wave
-  speed, then per stage the halo fill and the x, z, y sweeps, the last one
+  speed, then per stage the halo fill and the x, y, z sweeps, the last one
   with the stage update) issued through the C ABI on one stream.  Each sweep
   launch records its own start and stop times into HIP events
   (sc_sweep_only_timed), so nothing is inserted between the kernels; the
@@ -134,7 +135,7 @@ def time_step_kernels(solver, steps):
   u, s1, s2 = solver.u, solver.s1, solver.s2
   stages = ((N.SC_RK_EULER, 0.0, 0.0, None, u, s1), (N.SC_RK_BLEND, 0.75, 0.25, u, s1, s2),
             (N.SC_RK_BLEND, 1.0 / 3.0, 2.0 / 3.0, u, s2, u))
-  names = ('halo', 'sweep_x', 'sweep_z', 'sweep_y')
+  names = ('halo', 'sweep_x', 'sweep_y', 'sweep_z')
   acc = {n: [0.0, 0.0, 0.0] for n in names}
   acc['max_speed'] = [0.0]
   total = 0.0
@@ -157,7 +158,7 @@ def time_step_kernels(solver, steps):
       halo[st][0].record(stream)
       ctx.fill_halos(uk, stream)
       halo[st][1].record(stream)
-      for j, d in enumerate((0, 2, 1)):
+      for j, d in enumerate((0, 1, 2)):
         e0, e1 = sweeps[st][j]
         ctx.sweep_only_timed(d, form, c0, c1, solver.dt, base, uk, out, e0, e1, stream)
     t_end.record(stream)
@@ -165,7 +166,7 @@ def time_step_kernels(solver, steps):
     acc['max_speed'][0] += t_begin.elapsed_time(t_speed)
     for st in range(3):
       acc['halo'][st] += halo[st][0].elapsed_time(halo[st][1])
-      for j, n in enumerate(('sweep_x', 'sweep_z', 'sweep_y')):
+      for j, n in enumerate(('sweep_x', 'sweep_y', 'sweep_z')):
         e0, e1 = sweeps[st][j]
         acc[n][st] += e0.elapsed_time(e1)
     total += t_begin.elapsed_time(t_end)
@@ -182,8 +183,8 @@ def time_sweeps(solver, reps):
   ctx.fill_halos(solver.u)
   out = {}
   # RK3 stage-2 form with U0 (the last step's stage-1 state) distinct from Uk,
-  # as in a real stage; the y sweep reads the x and z sweeps' outputs.
-  for d in (0, 2):
+  # as in a real stage; the z sweep reads the x and y sweeps' outputs.
+  for d in (0, 1):
     ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u, solver.s2,
                    stream)
   for d, name in enumerate(('x', 'y', 'z')):
@@ -203,7 +204,7 @@ def time_sweeps(solver, reps):
 
 def pmc_probe(n):
   """Kernel launches for HBM-traffic counters: a copy of 64 Mi floats (256 MiB
-  read + 256 MiB written: the counters' calibration), then the x, z and y
+  read + 256 MiB written: the counters' calibration), then the x, y and z
   sweeps (RK3 stage-2 form, U0 distinct from Uk) twice each on the n^3
   Kelvin-Helmholtz state in the default launch geometry with the XCD-aware
   block order on, then the same six launches with it off (the last twelve
@@ -229,11 +230,11 @@ def pmc_probe(n):
   for remap in (1, 0):
     ctx.set_tuning({'xcd_remap': remap})
     print(f'[pmc-probe] tuning {ctx.get_tuning()}', flush=True)
-    for d in (0, 2, 1):
+    for d in (0, 1, 2):
       for _ in range(2):
         ctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, solver.dt, solver.s1, solver.u, solver.s2)
     torch.cuda.synchronize()
-  print(f'[pmc-probe] copy 268435456 B each way; sweeps x, z, y x2 at {n}^3, '
+  print(f'[pmc-probe] copy 268435456 B each way; sweeps x, y, z x2 at {n}^3, '
         'XCD-aware block order on, then off', flush=True)
 
 

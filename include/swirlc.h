@@ -172,11 +172,11 @@ sc_status sc_rk_stage(sc_ctx* ctx, int form, float c0, float c1,
 /* No counterpart in the reference: sc_rk_stage for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  Identical results (same kernels); the
- * RHS accumulator and z-divergence live in a file-scope 128 MiB device
- * array, one per GPU, held until the process ends, instead of the context's
- * scratch fields.  Limit: 2 * n_fields * ((nx+2h)(ny+2h)(nz+2h) rounded up
- * to 64) <= 33554432 floats (e.g. 5 fields of a 128^3 grid with halo 3),
- * else SC_ERR_UNSUPPORTED.  The
+ * RHS accumulator lives in a file-scope 128 MiB device array, one per GPU,
+ * held until the process ends, instead of the context's scratch fields.
+ * Limit: n_fields * ((nx+2h)(ny+2h)(nz+2h) rounded up to 64) <= 33554432
+ * floats (e.g. 5 fields of a 180^3 grid with halo 3), else
+ * SC_ERR_UNSUPPORTED.  The
  * context's device must be current.  Allocates nothing; needs no zeroing
  * (the x sweep writes every interior accumulator cell before it is read).
  * Every call on one GPU shares that GPU's array: calls that overlap on one
@@ -285,8 +285,8 @@ sc_status sc_hllc_branch_codes(sc_ctx* ctx, const float* const* q, int dim,
 
 /* Profiling: launches only the sweep kernel of direction dim (0 x, 1 y, 2 z)
  * of an sc_rk_stage call (form 0 = plain RHS into the context's scratch), on
- * uk whose halos are already filled.  A stage runs x, z, y in that order: the
- * x sweep writes the scratch RHS, the z sweep updates it, the y sweep
+ * uk whose halos are already filled.  A stage runs x, y, z in that order: the
+ * x sweep writes the scratch RHS, the y sweep updates it, the z sweep
  * finishes it (and writes out for form != 0).
  * No reference counterpart: it exposes one kernel of the fused stage so
  * its duration can be timed in isolation. */

@@ -29,10 +29,8 @@ struct sc_ctx {
   float* pe_y;
   float* pe_z;
   int grav_mask;
-  // scratch RHS accumulator and z-divergence (device), one full-size array
-  // per field each
+  // scratch RHS accumulator (device), one full-size array per field
   float* rhs_scratch[SC_MAX_FIELDS];
-  float* dz_scratch[SC_MAX_FIELDS];
   unsigned* speed_bits;
   // boundary face / flux values per [dim][side] (device) and dims using them
   sc::BoundaryBc* bbc_dev;
@@ -301,7 +299,6 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
   if (p.gravity_source) c->grav_mask = gdir == 3 ? 7 : (1 << gdir);
   for (int f = 0; f < c->nf && st == SC_OK; ++f) {
     check(hipMalloc((void**)&c->rhs_scratch[f], sizeof(float) * c->cells_full), "hipMalloc rhs");
-    check(hipMalloc((void**)&c->dz_scratch[f], sizeof(float) * c->cells_full), "hipMalloc dz");
   }
   if (st == SC_OK) check(hipMalloc((void**)&c->speed_bits, sizeof(unsigned) * 4), "hipMalloc speed");
   c->bbc_dims = bbc_dims;
@@ -312,10 +309,8 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
             "hipMemcpy boundary values");
   }
   if (st == SC_OK) {
-    for (int f = 0; f < c->nf; ++f) {
+    for (int f = 0; f < c->nf; ++f)
       check(hipMemset(c->rhs_scratch[f], 0, sizeof(float) * c->cells_full), "hipMemset rhs");
-      check(hipMemset(c->dz_scratch[f], 0, sizeof(float) * c->cells_full), "hipMemset dz");
-    }
   }
   if (st != SC_OK) {
     sc_ctx_destroy(c);
@@ -395,8 +390,6 @@ sc_status sc_ctx_destroy(sc_ctx* c) {
   if (c->pe_z) hipFree(c->pe_z);
   for (int f = 0; f < SC_MAX_FIELDS; ++f)
     if (c->rhs_scratch[f]) hipFree(c->rhs_scratch[f]);
-  for (int f = 0; f < SC_MAX_FIELDS; ++f)
-    if (c->dz_scratch[f]) hipFree(c->dz_scratch[f]);
   if (c->speed_bits) hipFree(c->speed_bits);
   if (c->bbc_dev) hipFree(c->bbc_dev);
   for (int f = 0; f < SC_MAX_FIELDS; ++f)
@@ -453,7 +446,6 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
   for (int f = 0; f < c->nf; ++f) {
     a.q.p[f] = q[f];
     a.r.p[f] = rhs[f];
-    a.dz.p[f] = c->dz_scratch[f];
   }
   a.rk_form = 0;
   return run_sweeps(c, a, (hipStream_t)stream);
@@ -462,26 +454,25 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               float* const* dacc, void* stream);
+                               void* stream);
 
 sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                       const float* const* u0, float* const* uk, float* const* out,
                       void* stream) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, c->dz_scratch,
-                       stream);
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, stream);
 }
 
 sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                               const float* const* u0, float* const* uk, float* const* out,
                               void* stream) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage_noalloc: null argument");
-  // Each field's accumulator and z-divergence start on a 256-byte boundary.
+  // Each field's accumulator starts on a 256-byte boundary.
   const long long stride = (c->cells_full + 63) / 64 * 64;
-  if (2 * stride * c->nf > kScratchFloats)
+  if (stride * c->nf > kScratchFloats)
     return fail(SC_ERR_UNSUPPORTED,
                 "sc_rk_stage_noalloc: grid too large for the 128 MiB scratch array (needs"
-                " 2 * n_fields * (nx+2h)(ny+2h)(nz+2h), each rounded up to 64, <= 33554432"
+                " n_fields * (nx+2h)(ny+2h)(nz+2h), each rounded up to 64, <= 33554432"
                 " floats); use sc_rk_stage");
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess || dev != c->device)
@@ -490,18 +481,14 @@ sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const flo
   if (base == nullptr)
     return fail(SC_ERR_HIP, "sc_rk_stage_noalloc: scratch array address not resolved");
   float* racc[SC_MAX_FIELDS] = {};
-  float* dacc[SC_MAX_FIELDS] = {};
-  for (int f = 0; f < c->nf; ++f) {
-    racc[f] = base + f * stride;
-    dacc[f] = base + (c->nf + f) * stride;
-  }
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, dacc, stream);
+  for (int f = 0; f < c->nf; ++f) racc[f] = base + f * stride;
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, stream);
 }
 
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               float* const* dacc, void* stream) {
+                               void* stream) {
   if (c == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
     return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
   if (form == SC_RK_AVG)
@@ -527,7 +514,6 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
     a.q.p[f] = uk[f];
     a.u0.p[f] = form == SC_RK_EULER ? nullptr : u0[f];
     a.r.p[f] = racc[f];
-    a.dz.p[f] = dacc[f];
     a.out.p[f] = out[f];
   }
   a.rk_form = form;
@@ -570,7 +556,6 @@ static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c
     a.q.p[f] = uk[f];
     a.u0.p[f] = (form == 0 || form == SC_RK_EULER) ? nullptr : u0[f];
     a.r.p[f] = c->rhs_scratch[f];
-    a.dz.p[f] = c->dz_scratch[f];
     a.out.p[f] = form == 0 ? nullptr : out[f];
   }
   a.rk_form = form;
@@ -1047,7 +1032,6 @@ struct sc_batch {
   float* hmin_dev = nullptr;
   unsigned* speed_bits = nullptr;
   float* r_arena[SC_MAX_FIELDS] = {};
-  float* dz_arena[SC_MAX_FIELDS] = {};
   float* pe_dev = nullptr;                 // potential tables of every instance
   sc::BoundaryBc* bbc_dev = nullptr;       // boundary face / flux values
   sc::BatchHaloTile* ptiles_dev = nullptr; // primitive-BC finish tiles
@@ -1147,7 +1131,6 @@ sc_status sc_batch_destroy(sc_batch* b) {
   hipFree(b->ptiles_dev);
   hipFree(b->speed_bits);
   for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->r_arena[f]);
-  for (int f = 0; f < SC_MAX_FIELDS; ++f) hipFree(b->dz_arena[f]);
   delete b;
   return SC_OK;
 }
@@ -1314,10 +1297,6 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
     st = hip_check(hipMalloc((void**)&b->r_arena[f], sizeof(float) * b->total), "hipMalloc R");
     if (st == SC_OK)
       st = hip_check(hipMemset(b->r_arena[f], 0, sizeof(float) * b->total), "hipMemset R");
-    if (st == SC_OK)
-      st = hip_check(hipMalloc((void**)&b->dz_arena[f], sizeof(float) * b->total), "hipMalloc Dz");
-    if (st == SC_OK)
-      st = hip_check(hipMemset(b->dz_arena[f], 0, sizeof(float) * b->total), "hipMemset Dz");
   }
   if (st != SC_OK) {
     sc_batch_destroy(b);
@@ -1405,7 +1384,6 @@ sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const flo
     ba.q.p[f] = uk[f];
     ba.u0.p[f] = form == SC_RK_BLEND ? u0[f] : nullptr;
     ba.r.p[f] = b->r_arena[f];
-    ba.dz.p[f] = b->dz_arena[f];
     ba.out.p[f] = out[f];
   }
   ba.inst = b->inst_dev;

@@ -33,7 +33,6 @@ struct BatchArgs {
   CFields q;        // arena bases
   CFields u0;
   Fields r;
-  Fields dz;
   Fields out;
   int nf;
   const BatchInst* inst;
@@ -52,7 +51,6 @@ __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int d
     a.q.p[f] = on && b.q.p[f] ? b.q.p[f] + in.off : nullptr;
     a.u0.p[f] = on && b.u0.p[f] ? b.u0.p[f] + in.off : nullptr;
     a.r.p[f] = on && b.r.p[f] ? b.r.p[f] + in.off : nullptr;
-    a.dz.p[f] = on && b.dz.p[f] ? b.dz.p[f] + in.off : nullptr;
     a.out.p[f] = on && b.out.p[f] ? b.out.p[f] + in.off : nullptr;
   }
   a.nx = in.nx;
@@ -126,19 +124,19 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
                        dim3(tiles.grid[0]), dim3(64, tune.by_x), 0, stream, bx);
   }
   {
+    BatchArgs by = b;
+    by.tiles = tiles.t[1];
+    by.ntiles = tiles.n[1];
+    hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS, FULL>), dim3(tiles.grid[1]),
+                       dim3(64, tune.by_y), 0, stream, by);
+  }
+  {
     BatchArgs bz = b;
     bz.proto.seg_len = tune.seg_z;
     bz.tiles = tiles.t[2];
     bz.ntiles = tiles.n[2];
     hipLaunchKernelGGL((batch_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, FULL>),
                        dim3(tiles.grid[2]), dim3(64, tune.by_z), 0, stream, bz);
-  }
-  {
-    BatchArgs by = b;
-    by.tiles = tiles.t[1];
-    by.ntiles = tiles.n[1];
-    hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS, FULL>), dim3(tiles.grid[1]),
-                       dim3(64, tune.by_y), 0, stream, by);
   }
   return hipGetLastError();
 }

@@ -86,14 +86,13 @@ __device__ __forceinline__ float rnn(float x) {
 }
 // tf.minimum / tf.maximum: a NaN operand gives NaN (fminf/fmaxf would return
 // the other operand), so unphysical states propagate exactly as in the
-// reference instead of silently selecting a different wave speed.
+// reference instead of silently selecting a different wave speed.  gfx950's
+// v_minimum3_f32 / v_maximum3_f32 propagate NaN in one instruction.
 __device__ __forceinline__ float tf_min(float a, float b) {
-  const float m = fminf(a, b);
-  return a != a ? a : (b != b ? b : m);
+  return __builtin_elementwise_minimum(a, b);
 }
 __device__ __forceinline__ float tf_max(float a, float b) {
-  const float m = fmaxf(a, b);
-  return a != a ? a : (b != b ? b : m);
+  return __builtin_elementwise_maximum(a, b);
 }
 
 // divide_no_nan with an IEEE division, for the boundary-condition paths whose
@@ -441,61 +440,71 @@ __device__ __forceinline__ void viscous_flux_cells(const float* A, const float* 
                                                    const float (*tb)[2], ViscCell ca,
                                                    ViscCell cb, const ViscConst& vc,
                                                    float* F) {
+  // The reference's face averages 0.5*(a + b) are carried here as the plain
+  // sums a + b, with each factor 1/2 moved to where the sum is used: scaling
+  // by a power of two commutes with rounding, so every result below is
+  // bit-identical to the direct form (outside the subnormal range) at fewer
+  // multiplies.
   const float ih = vc.inv_h[D];
-  // Backward (normal) derivatives on the face.
-  const float gu = (B[kU] - A[kU]) * ih;
-  const float gv = (B[kV] - A[kV]) * ih;
-  const float gw = (B[kW] - A[kW]) * ih;
-  const float gn[3] = {gu, gv, gw};
-  // Transverse derivatives interpolated onto the face: 0.5*(f[c-1] + f[c]).
-  float it[2][2];
+  const float ih2 = 2.0f * ih;
+  // Backward (normal) derivatives on the face; g2 = 2 * gn.
+  float gn[3], g2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float d = B[kU + i] - A[kU + i];
+    if (i == D) gn[i] = d * ih;
+    g2[i] = d * ih2;
+  }
+  // Transverse derivatives interpolated onto the face, times 2: f[c-1] + f[c].
+  float st[2][2];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    it[t][0] = 0.5f * (ta[t][0] + tb[t][0]);
-    it[t][1] = 0.5f * (ta[t][1] + tb[t][1]);
+    st[t][0] = ta[t][0] + tb[t][0];
+    st[t][1] = ta[t][1] + tb[t][1];
   }
-  // Divergence on the face, summed in x, y, z order.
+  // Twice the divergence on the face, summed in x, y, z order.
   float g3[3];
   {
     int t = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (k == D) {
-        g3[k] = gn[D];
+        g3[k] = g2[D];
       } else {
-        g3[k] = it[t][0];
+        g3[k] = st[t][0];
         ++t;
       }
     }
   }
-  const float div = (g3[0] + g3[1]) + g3[2];
-  // Strain rate S_iD on the face.
+  const float div2 = (g3[0] + g3[1]) + g3[2];
+  // Strain rate S_iD on the face: gn - div/3 and 0.5*(gn_i + transverse).
   float S[3];
   {
     int t = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       if (i == D) {
-        S[i] = gn[D] - kThird * div;
+        S[i] = gn[D] - (0.5f * kThird) * div2;
       } else {
-        S[i] = 0.5f * (gn[i] + it[t][1]);
+        S[i] = 0.25f * (g2[i] + st[t][1]);
         ++t;
       }
     }
   }
-  const float rho_f = 0.5f * (A[kRho] + B[kRho]);
-  const float c2 = (2.0f * rho_f) * vc.nu;
+  // 2 * rho_f * nu with rho_f = 0.5 * (rho[c-1] + rho[c]).
+  const float c2 = (A[kRho] + B[kRho]) * vc.nu;
   const float tx = c2 * S[0], ty = c2 * S[1], tz = c2 * S[2];
-  const float uf = 0.5f * (A[kU] + B[kU]);
-  const float vf = 0.5f * (A[kV] + B[kV]);
-  const float wf = 0.5f * (A[kW] + B[kW]);
-  // Heat flux -kappa_f * dT/dx, T = p/(rho R), kappa = ((1/Pr)*cp)*(nu*rho).
-  const float kf = 0.5f * (ca.k + cb.k);
-  const float q = -kf * ((cb.T - ca.T) * ih);
+  // Twice the face velocities.
+  const float uf2 = A[kU] + B[kU];
+  const float vf2 = A[kV] + B[kV];
+  const float wf2 = A[kW] + B[kW];
+  // Heat flux -kappa_f * dT/dx, T = p/(rho R), kappa = ((1/Pr)*cp)*(nu*rho),
+  // kappa_f = 0.5 * (kappa[c-1] + kappa[c]).
+  const float q = -(ca.k + cb.k) * ((cb.T - ca.T) * (0.5f * ih));
   F[1] = F[1] + (-tx);
   F[2] = F[2] + (-ty);
   F[3] = F[3] + (-tz);
-  F[4] = F[4] + (q + (-((uf * tx + vf * ty) + wf * tz)));
+  F[4] = F[4] + (q + (-(0.5f * ((uf2 * tx + vf2 * ty) + wf2 * tz))));
 }
 
 template <int D, int NS>

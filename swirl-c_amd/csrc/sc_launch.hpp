@@ -52,17 +52,7 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
     launch_sweep(sweep_march_kernel<0, SCHEME, FLUX, VISC, NS, 0, PD, BB, LEAN>, grid,
                  dim3(64, tune.by_x), stream, tune, ax);
   }
-  // z sweep (middle): Dz = div_z.
-  if (only_dir < 0 || only_dir == 2) {
-    SweepArgs az = a;
-    az.inv_hdir = 1.0f / h[2];
-    az.seg_len = tune.seg_z;
-    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
-                    (a.nz + tune.seg_z - 1) / tune.seg_z);
-    launch_sweep(sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, PD, BB, LEAN>, grid,
-                 dim3(64, tune.by_z), stream, tune, az);
-  }
-  // y sweep (last): R = (R - Dz) - div_y, then store R or the RK update.
+  // y sweep (middle): R = R - div_y.
   if (only_dir < 0 || only_dir == 1) {
     SweepArgs ay = a;
     ay.inv_hdir = 1.0f / h[1];
@@ -72,6 +62,16 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
                     (a.nz + tune.seg_y - 1) / tune.seg_y);
     launch_sweep(sweep_y_kernel<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>, grid,
                  dim3(64, tune.by_y), stream, tune, ay);
+  }
+  // z sweep (last): R - div_z, then store R or the RK update.
+  if (only_dir < 0 || only_dir == 2) {
+    SweepArgs az = a;
+    az.inv_hdir = 1.0f / h[2];
+    az.seg_len = tune.seg_z;
+    const dim3 grid((a.ny + 63) / 64, (a.nx + tune.by_z - 1) / tune.by_z,
+                    (a.nz + tune.seg_z - 1) / tune.seg_z);
+    launch_sweep(sweep_march_kernel<2, SCHEME, FLUX, VISC, NS, 2, PD, BB, LEAN>, grid,
+                 dim3(64, tune.by_z), stream, tune, az);
   }
   return hipGetLastError();
 }

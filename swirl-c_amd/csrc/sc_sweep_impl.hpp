@@ -3,21 +3,16 @@
 // One launch per direction.  Each launch reads the conserved fields of the
 // stage state once, converts them to primitives, reconstructs the face states
 // (WENO / MUSCL), solves the Riemann problem (HLL / HLLC), adds the viscous
-// and heat flux and takes the flux difference:
-//   x sweep:  R  = src - (F_x[i+1] - F_x[i]) / dx          (writes R)
-//   z sweep:  Dz = (F_z[k+1] - F_z[k]) / dz                 (writes Dz)
-//   y sweep:  R  = (R - Dz) - (F_y[j+1] - F_y[j]) / dy, then either writes R
-//             or the Runge-Kutta stage update of the state (RK axpy folded in).
-// The z sweep writes its divergence to an array of its own instead of
-// updating R in place: it then reads nothing but the state, which keeps its
-// registers within three waves per SIMD, and the y sweep, which reads R
-// anyway, reads Dz beside it (the same bytes move as with an in-place update).
-// The reference subtracts the divergences in x, y, z order
-// (equations/compressible_navier_stokes.py:255-264); here y is last because
-// its lanes run along the contiguous axis, so the RK operands (R, U0, Uk) of
-// every output cell are loaded at kernel start, off the critical path, while
-// a marching sweep would wait for them inside its loop.  The reordering only
-// changes the rounding of the three-term sum.
+// and heat flux and takes the flux difference, in the reference's x, y, z
+// order (equations/compressible_navier_stokes.py:255-264):
+//   x sweep:  R = src - (F_x[i+1] - F_x[i]) / dx            (writes R)
+//   y sweep:  R = R - (F_y[j+1] - F_y[j]) / dy               (updates R)
+//   z sweep:  R - (F_z[k+1] - F_z[k]) / dz, then either writes R or the
+//             Runge-Kutta stage update of the state (RK axpy folded in).
+// The RK operands (R, Uk, U0) are read by the z sweep: the marching sweeps
+// have memory bandwidth to spare (their arithmetic sets their time), while
+// the y sweep, whose windows re-read overlapping positions, is the one the
+// extra bytes would slow down.
 // Layout [z][x][y], y contiguous.  The x and z sweeps map 64 lanes to 64
 // consecutive y cells (coalesced) and march each thread along its pencil
 // with a register sliding window; the y sweep maps lanes along y and gets
@@ -84,7 +79,6 @@ struct SweepArgs {
   CFields q;    // stage state uk, halos filled
   CFields u0;   // RK base state
   Fields r;     // RHS accumulator
-  Fields dz;    // z-divergence of the stage (written by the z sweep, read by y)
   Fields out;   // RK stage output
   int nx, ny, nz, h;
   int NX, NY, NZ;
@@ -130,9 +124,12 @@ __device__ __forceinline__ TileIdx tile_of_block(int remap) {
     const int q = n / kNumXcd, r = n % kNumXcd;
     const int xcd = lin % kNumXcd, slot = lin / kNumXcd;
     const int t = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
-    bx = t % gx;
-    by = (t / gx) % gy;
-    bz = t / (gx * gy);
+    // The divisions are lowered to vector instructions; their results are
+    // uniform, and readfirstlane puts them back in scalar registers so the
+    // addresses derived from them stay scalar.
+    bx = __builtin_amdgcn_readfirstlane(t % gx);
+    by = __builtin_amdgcn_readfirstlane((t / gx) % gy);
+    bz = __builtin_amdgcn_readfirstlane(t / (gx * gy));
   }
   return TileIdx{bx, by, bz, (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
 }
@@ -249,6 +246,159 @@ __device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-uniform addressing.  Every access of the sweeps is a wave-uniform base
+// (a field pointer plus a uniform element offset, held in scalar registers)
+// plus the lane's 32-bit byte offset, which global loads and stores take
+// directly (the scalar-base form), so no 64-bit vector address arithmetic is
+// issued per access.  Two compiler habits defeat that form and are blocked
+// here: it hoists the extension of a loop-invariant lane offset out of a loop
+// and then adds it to each base in vector registers (lane_off() redefines the
+// offset in the block that uses it; this may cost one register move), and it
+// re-associates base + uniform offset + lane offset so that the sum of the
+// two offsets, shared by several fields, is formed once in vector registers
+// (wave_base() passes the base through readfirstlane, which the compiler
+// cannot see through and later drops, since its operand is already uniform).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned lane_off(unsigned lb) {
+  asm volatile("" : "+v"(lb));
+  return lb;
+}
+
+// Orders a march iteration's work: the value is taken as redefined here, so
+// its computation stays before this point and its uses after it (an empty
+// asm statement; volatile ones keep their relative order).  Without it the
+// compiler sinks the reconstruction into the flux block and computes the
+// transverse derivatives first, which makes every iteration wait for the
+// neighbour loads it has just issued.
+__device__ __forceinline__ void pin(float& x) { asm volatile("" : "+v"(x)); }
+
+typedef __attribute__((address_space(1))) float global_float;
+typedef __attribute__((address_space(1))) char global_char;
+
+// A wave-uniform 64-bit value moved to scalar registers (the compiler may
+// otherwise keep a loop-invariant uniform value in vector registers, and the
+// addresses formed from it then become vector arithmetic).  v must be the
+// same in every lane of the wave.
+__device__ __forceinline__ long long uniform_ll(long long v) {
+  const unsigned long long u = (unsigned long long)v;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// The base pointer p as a global-memory pointer.  p must be the same in every
+// lane of the wave (readfirstlane takes the first active lane's value).
+__device__ __forceinline__ global_float* wave_base(const float* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (global_float*)(((unsigned long long)hi << 32) | lo);
+}
+
+// Element at byte offset boff of the array starting at `base` (base
+// wave-uniform, boff the lane's offset).
+__device__ __forceinline__ float ld_plane(const float* base, unsigned boff) {
+  return *(const global_float*)((const global_char*)wave_base(base) + boff);
+}
+__device__ __forceinline__ void st_plane(float* base, unsigned boff, float v) {
+  *(global_float*)((global_char*)wave_base(base) + boff) = v;
+}
+// The same accesses without readfirstlane: the compiler may then form one
+// scalar base for accesses whose offsets differ by a constant, so fewer
+// scalar base registers are held (for the y sweep, whose many bases would
+// otherwise spill scalar registers).
+__device__ __forceinline__ float ld_lane(const float* base, unsigned boff) {
+  return *(const float*)((const char*)base + boff);
+}
+__device__ __forceinline__ void st_lane(float* base, unsigned boff, float v) {
+  *(float*)((char*)base + boff) = v;
+}
+
+// The loads above with a uniform element offset u: cell u + lane.
+template <int NF>
+__device__ __forceinline__ void load_raw_u(const SweepArgs& a, long long u, unsigned lb0,
+                                           float* q) {
+  const unsigned lb = lane_off(lb0);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) q[f] = ld_plane(a.q.p[f] + u, lb);
+}
+
+// Potential at (k, i, jb + lane) with k and i uniform.
+template <bool LEAN = false>
+__device__ __forceinline__ float pe_at_u(const SweepArgs& a, int k, int i, int jb,
+                                         unsigned lb0) {
+  if constexpr (LEAN) return 0.0f;
+  return a.has_pe ? (a.pe_x[i] - ld_plane(a.pe_y + jb, lane_off(lb0))) - a.pe_z[k] : 0.0f;
+}
+
+template <int NS, bool PD>
+__device__ __forceinline__ void set_pdyn_u(const SweepArgs& a, long long u, unsigned lb0,
+                                           bool interior, float* w) {
+  if constexpr (PD) w[7 + NS] = interior ? w[kP] - ld_plane(a.p_ref + u, lane_off(lb0)) : 0.0f;
+}
+
+// transverse_grads for a marching sweep's cell (k, i, j) whose lane-0 cell
+// sits at uniform element offset u (j = first y cell of the wave + lane), in
+// two halves: the loads (rho, u_t, u_D momenta of the four neighbours), issued
+// early in a march iteration, and the derivatives, formed once the
+// reconstruction has covered the loads' latency.
+struct TransRaw {
+  float v[2][2][3];  // [transverse dim t][side + / -][rho, m_t, m_D]
+};
+
+template <int D>
+__device__ __forceinline__ void transverse_load_u(const SweepArgs& a, long long u, unsigned lb0,
+                                                  TransRaw& r) {
+  const unsigned lb = lane_off(lb0);
+  int t = 0;
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) {
+    if (dd == D) continue;
+    const long long st = dd == 0 ? a.sx : (dd == 1 ? 1 : a.sz);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const long long un = s == 0 ? u + st : u - st;
+      r.v[t][s][0] = ld_plane(a.q.p[0] + un, lb);
+      r.v[t][s][1] = ld_plane(a.q.p[1 + dd] + un, lb);
+      r.v[t][s][2] = ld_plane(a.q.p[1 + D] + un, lb);
+    }
+    ++t;
+  }
+}
+
+template <int D, bool LEAN = false>
+__device__ __forceinline__ void transverse_grads_from(const SweepArgs& a, int k, int i, int j,
+                                                      const TransRaw& r, float (*out)[2]) {
+  int t = 0;
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) {
+    if (dd == D) continue;
+    float w[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int o = s == 0 ? 1 : -1;
+      const bool in = in_core(i + (dd == 0 ? o : 0), a.h, a.nx) &&
+                      in_core(j + (dd == 1 ? o : 0), a.h, a.ny) &&
+                      in_core(k + (dd == 2 ? o : 0), a.h, a.nz);
+      const float rho = r.v[t][s][0];
+      float m0 = r.v[t][s][1];
+      float m1 = r.v[t][s][2];
+      if (!LEAN && a.prim_bc_mask && in) {
+        if (a.prim_bc_mask & (1 << (1 + dd))) m0 = rho * dnn_exact(m0, rho);
+        if (a.prim_bc_mask & (1 << (1 + D))) m1 = rho * dnn_exact(m1, rho);
+      }
+      const float inv = frcp(rho);
+      w[s][0] = inv * m0;
+      w[s][1] = inv * m1;
+    }
+    const float ith = a.vc.inv_two_h[dd];
+    out[t][0] = (w[0][0] - w[1][0]) * ith;
+    out[t][1] = (w[0][1] - w[1][1]) * ith;
+    ++t;
+  }
+}
+
 // Gravity source of field f at a cell (physics_models.py:94-143,
 // gravity.py:64-102): (rho * G) * g_d on the selected momenta, else 0.
 template <bool LEAN = false>
@@ -258,16 +408,6 @@ __device__ __forceinline__ float source_of(const SweepArgs& a, int f, float rho)
     return (rho * kG) * a.gdir[f - 1];
   }
   return 0.0f;
-}
-
-// Runge-Kutta stage update of one value from preloaded operands
-// (rk3.py:69-83, rk2.py:67-86): 1 out = uk + L*dt; 2 out = u0*c0 +
-// (uk + L*dt)*c1; 3 out = 0.5*(u0 + (uk + dt*L)).
-__device__ __forceinline__ float rk_update(const SweepArgs& a, float uk, float u0, float val,
-                                           float dt) {
-  if (a.rk_form == 1) return uk + val * dt;
-  if (a.rk_form == 2) return u0 * a.c0 + (uk + val * dt) * a.c1;
-  return 0.5f * (u0 + (uk + dt * val));
 }
 
 // Face-state overwrite of boundary_face_update_1d on one side's reconstructed
@@ -334,8 +474,62 @@ __device__ __forceinline__ void face_flux(const BoundaryBc* b, long long pidx, f
 // Marching sweep along x (DIR 0) or z (DIR 2).  Block (64, BY): lane -> y,
 // threadIdx.y -> the other transverse index, blockIdx.z -> segment of
 // seg_len cells along the sweep.  PHASE 0: first sweep (writes R = src - div);
-// PHASE 2: middle sweep (writes Dz = div).
+// PHASE 2: last sweep (R - div into R, or the Runge-Kutta stage update of the
+// state into out).
 // ---------------------------------------------------------------------------
+
+// Operands of the last sweep's output cell: R, and for RK forms 2 and 3 the
+// base state u0 (the stage state uk comes from the march's own registers).
+template <int NF>
+struct OutOps {
+  float r[NF], u0[NF];
+};
+
+template <int NF>
+__device__ __forceinline__ void load_out_ops(const SweepArgs& a, long long u, unsigned lb,
+                                             OutOps<NF>& o) {
+  {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) o.r[f] = ld_plane(a.r.p[f] + u, l);
+  }
+  if (a.rk_form == 2 || a.rk_form == 3) {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) o.u0[f] = ld_plane(a.u0.p[f] + u, l);
+  }
+}
+
+// Stores R - div (rk_form 0) into R, else the RK stage update (rk3.py:69-83,
+// rk2.py:67-86): 1 out = uk + L*dt; 2 out = u0*c0 + (uk + L*dt)*c1;
+// 3 out = 0.5*(u0 + (uk + dt*L)); one branch on the form for all fields.
+template <int NF>
+__device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsigned lb,
+                                          const OutOps<NF>& o, const float* uk,
+                                          const float* div, float dt) {
+  float val[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) val[f] = o.r[f] - div[f] * a.inv_hdir;
+  if (a.rk_form == 0) {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) st_plane(a.r.p[f] + u, l, val[f]);
+  } else if (a.rk_form == 2) {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      st_plane(a.out.p[f] + u, l, o.u0[f] * a.c0 + (uk[f] + val[f] * dt) * a.c1);
+  } else if (a.rk_form == 1) {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) st_plane(a.out.p[f] + u, l, uk[f] + val[f] * dt);
+  } else {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      st_plane(a.out.p[f] + u, l, 0.5f * (o.u0[f] + (uk[f] + dt * val[f])));
+  }
+}
 
 // BB: build with the boundary face / flux value handling (only launched when
 // the configuration has such values, so other runs carry none of its code).
@@ -346,8 +540,11 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   constexpr int WN = 2 * RAD + 1;
   constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   constexpr int NF = 5 + NS;
-  const int j = a.h + ti.bx * 64 + ti.tx;
-  const int o = a.h + ti.by * ti.by_dim + ti.ty;
+  // Lanes run along y (64 consecutive cells from jb); the block's waves differ
+  // in threadIdx.y only (blocks are 64 wide), so o is wave-uniform.
+  const int jb = a.h + ti.bx * 64;
+  const int j = jb + ti.tx;
+  const int o = a.h + ti.by * ti.by_dim + __builtin_amdgcn_readfirstlane(ti.ty);
   const int n_march = DIR == 0 ? a.nx : a.nz;
   const int n_other = DIR == 0 ? a.nz : a.nx;
   if (j >= a.h + a.ny || o >= a.h + n_other) return;
@@ -356,7 +553,11 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   const int c_end = min(c_begin + a.seg_len, a.h + n_march);
   const long long sm = DIR == 0 ? a.sx : a.sz;
   const long long so = DIR == 0 ? a.sz : a.sx;
-  const long long base = (long long)o * so + j;
+  // Uniform element offset of the wave's lane-0 cell at march coordinate c,
+  // and the lane's byte offset from it.
+  const long long ub = uniform_ll((long long)o * so + jb);
+  auto uo = [&](int c) { return ub + (long long)c * sm; };
+  const unsigned lb = (unsigned)ti.tx * 4u;
   // Cell (k, i, j) of march coordinate c.
   auto kk = [&](int c) { return DIR == 0 ? o : c; };
   auto ii = [&](int c) { return DIR == 0 ? c : o; };
@@ -365,11 +566,26 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   // WN-1 cells are converted here (cons_to_prim rounds the same in every
   // copy, so results do not depend on where a segment starts).
   float W[WN][NV];
+  // The last sweep keeps the raw stage values of cells c - 1 .. c + RAD - 1
+  // at iteration c: hq[0] is the output cell's uk, read once with the
+  // window instead of again from memory.
+  constexpr int NH = PHASE == 2 ? RAD + 1 : 1;
+  float hq[NH][NF];
 #pragma unroll
   for (int w = 0; w < WN - 1; ++w) {
     const int c = c_begin - 1 - RAD + w;
-    load_prim<NS, PD, LEAN>(a, base + (long long)c * sm, in_core(c, a.h, n_march),
-                            pe_at<LEAN>(a, kk(c), ii(c), j), W[w]);
+    const bool in = in_core(c, a.h, n_march);
+    float q[NF];
+    load_raw_u<NF>(a, uo(c), lb, q);
+    if constexpr (PHASE == 2) {
+      if (w >= RAD - 1) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) hq[w - RAD + 1][f] = q[f];
+      }
+    }
+    prim_bc_round<NF, LEAN>(a, in, q);
+    cons_to_prim<NS>(q, pe_at_u<LEAN>(a, kk(c), ii(c), jb, lb), W[w]);
+    set_pdyn_u<NS, PD>(a, uo(c), lb, in, W[w]);
   }
   float negPrev[NV];
   float Fprev[NF];
@@ -384,17 +600,31 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   // Raw conserved values of the cell entering the window, loaded one
   // iteration ahead so the loads overlap the arithmetic of the current cell.
   float qn[NF];
-  load_raw<NF>(a, base + (long long)(c_begin - 1 + RAD) * sm, qn);
+  load_raw_u<NF>(a, uo(c_begin - 1 + RAD), lb, qn);
+  // The last sweep's RK time step (a device scalar).
+  const float dt = (PHASE == 2 && a.rk_form != 0) ? *a.dt : 0.0f;
   for (int c = c_begin - 1; c <= c_end; ++c) {
+    // Loads are issued in the order their values are consumed (the memory
+    // counter a wait names is in order): the transverse neighbours (used
+    // after the reconstruction), the next window cell (next iteration), and,
+    // in the last sweep, the output cell's operands (used at the store).
+    TransRaw tr;
+    if constexpr (VISC) transverse_load_u<DIR>(a, uo(c), lb, tr);
+    float qraw[NF];
     {
       const int cn = c + RAD;
       float qc[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) qc[f] = qn[f];
-      if (c < c_end) load_raw<NF>(a, base + (long long)(cn + 1) * sm, qn);
+      if (c < c_end) load_raw_u<NF>(a, uo(cn + 1), lb, qn);
+      if constexpr (PHASE == 2) {
+        // Enters the history ring at the end of the iteration.
+#pragma unroll
+        for (int f = 0; f < NF; ++f) qraw[f] = qc[f];
+      }
       prim_bc_round<NF, LEAN>(a, in_core(cn, a.h, n_march), qc);
-      cons_to_prim<NS>(qc, pe_at<LEAN>(a, kk(cn), ii(cn), j), W[WN - 1]);
-      set_pdyn<NS, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n_march), W[WN - 1]);
+      cons_to_prim<NS>(qc, pe_at_u<LEAN>(a, kk(cn), ii(cn), jb, lb), W[WN - 1]);
+      set_pdyn_u<NS, PD>(a, uo(cn), lb, in_core(cn, a.h, n_march), W[WN - 1]);
     }
     float negP[NV], posM[NV];
 #pragma unroll
@@ -407,8 +637,29 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
     float tCur[2][2];
     ViscCell vCur{0.0f, 0.0f};
     if constexpr (VISC) {
-      transverse_grads<DIR, LEAN>(a, kk(c), ii(c), j, tCur);
+      // Reconstruction first, then the neighbour values (their loads have
+      // had the reconstruction's time to arrive), then the derivatives.
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        pin(negP[v]);
+        pin(posM[v]);
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) pin(tr.v[t][sd][q]);
+      transverse_grads_from<DIR, LEAN>(a, kk(c), ii(c), j, tr, tCur);
       vCur = visc_cell(W[RAD], a.vc);
+      pin(tCur[0][0]);
+      pin(tCur[0][1]);
+      pin(tCur[1][0]);
+      pin(tCur[1][1]);
+    }
+    OutOps<NF> oo;
+    if constexpr (PHASE == 2) {
+      if (c > c_begin) load_out_ops<NF>(a, uo(c - 1), lb, oo);
     }
     if (c >= c_begin) {
       float F[NF];
@@ -417,21 +668,26 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
           (BB && ((a.bbc_dims >> DIR) & 1) && side >= 0) ? a.bbc + 2 * DIR + side : nullptr;
       // Boundary plane cell of this face: x faces [z][y], z faces [x][y].
       face_flux<DIR, FLUX, VISC, NS, PD>(bb, (long long)o * a.NY + j, negPrev, posM,
-                                         pe_at<LEAN>(a, kk(c), ii(c), j),
+                                         pe_at_u<LEAN>(a, kk(c), ii(c), jb, lb),
                                          W[RAD - 1], W[RAD], tPrev, tCur, vPrev, vCur, a.vc,
                                          F);
       if (c > c_begin) {
-        const long long idx = base + (long long)(c - 1) * sm;
+        const long long u = uo(c - 1);
         if constexpr (PHASE == 0) {
+          const unsigned ls = lane_off(lb);
           // The gravity source takes d_rho = rho - rho_ref with a reference state.
-          const float rho = PD ? W[RAD - 1][kRho] - a.rho_ref[idx] : W[RAD - 1][kRho];
+          const float rho =
+              PD ? W[RAD - 1][kRho] - ld_plane(a.rho_ref + u, ls) : W[RAD - 1][kRho];
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            a.r.p[f][idx] = source_of<LEAN>(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir;
+            st_plane(a.r.p[f] + u, ls,
+                     source_of<LEAN>(a, f, rho) - (F[f] - Fprev[f]) * a.inv_hdir);
           }
         } else {
+          float div[NF];
 #pragma unroll
-          for (int f = 0; f < NF; ++f) a.dz.p[f][idx] = (F[f] - Fprev[f]) * a.inv_hdir;
+          for (int f = 0; f < NF; ++f) div[f] = F[f] - Fprev[f];
+          store_out<NF>(a, u, lb, oo, hq[0], div, dt);
         }
       }
 #pragma unroll
@@ -451,6 +707,14 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 #pragma unroll
       for (int v = 0; v < NV; ++v) W[w][v] = W[w + 1][v];
     }
+    if constexpr (PHASE == 2) {
+#pragma unroll
+      for (int m = 0; m < NH - 1; ++m)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) hq[m][f] = hq[m + 1][f];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) hq[NH - 1][f] = qraw[f];
+    }
   }
 }
 
@@ -463,11 +727,11 @@ sweep_march_kernel(const SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// y sweep (contiguous direction, last).  Block (64, BY): one wave per window
-// of 64 consecutive positions of a z plane's run of rows (y_windows); the
-// stencil comes from DPP lane shifts (every lane of the wave is active at
-// each shift).  Lanes K .. K + OUTW - 1 at interior cells output: R - div
-// into R (rk_form 0) or the RK stage update into out.
+// y sweep (contiguous direction, middle).  Block (64, BY): one wave per
+// window of 64 consecutive positions of a z plane's run of rows (y_windows);
+// the stencil comes from DPP lane shifts (every lane of the wave is active at
+// each shift).  Lanes K .. K + OUTW - 1 at interior cells output R - div
+// into R.
 // ---------------------------------------------------------------------------
 template <int SCHEME>
 struct YGeom {
@@ -530,19 +794,11 @@ __device__ __forceinline__ YPos y_pos(const SweepArgs& a, int win, int lane) {
 // of the current one.
 template <int NF, bool VISC, bool PD>
 struct YLoad {
-  float q[NF];        // stage state (reconstructed, and uk of the RK update)
-  float r[NF], dz[NF], u0[NF];
+  float q[NF];        // stage state (reconstructed)
+  float r[NF];        // RHS accumulator
   float tx[2][3];     // rows i + 1, i - 1: rho, rho_u, rho_v
   float pref;         // p_ref (reference-state builds)
 };
-
-// Element at byte offset boff of the plane starting at `plane`.
-__device__ __forceinline__ float ld_plane(const float* plane, unsigned boff) {
-  return *(const float*)((const char*)plane + boff);
-}
-__device__ __forceinline__ void st_plane(float* plane, unsigned boff, float v) {
-  *(float*)((char*)plane + boff) = v;
-}
 
 // Every lane loads (the positions of lanes that do not output are valid cells
 // of the plane's interior rows, and plane k may be the first halo plane), so
@@ -551,28 +807,19 @@ template <int NF, bool VISC, bool PD>
 __device__ __forceinline__ void y_load(const SweepArgs& a, int k, const YPos& p,
                                        YLoad<NF, VISC, PD>& L) {
   const long long pk = (long long)k * a.sz;
-  const unsigned b = p.boff;
+  const unsigned b = lane_off(p.boff);
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     L.q[f] = ld_plane(a.q.p[f] + pk, b);
-    L.r[f] = ld_plane(a.r.p[f] + pk, b);
-    L.dz[f] = ld_plane(a.dz.p[f] + pk, b);
-  }
-  if (a.rk_form == 2 || a.rk_form == 3) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) L.u0[f] = ld_plane(a.u0.p[f] + pk, b);
-  } else {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) L.u0[f] = 0.0f;
+    L.r[f] = ld_lane(a.r.p[f] + pk, b);
   }
   if constexpr (VISC) {
-    const unsigned bx = (unsigned)a.sx * 4u;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const unsigned bn = s == 0 ? b + bx : b - bx;
-      L.tx[s][0] = ld_plane(a.q.p[0] + pk, bn);
-      L.tx[s][1] = ld_plane(a.q.p[1] + pk, bn);
-      L.tx[s][2] = ld_plane(a.q.p[2] + pk, bn);
+      const long long pn = s == 0 ? pk + a.sx : pk - a.sx;
+      L.tx[s][0] = ld_plane(a.q.p[0] + pn, b);
+      L.tx[s][1] = ld_plane(a.q.p[1] + pn, b);
+      L.tx[s][2] = ld_plane(a.q.p[2] + pn, b);
     }
   }
   if constexpr (PD) L.pref = ld_plane(a.p_ref + pk, b);
@@ -593,18 +840,16 @@ __device__ __forceinline__ void vel2_of(const SweepArgs& a, float rho, float m0,
 }
 
 // One window of the y sweep at plane k from its loaded operands (zp, zm:
-// rho, rho_w, rho_v of planes k + 1 and k - 1): (R - Dz) - div_y into R
-// (rk_form 0) or the RK stage update into out.
+// rho, rho_w, rho_v of planes k + 1 and k - 1): R - div_y into R.
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD, bool BB, bool LEAN>
 __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& p,
                                          const YLoad<5 + NS, VISC, PD>& L, const float* zp,
-                                         const float* zm, float dt) {
+                                         const float* zm) {
   constexpr int RAD = YGeom<SCHEME>::RAD;
   constexpr int WN = 2 * RAD + 1;
   constexpr int NV = NVar<NS>::value + (PD ? 1 : 0);
   constexpr int NF = 5 + NS;
   const int yc = p.yc, i = p.i;
-  const bool rk = a.rk_form != 0;
   const bool interior = in_core(yc, a.h, a.ny);
   float W0[NV];
   {
@@ -678,16 +923,10 @@ __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& 
   for (int f = 0; f < NF; ++f) Fn[f] = wave_shl1(F[f]);
   if (p.out_lane) {
     const long long pk = (long long)k * a.sz;
+    const unsigned b = lane_off(p.boff);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      const float rin = L.r[f] - L.dz[f];
-      const float val = rin - (Fn[f] - F[f]) * a.inv_hdir;
-      if (rk) {
-        st_plane(a.out.p[f] + pk, p.boff, rk_update(a, L.q[f], L.u0[f], val, dt));
-      } else {
-        st_plane(a.r.p[f] + pk, p.boff, val);
-      }
-    }
+    for (int f = 0; f < NF; ++f)
+      st_lane(a.r.p[f] + pk, b, L.r[f] - (Fn[f] - F[f]) * a.inv_hdir);
   }
 }
 
@@ -707,7 +946,6 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
   // uniform per wave
   if (k0 >= a.h + a.nz || win >= y_windows(a.nx, a.ny, a.h, OUTW)) return;
   const int k1 = min(k0 + zseg, a.h + a.nz);
-  const float dt = a.rk_form != 0 ? *a.dt : 0.0f;
   const YPos p = y_pos<SCHEME>(a, win, lane);
   float zm[3] = {0.0f, 0.0f, 0.0f};
   if constexpr (VISC) {
@@ -724,7 +962,7 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
     y_load(a, k + 1, p, B);
     {
       const float zp[3] = {B.q[0], B.q[3], B.q[2]};
-      y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, p, A, zp, zm, dt);
+      y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k, p, A, zp, zm);
       zm[0] = A.q[0];
       zm[1] = A.q[3];
       zm[2] = A.q[2];
@@ -733,7 +971,7 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
     y_load(a, k + 2, p, A);
     {
       const float zp[3] = {A.q[0], A.q[3], A.q[2]};
-      y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k + 1, p, B, zp, zm, dt);
+      y_window<SCHEME, FLUX, VISC, NS, PD, BB, LEAN>(a, k + 1, p, B, zp, zm);
       zm[0] = B.q[0];
       zm[1] = B.q[3];
       zm[2] = B.q[2];

@@ -98,9 +98,9 @@ def tune_context(nctx, q, out, dt_dev, u0=None, reps=5, march_space=MARCH_SPACE,
     timings.append((d, t, ms))
   best = _pick(timings, base)
   # The block order (XCD-aware remap or not) is shared by the three sweeps:
-  # time the whole x, z, y sequence both ways.
+  # time the whole x, y, z sequence both ways.
   def all_three():
-    for d in (0, 2, 1):
+    for d in (0, 1, 2):
       nctx.sweep_only(d, N.SC_RK_BLEND, 0.75, 0.25, dt_dev, u0, q, out)
 
   for remap in (base['xcd_remap'], 1 - base['xcd_remap']):

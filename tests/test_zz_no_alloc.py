@@ -64,7 +64,7 @@ def test_noalloc_stage_matches_rk_stage_on_two_calls_in_a_row():
 
 
 def test_noalloc_stage_rejects_grids_beyond_its_array():
-  n, h = (150, 150, 150), 3
+  n, h = (200, 200, 200), 3
   cfg = grid_cfg(n, h)
   cfg.update({'interpolation_scheme': 'WENO_5', 'numeric_flux_scheme': 'HLLC',
               'include_diffusion': False, 'bc': P._bcs('periodic', n, h)})
