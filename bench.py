@@ -120,14 +120,16 @@ def parse():
 
 
 def time_step_kernels(solver, steps):
-  """Kernel durations inside RK3 steps: a step's launch sequence (CFL wave
-  speed, then per stage the halo fill and the x, y, z sweeps, the last one
-  with the stage update) issued through the C ABI on one stream.  Each sweep
-  launch records its own start and stop times into HIP events
-  (sc_sweep_only_timed), so nothing is inserted between the kernels; the
-  wave speed and halo fill are bracketed by events recorded on the stream.
-  Returns {kernel: [ms per launch at stage 1, 2, 3]} (the wave speed as a
-  one-element list) averaged over `steps`, and the mean step time (ms)."""
+  """Kernel durations inside RK3 steps: a step's launch sequence (the CFL dt
+  from the maximum wave speed the previous step's last stage accumulated,
+  then per stage the halo fill and the x, y, z sweeps, the last one with the
+  stage update, and in stage 3 the wave-speed accumulation) issued through
+  the C ABI on one stream.  Each sweep launch records its own start and stop
+  times into HIP events (sc_sweep_only_timed), so nothing is inserted between
+  the kernels; the dt kernel and halo fill are bracketed by events recorded
+  on the stream.  Returns {kernel: [ms per launch at stage 1, 2, 3]} (the dt
+  kernel as a one-element list) averaged over `steps`, and the mean step time
+  (ms)."""
   import torch
   from swirlc_amd import _native as N
   ctx = solver.ctx
@@ -137,7 +139,7 @@ def time_step_kernels(solver, steps):
             (N.SC_RK_BLEND, 1.0 / 3.0, 2.0 / 3.0, u, s2, u))
   names = ('halo', 'sweep_x', 'sweep_y', 'sweep_z')
   acc = {n: [0.0, 0.0, 0.0] for n in names}
-  acc['max_speed'] = [0.0]
+  acc['cfl_dt'] = [0.0]
   total = 0.0
 
   def event():
@@ -145,6 +147,7 @@ def time_step_kernels(solver, steps):
     e.record(stream)  # creates the HIP event
     return e
 
+  solver.refresh_speed()
   for _ in range(steps):
     t_begin, t_speed = event(), event()
     halo = [(event(), event()) for _ in range(3)]
@@ -152,7 +155,7 @@ def time_step_kernels(solver, steps):
     t_end = event()
     torch.cuda.synchronize()
     t_begin.record(stream)
-    ctx.max_wave_speed(u, solver.speed, float(solver.cfl), solver.dt, stream)
+    ctx.cfl_dt(solver.speed, float(solver.cfl), solver.dt, stream)
     t_speed.record(stream)
     for st, (form, c0, c1, base, uk, out) in enumerate(stages):
       halo[st][0].record(stream)
@@ -160,10 +163,11 @@ def time_step_kernels(solver, steps):
       halo[st][1].record(stream)
       for j, d in enumerate((0, 1, 2)):
         e0, e1 = sweeps[st][j]
-        ctx.sweep_only_timed(d, form, c0, c1, solver.dt, base, uk, out, e0, e1, stream)
+        ctx.sweep_only_timed(d, form, c0, c1, solver.dt, base, uk, out, e0, e1, stream,
+                             speed_dev=solver.speed if (st == 2 and d == 2) else None)
     t_end.record(stream)
     t_end.synchronize()
-    acc['max_speed'][0] += t_begin.elapsed_time(t_speed)
+    acc['cfl_dt'][0] += t_begin.elapsed_time(t_speed)
     for st in range(3):
       acc['halo'][st] += halo[st][0].elapsed_time(halo[st][1])
       for j, n in enumerate(('sweep_x', 'sweep_y', 'sweep_z')):

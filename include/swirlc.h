@@ -203,6 +203,23 @@ sc_status sc_ctx_set_reference_state(sc_ctx* ctx, const float* p_ref, const floa
 sc_status sc_max_wave_speed(sc_ctx* ctx, const float* const* q, float cfl,
                             float* out_dev, float* dt_dev, void* stream);
 
+/* sc_rk_stage that also max-accumulates the wave speed of sc_max_wave_speed
+ * over out's interior cells into *speed_dev (a float in device memory,
+ * compared as its bits: clear it with sc_cfl_dt or to 0 before the first
+ * accumulation).  The last sweep computes it from the values it writes, so a
+ * time step's stage 3 leaves the next step's CFL maximum without re-reading
+ * the state; the result equals sc_max_wave_speed on out bit for bit.  No
+ * reference counterpart (the reference takes a fixed cfg.dt). */
+sc_status sc_rk_stage_speed(sc_ctx* ctx, int form, float c0, float c1,
+                            const float* dt_dev, const float* const* u0,
+                            float* const* uk, float* const* out, float* speed_dev,
+                            void* stream);
+
+/* dt = cfl * min(dx,dy,dz) / *speed_dev written to *dt_dev, then *speed_dev
+ * cleared to 0 (for the next sc_rk_stage_speed accumulation).  No reference
+ * counterpart. */
+sc_status sc_cfl_dt(sc_ctx* ctx, float* speed_dev, float cfl, float* dt_dev, void* stream);
+
 /* Face states of one field: (neg, pos) = left / right states of face i-1/2
  * stored at index i along dim (0 x, 1 y, 2 z) of the [z][x][y] device array
  * with extents[3] = {Nx, Ny, Nz} (halos included).  scheme SC_WENO_5 /
@@ -297,11 +314,13 @@ sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
 /* Profiling: sc_sweep_only whose kernel launch records its own start and stop
  * times into the HIP events ev_start / ev_stop (hipExtLaunchKernel: no extra
  * packets between kernels), so a kernel's duration inside a step sequence
- * can be read with hipEventElapsedTime.  No reference counterpart. */
+ * can be read with hipEventElapsedTime.  speed_dev (or NULL): as in
+ * sc_rk_stage_speed, for the z sweep of an RK form.  No reference
+ * counterpart. */
 sc_status sc_sweep_only_timed(sc_ctx* ctx, int dim, int form, float c0, float c1,
                               const float* dt_dev, const float* const* u0,
-                              float* const* uk, float* const* out, void* ev_start,
-                              void* ev_stop, void* stream);
+                              float* const* uk, float* const* out, float* speed_dev,
+                              void* ev_start, void* ev_stop, void* stream);
 
 /* Launch geometry of the sweep kernels (no reference counterpart: the
  * reference leaves scheduling to XLA).  seg_*: cells per thread of a marching

@@ -454,13 +454,37 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               void* stream);
+                               unsigned* speed_bits, void* stream);
 
 sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                       const float* const* u0, float* const* uk, float* const* out,
                       void* stream) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, stream);
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, nullptr, stream);
+}
+
+sc_status sc_rk_stage_speed(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
+                            const float* const* u0, float* const* uk, float* const* out,
+                            float* speed_dev, void* stream) {
+  if (c == nullptr || speed_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_rk_stage_speed: null argument");
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch,
+                       (unsigned*)speed_dev, stream);
+}
+
+static float min_spacing(const sc_ctx* c) {
+  float hmin = c->h[0];
+  if (c->h[1] < hmin) hmin = c->h[1];
+  if (c->h[2] < hmin) hmin = c->h[2];
+  return hmin;
+}
+
+sc_status sc_cfl_dt(sc_ctx* c, float* speed_dev, float cfl, float* dt_dev, void* stream) {
+  if (c == nullptr || speed_dev == nullptr || dt_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_cfl_dt: null argument");
+  return hip_check(sc::launch_cfl_dt_reset((unsigned*)speed_dev, cfl, min_spacing(c), dt_dev,
+                                           (hipStream_t)stream),
+                   "cfl dt kernel");
 }
 
 sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
@@ -482,13 +506,13 @@ sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const flo
     return fail(SC_ERR_HIP, "sc_rk_stage_noalloc: scratch array address not resolved");
   float* racc[SC_MAX_FIELDS] = {};
   for (int f = 0; f < c->nf; ++f) racc[f] = base + f * stride;
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, stream);
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, nullptr, stream);
 }
 
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               void* stream) {
+                               unsigned* speed_bits, void* stream) {
   if (c == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
     return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
   if (form == SC_RK_AVG)
@@ -520,33 +544,36 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
   a.c0 = c0;
   a.c1 = c1;
   a.dt = dt_dev;
+  a.speed_bits = speed_bits;
   return run_sweeps(c, a, (hipStream_t)stream);
 }
 
 static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c1,
                                  const float* dt_dev, const float* const* u0,
-                                 float* const* uk, float* const* out, void* stream,
-                                 void* ev_start, void* ev_stop);
+                                 float* const* uk, float* const* out, unsigned* speed_bits,
+                                 void* stream, void* ev_start, void* ev_stop);
 
 sc_status sc_sweep_only(sc_ctx* c, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0, float* const* uk,
                         float* const* out, void* stream) {
-  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, stream, nullptr, nullptr);
+  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, nullptr, stream, nullptr,
+                         nullptr);
 }
 
 sc_status sc_sweep_only_timed(sc_ctx* c, int dim, int form, float c0, float c1,
                               const float* dt_dev, const float* const* u0, float* const* uk,
-                              float* const* out, void* ev_start, void* ev_stop,
-                              void* stream) {
+                              float* const* out, float* speed_dev, void* ev_start,
+                              void* ev_stop, void* stream) {
   if (ev_start == nullptr || ev_stop == nullptr)
     return fail(SC_ERR_INVALID, "sc_sweep_only_timed: null event");
-  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, stream, ev_start, ev_stop);
+  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, (unsigned*)speed_dev, stream,
+                         ev_start, ev_stop);
 }
 
 static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c1,
                                  const float* dt_dev, const float* const* u0,
-                                 float* const* uk, float* const* out, void* stream,
-                                 void* ev_start, void* ev_stop) {
+                                 float* const* uk, float* const* out, unsigned* speed_bits,
+                                 void* stream, void* ev_start, void* ev_stop) {
   if (c == nullptr || uk == nullptr) return fail(SC_ERR_INVALID, "sc_sweep_only: null argument");
   if (dim < 0 || dim > 2) return fail(SC_ERR_INVALID, "sc_sweep_only: dim must be 0..2");
   if (form != 0 && (out == nullptr || dt_dev == nullptr || (form != SC_RK_EULER && u0 == nullptr)))
@@ -562,6 +589,7 @@ static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c
   a.c0 = c0;
   a.c1 = c1;
   a.dt = dt_dev;
+  a.speed_bits = form == 0 ? nullptr : speed_bits;
   sc::SweepTuning t = c->tune;
   t.ev_start = (hipEvent_t)ev_start;
   t.ev_stop = (hipEvent_t)ev_stop;
@@ -584,9 +612,7 @@ sc_status sc_max_wave_speed(sc_ctx* c, const float* const* q, float cfl, float* 
   a.pe_x = c->pe_x;
   a.pe_y = c->pe_y;
   a.pe_z = c->pe_z;
-  float hmin = c->h[0];
-  if (c->h[1] < hmin) hmin = c->h[1];
-  if (c->h[2] < hmin) hmin = c->h[2];
+  const float hmin = min_spacing(c);
   return hip_check(sc::launch_max_speed(a, (unsigned*)out_dev, cfl, hmin, dt_dev,
                                         (hipStream_t)stream),
                    "max speed kernel");

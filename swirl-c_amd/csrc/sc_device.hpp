@@ -126,6 +126,18 @@ __device__ __forceinline__ void cons_to_prim(const float* q, float pe, float* w)
   w[kH] = __builtin_fmaf(w[kP], inv, w[kE]);
 }
 
+// Largest signal speed max(|u|, |v|, |w|) + c of one cell's conserved fields
+// (c = sqrt(gamma p / rho), IEEE division and square root), the CFL wave
+// speed of demo/kelvin_helmholtz_instability.py:66-73 taken per cell.  One
+// function for every kernel that computes it, so their maxima agree bit for
+// bit.
+__device__ __forceinline__ float wave_speed(const float* q, float pe) {
+  float w[7];
+  cons_to_prim<0>(q, pe, w);
+  const float c = sqrtf(kGamma * w[kP] / w[kRho]);
+  return fmaxf(fmaxf(fabsf(w[kU]) + c, fabsf(w[kV]) + c), fabsf(w[kW]) + c);
+}
+
 // ---------------------------------------------------------------------------
 // Face reconstruction of one variable at one cell.  a[] holds the cell values
 // at offsets -R..R.  Returns fp = left state of face c+1/2 and fm = right

@@ -287,12 +287,8 @@ __global__ void __launch_bounds__(256) max_speed_kernel(const SpeedArgs a, unsig
       for (int u = 0; u < U; ++u) {
         const int j = j0 + 64 * u;
         if (j >= a.h + a.ny) break;
-        float w[7];
         const float pe = a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
-        cons_to_prim<0>(q[u], pe, w);
-        const float c = sqrtf(kGamma * w[kP] / w[kRho]);
-        const float s = fmaxf(fmaxf(fabsf(w[kU]) + c, fabsf(w[kV]) + c), fabsf(w[kW]) + c);
-        m = fmaxf(m, s);
+        m = fmaxf(m, wave_speed(q[u], pe));
       }
     }
   }
@@ -310,6 +306,13 @@ __global__ void __launch_bounds__(256) max_speed_kernel(const SpeedArgs a, unsig
 
 __global__ void cfl_dt_kernel(const unsigned* bits, float cfl, float hmin, float* dt) {
   *dt = cfl * hmin / __uint_as_float(*bits);
+}
+
+// dt from the accumulated maximum, then the maximum cleared for the next
+// accumulation (one thread: the read precedes the write).
+__global__ void cfl_dt_reset_kernel(unsigned* bits, float cfl, float hmin, float* dt) {
+  *dt = cfl * hmin / __uint_as_float(*bits);
+  *bits = 0u;
 }
 
 __global__ void batch_speed_init_kernel(unsigned* bits, int n) {
@@ -477,6 +480,12 @@ hipError_t launch_halo_prim_finish(const HaloArgs& a, hipStream_t stream) {
   const long long n = (long long)a.N[0] * a.N[1] * a.N[2];
   const unsigned grid = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(halo_prim_finish_kernel, grid, 256, 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl_dt_reset(unsigned* max_bits, float cfl, float hmin, float* dt,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(cfl_dt_reset_kernel, 1, 1, 0, stream, max_bits, cfl, hmin, dt);
   return hipGetLastError();
 }
 

@@ -93,6 +93,10 @@ struct SpeedArgs {
 
 // *max_bits = bits of max over interior of max_d(|u_d| + c); then
 // *dt = cfl * hmin / max if dt is not null.
+// dt = cfl * hmin / (max speed held in max_bits), then max_bits = 0.
+hipError_t launch_cfl_dt_reset(unsigned* max_bits, float cfl, float hmin, float* dt,
+                               hipStream_t stream);
+
 hipError_t launch_max_speed(const SpeedArgs& a, unsigned* max_bits, float cfl,
                             float hmin, float* dt, hipStream_t stream);
 

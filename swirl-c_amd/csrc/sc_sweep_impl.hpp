@@ -107,6 +107,9 @@ struct SweepArgs {
   // (compressible_navier_stokes.py:273-335).
   const float* p_ref;
   const float* rho_ref;
+  // Last sweep of an RK stage: when set, the largest wave speed of the
+  // stage's output cells is max-accumulated here (float bits).
+  unsigned* speed_bits;
 };
 
 // The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
@@ -503,10 +506,13 @@ __device__ __forceinline__ void load_out_ops(const SweepArgs& a, long long u, un
 // Stores R - div (rk_form 0) into R, else the RK stage update (rk3.py:69-83,
 // rk2.py:67-86): 1 out = uk + L*dt; 2 out = u0*c0 + (uk + L*dt)*c1;
 // 3 out = 0.5*(u0 + (uk + dt*L)); one branch on the form for all fields.
+// With a.speed_bits set, the output cell's wave speed is max-folded into
+// smax (pe: the cell's gravitational potential).
 template <int NF>
 __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsigned lb,
                                           const OutOps<NF>& o, const float* uk,
-                                          const float* div, float dt) {
+                                          const float* div, float dt, float pe,
+                                          float& smax) {
   float val[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) val[f] = o.r[f] - div[f] * a.inv_hdir;
@@ -514,21 +520,25 @@ __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsig
     const unsigned l = lane_off(lb);
 #pragma unroll
     for (int f = 0; f < NF; ++f) st_plane(a.r.p[f] + u, l, val[f]);
-  } else if (a.rk_form == 2) {
-    const unsigned l = lane_off(lb);
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-      st_plane(a.out.p[f] + u, l, o.u0[f] * a.c0 + (uk[f] + val[f] * dt) * a.c1);
-  } else if (a.rk_form == 1) {
-    const unsigned l = lane_off(lb);
-#pragma unroll
-    for (int f = 0; f < NF; ++f) st_plane(a.out.p[f] + u, l, uk[f] + val[f] * dt);
-  } else {
-    const unsigned l = lane_off(lb);
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-      st_plane(a.out.p[f] + u, l, 0.5f * (o.u0[f] + (uk[f] + dt * val[f])));
+    return;
   }
+  float res[NF];
+  if (a.rk_form == 2) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) res[f] = o.u0[f] * a.c0 + (uk[f] + val[f] * dt) * a.c1;
+  } else if (a.rk_form == 1) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) res[f] = uk[f] + val[f] * dt;
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) res[f] = 0.5f * (o.u0[f] + (uk[f] + dt * val[f]));
+  }
+  {
+    const unsigned l = lane_off(lb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) st_plane(a.out.p[f] + u, l, res[f]);
+  }
+  if (a.speed_bits != nullptr) smax = fmaxf(smax, wave_speed(res, pe));
 }
 
 // BB: build with the boundary face / flux value handling (only launched when
@@ -601,8 +611,10 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
   // iteration ahead so the loads overlap the arithmetic of the current cell.
   float qn[NF];
   load_raw_u<NF>(a, uo(c_begin - 1 + RAD), lb, qn);
-  // The last sweep's RK time step (a device scalar).
+  // The last sweep's RK time step (a device scalar) and the running maximum
+  // of its output cells' wave speeds.
   const float dt = (PHASE == 2 && a.rk_form != 0) ? *a.dt : 0.0f;
+  float smax = 0.0f;
   for (int c = c_begin - 1; c <= c_end; ++c) {
     // Loads are issued in the order their values are consumed (the memory
     // counter a wait names is in order): the transverse neighbours (used
@@ -687,7 +699,8 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
           float div[NF];
 #pragma unroll
           for (int f = 0; f < NF; ++f) div[f] = F[f] - Fprev[f];
-          store_out<NF>(a, u, lb, oo, hq[0], div, dt);
+          store_out<NF>(a, u, lb, oo, hq[0], div, dt,
+                        pe_at_u<LEAN>(a, kk(c - 1), ii(c - 1), jb, lb), smax);
         }
       }
 #pragma unroll
@@ -714,6 +727,15 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
         for (int f = 0; f < NF; ++f) hq[m][f] = hq[m + 1][f];
 #pragma unroll
       for (int f = 0; f < NF; ++f) hq[NH - 1][f] = qraw[f];
+    }
+  }
+  if constexpr (PHASE == 2) {
+    // One atomic per wave: the maximum over the wave's active lanes (each
+    // lane's maximum is finite or 0: fmaxf drops NaN, as the stand-alone
+    // wave-speed kernel does; non-negative floats order as their bits).
+    if (a.speed_bits != nullptr && a.rk_form != 0) {
+      const unsigned m = __builtin_amdgcn_wave_reduce_max_u32(__float_as_uint(smax), 0);
+      if (ti.tx == 0) atomicMax(a.speed_bits, m);  // lane 0 is active if any lane is
     }
   }
 }

@@ -34,6 +34,7 @@ EXPORTED = (
     'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
     'sc_face_bc_elementwise', 'sc_last_error',
     'sc_version', 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
+    'sc_rk_stage_speed', 'sc_cfl_dt',
 )
 
 
@@ -142,6 +143,8 @@ def lib():
     L.sc_rhs.argtypes = [vp, fpp, fpp, vp]
     L.sc_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_rk_stage_noalloc.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
+    L.sc_rk_stage_speed.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp, vp]
+    L.sc_cfl_dt.argtypes = [vp, vp, f, vp, vp]
     L.sc_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
     ip3 = ctypes.POINTER(ctypes.c_int)
     L.sc_face_interp.argtypes = [vp, ip3, i, i, vp, vp, vp]
@@ -156,7 +159,7 @@ def lib():
     L.sc_face_bc_elementwise.argtypes = [i, vp, vp, vp, ctypes.c_longlong, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
-    L.sc_sweep_only_timed.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp, vp, vp]
+    L.sc_sweep_only_timed.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp, vp, vp, vp]
     L.sc_batch_create.argtypes = [ctypes.POINTER(ScParams), i, i, ctypes.POINTER(ScInstance),
                                   ctypes.POINTER(vp)]
     L.sc_batch_destroy.argtypes = [vp]
@@ -182,7 +185,8 @@ def lib():
                  'sc_batch_set_tuning', 'sc_ctx_set_reference_state', 'sc_face_interp',
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
                  'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise',
-                 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values'):
+                 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
+                 'sc_rk_stage_speed', 'sc_cfl_dt'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

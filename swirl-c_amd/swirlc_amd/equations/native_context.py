@@ -387,6 +387,22 @@ class NativeContext:
         N.ptr_array(u0) if u0 is not None else None,
         N.ptr_array(uk), N.ptr_array(out), N.stream_handle(stream)), 'sc_rk_stage')
 
+  def rk_stage_speed(self, form, c0, c1, dt_dev, u0, uk, out, speed_dev, stream=None):
+    """rk_stage that also max-accumulates out's CFL wave speed into
+    speed_dev (as max_wave_speed computes it; cleared by cfl_dt)."""
+    N.check(N.lib().sc_rk_stage_speed(
+        self._ctx, form, ctypes.c_float(c0), ctypes.c_float(c1),
+        ctypes.c_void_p(dt_dev.data_ptr()),
+        N.ptr_array(u0) if u0 is not None else None,
+        N.ptr_array(uk), N.ptr_array(out), ctypes.c_void_p(speed_dev.data_ptr()),
+        N.stream_handle(stream)), 'sc_rk_stage_speed')
+
+  def cfl_dt(self, speed_dev, cfl, dt_dev, stream=None):
+    """dt_dev = cfl * min spacing / speed_dev, then speed_dev = 0."""
+    N.check(N.lib().sc_cfl_dt(
+        self._ctx, ctypes.c_void_p(speed_dev.data_ptr()), ctypes.c_float(cfl),
+        ctypes.c_void_p(dt_dev.data_ptr()), N.stream_handle(stream)), 'sc_cfl_dt')
+
   def sweep_only(self, dim, form, c0, c1, dt_dev, u0, uk, out, stream=None):
     """Launches one directional sweep kernel alone (for kernel timing)."""
     N.check(N.lib().sc_sweep_only(
@@ -397,14 +413,16 @@ class NativeContext:
         N.stream_handle(stream)), 'sc_sweep_only')
 
   def sweep_only_timed(self, dim, form, c0, c1, dt_dev, u0, uk, out, ev_start, ev_stop,
-                       stream=None):
+                       stream=None, speed_dev=None):
     """sweep_only with the kernel's own start / stop times recorded into two
-    torch.cuda.Event objects (each recorded once before, so it exists)."""
+    torch.cuda.Event objects (each recorded once before, so it exists);
+    speed_dev as in rk_stage_speed."""
     N.check(N.lib().sc_sweep_only_timed(
         self._ctx, int(dim), int(form), ctypes.c_float(c0), ctypes.c_float(c1),
         ctypes.c_void_p(dt_dev.data_ptr()) if dt_dev is not None else None,
         N.ptr_array(u0) if u0 is not None else None, N.ptr_array(uk),
         N.ptr_array(out) if out is not None else None,
+        ctypes.c_void_p(speed_dev.data_ptr()) if speed_dev is not None else None,
         ctypes.c_void_p(ev_start.cuda_event), ctypes.c_void_p(ev_stop.cuda_event),
         N.stream_handle(stream)), 'sc_sweep_only_timed')
 
