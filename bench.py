@@ -120,16 +120,17 @@ def parse():
 
 
 def time_step_kernels(solver, steps):
-  """Kernel durations inside RK3 steps: a step's launch sequence (the CFL dt
-  from the maximum wave speed the previous step's last stage accumulated,
-  then per stage the halo fill and the x, y, z sweeps, the last one with the
-  stage update, and in stage 3 the wave-speed accumulation) issued through
-  the C ABI on one stream.  Each sweep launch records its own start and stop
-  times into HIP events (sc_sweep_only_timed), so nothing is inserted between
-  the kernels; the dt kernel and halo fill are bracketed by events recorded
-  on the stream.  Returns {kernel: [ms per launch at stage 1, 2, 3]} (the dt
-  kernel as a one-element list) averaged over `steps`, and the mean step time
-  (ms)."""
+  """Kernel durations inside RK3 steps: the launch sequence of the solver's
+  chained step (the CFL dt from the maximum wave speed the previous step's
+  last stage accumulated, then per stage the x, y, z sweeps, the last one with
+  the stage update, the output's periodic halo images and in stage 3 the
+  wave-speed accumulation; on grids that are not all-periodic a halo fill of
+  the stage output follows) issued through the C ABI on one stream.  Each
+  sweep launch records its own start and stop times into HIP events
+  (sc_sweep_only_timed), so nothing is inserted between the kernels; the dt
+  kernel and any halo fill are bracketed by events recorded on the stream.
+  Returns {kernel: [ms per launch at stage 1, 2, 3]} (the dt kernel as a
+  one-element list) averaged over `steps`, and the mean step time (ms)."""
   import torch
   from swirlc_amd import _native as N
   ctx = solver.ctx
@@ -137,9 +138,12 @@ def time_step_kernels(solver, steps):
   u, s1, s2 = solver.u, solver.s1, solver.s2
   stages = ((N.SC_RK_EULER, 0.0, 0.0, None, u, s1), (N.SC_RK_BLEND, 0.75, 0.25, u, s1, s2),
             (N.SC_RK_BLEND, 1.0 / 3.0, 2.0 / 3.0, u, s2, u))
-  names = ('halo', 'sweep_x', 'sweep_y', 'sweep_z')
+  names = ('sweep_x', 'sweep_y', 'sweep_z')
   acc = {n: [0.0, 0.0, 0.0] for n in names}
   acc['cfl_dt'] = [0.0]
+  fused_halo = ctx.out_images_fused()
+  if not fused_halo:
+    acc['halo'] = [0.0, 0.0, 0.0]
   total = 0.0
 
   def event():
@@ -147,7 +151,7 @@ def time_step_kernels(solver, steps):
     e.record(stream)  # creates the HIP event
     return e
 
-  solver.refresh_speed()
+  solver.refresh()
   for _ in range(steps):
     t_begin, t_speed = event(), event()
     halo = [(event(), event()) for _ in range(3)]
@@ -158,19 +162,22 @@ def time_step_kernels(solver, steps):
     ctx.cfl_dt(solver.speed, float(solver.cfl), solver.dt, stream)
     t_speed.record(stream)
     for st, (form, c0, c1, base, uk, out) in enumerate(stages):
-      halo[st][0].record(stream)
-      ctx.fill_halos(uk, stream)
-      halo[st][1].record(stream)
       for j, d in enumerate((0, 1, 2)):
         e0, e1 = sweeps[st][j]
         ctx.sweep_only_timed(d, form, c0, c1, solver.dt, base, uk, out, e0, e1, stream,
+                             flags=N.SC_STAGE_FILL_OUT,
                              speed_dev=solver.speed if (st == 2 and d == 2) else None)
+      if not fused_halo:
+        halo[st][0].record(stream)
+        ctx.fill_halos(out, stream)
+        halo[st][1].record(stream)
     t_end.record(stream)
     t_end.synchronize()
     acc['cfl_dt'][0] += t_begin.elapsed_time(t_speed)
     for st in range(3):
-      acc['halo'][st] += halo[st][0].elapsed_time(halo[st][1])
-      for j, n in enumerate(('sweep_x', 'sweep_y', 'sweep_z')):
+      if not fused_halo:
+        acc['halo'][st] += halo[st][0].elapsed_time(halo[st][1])
+      for j, n in enumerate(names):
         e0, e1 = sweeps[st][j]
         acc[n][st] += e0.elapsed_time(e1)
     total += t_begin.elapsed_time(t_end)

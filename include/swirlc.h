@@ -203,20 +203,30 @@ sc_status sc_ctx_set_reference_state(sc_ctx* ctx, const float* p_ref, const floa
 sc_status sc_max_wave_speed(sc_ctx* ctx, const float* const* q, float cfl,
                             float* out_dev, float* dt_dev, void* stream);
 
-/* sc_rk_stage that also max-accumulates the wave speed of sc_max_wave_speed
- * over out's interior cells into *speed_dev (a float in device memory,
- * compared as its bits: clear it with sc_cfl_dt or to 0 before the first
- * accumulation).  The last sweep computes it from the values it writes, so a
- * time step's stage 3 leaves the next step's CFL maximum without re-reading
- * the state; the result equals sc_max_wave_speed on out bit for bit.  No
- * reference counterpart (the reference takes a fixed cfg.dt). */
-sc_status sc_rk_stage_speed(sc_ctx* ctx, int form, float c0, float c1,
-                            const float* dt_dev, const float* const* u0,
-                            float* const* uk, float* const* out, float* speed_dev,
-                            void* stream);
+/* Flags of sc_rk_stage_ex. */
+#define SC_STAGE_HALOS_READY 1 /* uk's halo cells already hold their BC values */
+#define SC_STAGE_FILL_OUT 2    /* leave out with its halo cells filled */
+
+/* sc_rk_stage for a sequence of stages (no reference counterpart; the
+ * reference's rk3.integrate refills halos at every RHS evaluation, which this
+ * does too unless told otherwise).  SC_STAGE_HALOS_READY: uk's halos are not
+ * filled again (the caller states they hold the fill of uk's interior, e.g.
+ * uk was the out of a call with SC_STAGE_FILL_OUT).  SC_STAGE_FILL_OUT: out's
+ * halo cells are filled before the call returns (on all-periodic grids whose
+ * extents are all >= 2h the last sweep writes each output cell's periodic
+ * images, the same values the fill copies; otherwise sc_fill_halos runs on
+ * out).  speed_dev (or NULL): a float in device memory into which the wave
+ * speed of sc_max_wave_speed over out's interior cells is max-accumulated
+ * (compared as its bits; clear it with sc_cfl_dt or to 0 first) - the last
+ * sweep computes it from the values it writes, so a time step's stage 3 leaves
+ * the next step's CFL maximum without re-reading the state, equal to
+ * sc_max_wave_speed on out bit for bit. */
+sc_status sc_rk_stage_ex(sc_ctx* ctx, int form, float c0, float c1, const float* dt_dev,
+                         const float* const* u0, float* const* uk, float* const* out,
+                         int flags, float* speed_dev, void* stream);
 
 /* dt = cfl * min(dx,dy,dz) / *speed_dev written to *dt_dev, then *speed_dev
- * cleared to 0 (for the next sc_rk_stage_speed accumulation).  No reference
+ * cleared to 0 (for the next sc_rk_stage_ex accumulation).  No reference
  * counterpart. */
 sc_status sc_cfl_dt(sc_ctx* ctx, float* speed_dev, float cfl, float* dt_dev, void* stream);
 
@@ -314,13 +324,15 @@ sc_status sc_sweep_only(sc_ctx* ctx, int dim, int form, float c0, float c1,
 /* Profiling: sc_sweep_only whose kernel launch records its own start and stop
  * times into the HIP events ev_start / ev_stop (hipExtLaunchKernel: no extra
  * packets between kernels), so a kernel's duration inside a step sequence
- * can be read with hipEventElapsedTime.  speed_dev (or NULL): as in
- * sc_rk_stage_speed, for the z sweep of an RK form.  No reference
- * counterpart. */
+ * can be read with hipEventElapsedTime.  flags (SC_STAGE_FILL_OUT only: the
+ * periodic images the z sweep writes on all-periodic grids) and speed_dev
+ * (or NULL) as in sc_rk_stage_ex, for the z sweep of an RK form.  No
+ * reference counterpart. */
 sc_status sc_sweep_only_timed(sc_ctx* ctx, int dim, int form, float c0, float c1,
                               const float* dt_dev, const float* const* u0,
-                              float* const* uk, float* const* out, float* speed_dev,
-                              void* ev_start, void* ev_stop, void* stream);
+                              float* const* uk, float* const* out, int flags,
+                              float* speed_dev, void* ev_start, void* ev_stop,
+                              void* stream);
 
 /* Launch geometry of the sweep kernels (no reference counterpart: the
  * reference leaves scheduling to XLA).  seg_*: cells per thread of a marching

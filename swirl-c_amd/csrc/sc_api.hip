@@ -454,21 +454,23 @@ sc_status sc_rhs(sc_ctx* c, float* const* q, float* const* rhs, void* stream) {
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               unsigned* speed_bits, void* stream);
+                               int flags, unsigned* speed_bits, void* stream);
 
 sc_status sc_rk_stage(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
                       const float* const* u0, float* const* uk, float* const* out,
                       void* stream) {
   if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, nullptr, stream);
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, 0, nullptr,
+                       stream);
 }
 
-sc_status sc_rk_stage_speed(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
-                            const float* const* u0, float* const* uk, float* const* out,
-                            float* speed_dev, void* stream) {
-  if (c == nullptr || speed_dev == nullptr)
-    return fail(SC_ERR_INVALID, "sc_rk_stage_speed: null argument");
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch,
+sc_status sc_rk_stage_ex(sc_ctx* c, int form, float c0, float c1, const float* dt_dev,
+                         const float* const* u0, float* const* uk, float* const* out, int flags,
+                         float* speed_dev, void* stream) {
+  if (c == nullptr) return fail(SC_ERR_INVALID, "sc_rk_stage_ex: null argument");
+  if (flags & ~(SC_STAGE_HALOS_READY | SC_STAGE_FILL_OUT))
+    return fail(SC_ERR_INVALID, "sc_rk_stage_ex: unknown flag");
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, c->rhs_scratch, flags,
                        (unsigned*)speed_dev, stream);
 }
 
@@ -506,13 +508,13 @@ sc_status sc_rk_stage_noalloc(sc_ctx* c, int form, float c0, float c1, const flo
     return fail(SC_ERR_HIP, "sc_rk_stage_noalloc: scratch array address not resolved");
   float* racc[SC_MAX_FIELDS] = {};
   for (int f = 0; f < c->nf; ++f) racc[f] = base + f * stride;
-  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, nullptr, stream);
+  return rk_stage_with(c, form, c0, c1, dt_dev, u0, uk, out, racc, 0, nullptr, stream);
 }
 
 static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, float* const* racc,
-                               unsigned* speed_bits, void* stream) {
+                               int flags, unsigned* speed_bits, void* stream) {
   if (c == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
     return fail(SC_ERR_INVALID, "sc_rk_stage: null argument");
   if (form == SC_RK_AVG)
@@ -531,8 +533,10 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
     if (form != SC_RK_EULER && u0[f] == nullptr)
       return fail(SC_ERR_INVALID, "sc_rk_stage: null u0 field pointer");
   }
-  sc_status st = sc_fill_halos(c, uk, stream);
-  if (st != SC_OK) return st;
+  if (!(flags & SC_STAGE_HALOS_READY)) {
+    sc_status st = sc_fill_halos(c, uk, stream);
+    if (st != SC_OK) return st;
+  }
   sc::SweepArgs a = c->sweep;
   for (int f = 0; f < c->nf; ++f) {
     a.q.p[f] = uk[f];
@@ -545,35 +549,44 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
   a.c1 = c1;
   a.dt = dt_dev;
   a.speed_bits = speed_bits;
-  return run_sweeps(c, a, (hipStream_t)stream);
+  // out's halos: written by the last sweep as periodic images where that is
+  // exactly the fill (all-periodic, every extent >= 2h), else filled after.
+  const bool fill_out = (flags & SC_STAGE_FILL_OUT) != 0;
+  a.out_images = fill_out && c->halo_all_periodic && a.nx >= 2 * a.h && a.ny >= 2 * a.h &&
+                 a.nz >= 2 * a.h;
+  sc_status st = run_sweeps(c, a, (hipStream_t)stream);
+  if (st != SC_OK || !fill_out || a.out_images) return st;
+  return sc_fill_halos(c, out, stream);
 }
 
 static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c1,
                                  const float* dt_dev, const float* const* u0,
-                                 float* const* uk, float* const* out, unsigned* speed_bits,
-                                 void* stream, void* ev_start, void* ev_stop);
+                                 float* const* uk, float* const* out, int flags,
+                                 unsigned* speed_bits, void* stream, void* ev_start,
+                                 void* ev_stop);
 
 sc_status sc_sweep_only(sc_ctx* c, int dim, int form, float c0, float c1,
                         const float* dt_dev, const float* const* u0, float* const* uk,
                         float* const* out, void* stream) {
-  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, nullptr, stream, nullptr,
-                         nullptr);
+  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, 0, nullptr, stream,
+                         nullptr, nullptr);
 }
 
 sc_status sc_sweep_only_timed(sc_ctx* c, int dim, int form, float c0, float c1,
                               const float* dt_dev, const float* const* u0, float* const* uk,
-                              float* const* out, float* speed_dev, void* ev_start,
-                              void* ev_stop, void* stream) {
+                              float* const* out, int flags, float* speed_dev,
+                              void* ev_start, void* ev_stop, void* stream) {
   if (ev_start == nullptr || ev_stop == nullptr)
     return fail(SC_ERR_INVALID, "sc_sweep_only_timed: null event");
-  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, (unsigned*)speed_dev, stream,
-                         ev_start, ev_stop);
+  return sweep_only_with(c, dim, form, c0, c1, dt_dev, u0, uk, out, flags,
+                         (unsigned*)speed_dev, stream, ev_start, ev_stop);
 }
 
 static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c1,
                                  const float* dt_dev, const float* const* u0,
-                                 float* const* uk, float* const* out, unsigned* speed_bits,
-                                 void* stream, void* ev_start, void* ev_stop) {
+                                 float* const* uk, float* const* out, int flags,
+                                 unsigned* speed_bits, void* stream, void* ev_start,
+                                 void* ev_stop) {
   if (c == nullptr || uk == nullptr) return fail(SC_ERR_INVALID, "sc_sweep_only: null argument");
   if (dim < 0 || dim > 2) return fail(SC_ERR_INVALID, "sc_sweep_only: dim must be 0..2");
   if (form != 0 && (out == nullptr || dt_dev == nullptr || (form != SC_RK_EULER && u0 == nullptr)))
@@ -590,6 +603,8 @@ static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c
   a.c1 = c1;
   a.dt = dt_dev;
   a.speed_bits = form == 0 ? nullptr : speed_bits;
+  a.out_images = form != 0 && (flags & SC_STAGE_FILL_OUT) && c->halo_all_periodic &&
+                 a.nx >= 2 * a.h && a.ny >= 2 * a.h && a.nz >= 2 * a.h;
   sc::SweepTuning t = c->tune;
   t.ev_start = (hipEvent_t)ev_start;
   t.ev_stop = (hipEvent_t)ev_stop;

@@ -110,6 +110,10 @@ struct SweepArgs {
   // Last sweep of an RK stage: when set, the largest wave speed of the
   // stage's output cells is max-accumulated here (float bits).
   unsigned* speed_bits;
+  // Last sweep of an RK stage on an all-periodic grid with every extent >= h:
+  // also write each output cell's periodic images (the halo cells the
+  // periodic fill would copy from it), so out leaves with its halos filled.
+  int out_images;
 };
 
 // The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
@@ -506,13 +510,49 @@ __device__ __forceinline__ void load_out_ops(const SweepArgs& a, long long u, un
 // Stores R - div (rk_form 0) into R, else the RK stage update (rk3.py:69-83,
 // rk2.py:67-86): 1 out = uk + L*dt; 2 out = u0*c0 + (uk + L*dt)*c1;
 // 3 out = 0.5*(u0 + (uk + dt*L)); one branch on the form for all fields.
+// Periodic images of the output cell (k, i, j) whose lane-0 cell sits at
+// uniform element offset u (k, i wave-uniform, j per lane), for extents >= 2h:
+// along each dim the cell is copied n further when it lies within h of the
+// low face (c < 2h) or n back when within h of the high face (c >= n), and to
+// every combination of those shifts, which are exactly the halo cells the
+// all-periodic fill copies from it.
+template <int NF>
+__device__ __forceinline__ void store_images(const SweepArgs& a, long long u, unsigned lb,
+                                             int k, int i, int j, const float* res) {
+  const int h2 = 2 * a.h;
+  const long long zs = k < h2 ? (long long)a.nz * a.sz : (k >= a.nz ? -(long long)a.nz * a.sz : 0);
+  const long long xs = i < h2 ? (long long)a.nx * a.sx : (i >= a.nx ? -(long long)a.nx * a.sx : 0);
+  const bool ylo = j < h2, yhi = j >= a.ny;
+  // Rare stores: their addresses are formed per lane in vector registers
+  // (the byte offset of cell u + lane plus the image shift), which holds no
+  // scalar base per image.
+  const long long lane_byte = (long long)lane_off(lb);
+#pragma unroll
+  for (int zm = 0; zm < 2; ++zm) {
+    if (zm && zs == 0) continue;
+#pragma unroll
+    for (int xm = 0; xm < 2; ++xm) {
+      if (xm && xs == 0) continue;
+      const long long vb = (u + (zm ? zs : 0) + (xm ? xs : 0)) * 4 + lane_byte;
+#pragma unroll
+      for (int ym = -1; ym <= 1; ++ym) {
+        if (ym == 0 ? !(zm || xm) : !(ym > 0 ? ylo : yhi)) continue;
+        const long long b = vb + (long long)ym * a.ny * 4;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) *(float*)((char*)a.out.p[f] + b) = res[f];
+      }
+    }
+  }
+}
+
 // With a.speed_bits set, the output cell's wave speed is max-folded into
-// smax (pe: the cell's gravitational potential).
+// smax (pe: the cell's gravitational potential); with a.out_images set, the
+// cell's periodic images are written too.
 template <int NF>
 __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsigned lb,
                                           const OutOps<NF>& o, const float* uk,
                                           const float* div, float dt, float pe,
-                                          float& smax) {
+                                          float& smax, int k, int i, int j) {
   float val[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) val[f] = o.r[f] - div[f] * a.inv_hdir;
@@ -538,6 +578,7 @@ __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsig
 #pragma unroll
     for (int f = 0; f < NF; ++f) st_plane(a.out.p[f] + u, l, res[f]);
   }
+  if (a.out_images) store_images<NF>(a, u, lb, k, i, j, res);
   if (a.speed_bits != nullptr) smax = fmaxf(smax, wave_speed(res, pe));
 }
 
@@ -700,7 +741,8 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 #pragma unroll
           for (int f = 0; f < NF; ++f) div[f] = F[f] - Fprev[f];
           store_out<NF>(a, u, lb, oo, hq[0], div, dt,
-                        pe_at_u<LEAN>(a, kk(c - 1), ii(c - 1), jb, lb), smax);
+                        pe_at_u<LEAN>(a, kk(c - 1), ii(c - 1), jb, lb), smax, kk(c - 1),
+                        ii(c - 1), j);
         }
       }
 #pragma unroll

@@ -22,6 +22,7 @@ SC_HLL, SC_HLLC = 0, 1
 SC_BC_PERIODIC, SC_BC_DIRICHLET, SC_BC_NEUMANN, SC_BC_REFLECTIVE = 0, 1, 2, 3
 SC_BC_NONE = -1  # no BC along a dim (its halo cells are left alone)
 SC_RK_EULER, SC_RK_BLEND, SC_RK_AVG = 1, 2, 3
+SC_STAGE_HALOS_READY, SC_STAGE_FILL_OUT = 1, 2
 
 # Every entry point include/swirlc.h declares.
 EXPORTED = (
@@ -34,7 +35,7 @@ EXPORTED = (
     'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
     'sc_face_bc_elementwise', 'sc_last_error',
     'sc_version', 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
-    'sc_rk_stage_speed', 'sc_cfl_dt',
+    'sc_rk_stage_ex', 'sc_cfl_dt',
 )
 
 
@@ -143,7 +144,7 @@ def lib():
     L.sc_rhs.argtypes = [vp, fpp, fpp, vp]
     L.sc_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
     L.sc_rk_stage_noalloc.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
-    L.sc_rk_stage_speed.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp, vp]
+    L.sc_rk_stage_ex.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, i, vp, vp]
     L.sc_cfl_dt.argtypes = [vp, vp, f, vp, vp]
     L.sc_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
     ip3 = ctypes.POINTER(ctypes.c_int)
@@ -159,7 +160,7 @@ def lib():
     L.sc_face_bc_elementwise.argtypes = [i, vp, vp, vp, ctypes.c_longlong, vp]
     L.sc_hllc_branch_codes.argtypes = [vp, fpp, i, vp, vp]
     L.sc_sweep_only.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp]
-    L.sc_sweep_only_timed.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, vp, vp, vp, vp]
+    L.sc_sweep_only_timed.argtypes = [vp, i, i, f, f, vp, fpp, fpp, fpp, i, vp, vp, vp, vp]
     L.sc_batch_create.argtypes = [ctypes.POINTER(ScParams), i, i, ctypes.POINTER(ScInstance),
                                   ctypes.POINTER(vp)]
     L.sc_batch_destroy.argtypes = [vp]
@@ -186,7 +187,7 @@ def lib():
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
                  'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise',
                  'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
-                 'sc_rk_stage_speed', 'sc_cfl_dt'):
+                 'sc_rk_stage_ex', 'sc_cfl_dt'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

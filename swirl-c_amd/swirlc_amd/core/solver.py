@@ -38,34 +38,43 @@ class Solver:
     self.use_graph = use_graph
     self._graph = None
     self.steps_taken = 0
-    # RK3 with a CFL time step: each step's last stage also accumulates the
-    # wave speed of the state it writes, so the next step's dt needs no
-    # separate pass over the state (seeded from the initial state on the
-    # first step; call refresh_speed() after changing the state in place).
-    self._fused_cfl = (self.cfl is not None and self.scheme == 'rk3' and
-                       hasattr(self.ctx, 'rk_stage_speed'))
-    self._speed_seeded = False
+    # RK3 stages chained through sc_rk_stage_ex: each stage leaves its output
+    # with the halo cells filled (on all-periodic grids its last sweep writes
+    # them), so the next stage fills none, and with a CFL time step the last
+    # stage also accumulates the wave speed of the state it writes, so the
+    # next step's dt needs no separate pass over the state.  The first step
+    # seeds both from the state; call refresh() after changing the state in
+    # place between steps.
+    self._chained = self.scheme == 'rk3' and hasattr(self.ctx, 'rk_stage_ex')
+    self._seeded = False
 
   @property
   def state(self):
     return dict(zip(self.ctx.names, self.u))
 
-  def refresh_speed(self):
-    """Seeds the CFL maximum from the current state (fused-CFL stepping)."""
-    if self._fused_cfl:
-      self.ctx.max_wave_speed(self.u, self.speed, float(self.cfl), self.dt)
-      self._speed_seeded = True
+  def refresh(self):
+    """Fills the state's halos and seeds the CFL maximum from the state (the
+    chained stepping keeps both current from step to step)."""
+    if self._chained:
+      self.ctx.fill_halos(self.u)
+      if self.cfl is not None:
+        self.ctx.max_wave_speed(self.u, self.speed, float(self.cfl), self.dt)
+      self._seeded = True
 
   def _step_body(self):
     c = self.ctx
-    if self._fused_cfl:
-      # dt from the maximum the previous step's last stage left, which is
-      # then cleared for this step's last stage to accumulate into.
-      c.cfl_dt(self.speed, float(self.cfl), self.dt)
-      c.rk_stage(N.SC_RK_EULER, 0.0, 0.0, self.dt, None, self.u, self.s1)
-      c.rk_stage(N.SC_RK_BLEND, _C11, 1.0 - _C11, self.dt, self.u, self.s1, self.s2)
-      c.rk_stage_speed(N.SC_RK_BLEND, _C21, 1.0 - _C21, self.dt, self.u, self.s2, self.u,
-                       self.speed)
+    if self._chained:
+      flags = N.SC_STAGE_HALOS_READY | N.SC_STAGE_FILL_OUT
+      speed = None
+      if self.cfl is not None:
+        # dt from the maximum the previous step's last stage left, which is
+        # then cleared for this step's last stage to accumulate into.
+        c.cfl_dt(self.speed, float(self.cfl), self.dt)
+        speed = self.speed
+      c.rk_stage_ex(N.SC_RK_EULER, 0.0, 0.0, self.dt, None, self.u, self.s1, flags)
+      c.rk_stage_ex(N.SC_RK_BLEND, _C11, 1.0 - _C11, self.dt, self.u, self.s1, self.s2, flags)
+      c.rk_stage_ex(N.SC_RK_BLEND, _C21, 1.0 - _C21, self.dt, self.u, self.s2, self.u, flags,
+                    speed)
       return
     if self.cfl is not None:
       c.max_wave_speed(self.u, self.speed, float(self.cfl), self.dt)
@@ -92,8 +101,8 @@ class Solver:
     self._graph = g
 
   def step(self, n=1):
-    if self._fused_cfl and not self._speed_seeded:
-      self.refresh_speed()
+    if self._chained and not self._seeded:
+      self.refresh()
     for _ in range(n):
       if self.use_graph:
         if self._graph is None:

@@ -376,6 +376,15 @@ class NativeContext:
     N.check(N.lib().sc_fill_halos(self._ctx, N.ptr_array(q), N.stream_handle(stream)),
             'sc_fill_halos')
 
+  def out_images_fused(self):
+    """Whether rk_stage_ex with SC_STAGE_FILL_OUT fills out's halos inside its
+    last sweep (every face periodic, no primitive-variable BC, every extent
+    >= 2 h), rather than with a halo fill after it."""
+    p = self._params
+    periodic = all(p.bc[f][d][s].type == N.SC_BC_PERIODIC
+                   for f in range(self.nf) for d in range(3) for s in range(2))
+    return periodic and p.prim_bc_mask == 0 and min(self.n) >= 2 * self.hw
+
   def rhs(self, q, r, stream=None):
     N.check(N.lib().sc_rhs(self._ctx, N.ptr_array(q), N.ptr_array(r),
                            N.stream_handle(stream)), 'sc_rhs')
@@ -387,15 +396,19 @@ class NativeContext:
         N.ptr_array(u0) if u0 is not None else None,
         N.ptr_array(uk), N.ptr_array(out), N.stream_handle(stream)), 'sc_rk_stage')
 
-  def rk_stage_speed(self, form, c0, c1, dt_dev, u0, uk, out, speed_dev, stream=None):
-    """rk_stage that also max-accumulates out's CFL wave speed into
-    speed_dev (as max_wave_speed computes it; cleared by cfl_dt)."""
-    N.check(N.lib().sc_rk_stage_speed(
+  def rk_stage_ex(self, form, c0, c1, dt_dev, u0, uk, out, flags=0, speed_dev=None,
+                  stream=None):
+    """rk_stage with sc_rk_stage_ex's flags (SC_STAGE_HALOS_READY: uk's halos
+    are not refilled; SC_STAGE_FILL_OUT: out leaves with its halos filled)
+    and, with speed_dev, out's CFL wave speed max-accumulated into it (as
+    max_wave_speed computes it; cleared by cfl_dt)."""
+    N.check(N.lib().sc_rk_stage_ex(
         self._ctx, form, ctypes.c_float(c0), ctypes.c_float(c1),
         ctypes.c_void_p(dt_dev.data_ptr()),
         N.ptr_array(u0) if u0 is not None else None,
-        N.ptr_array(uk), N.ptr_array(out), ctypes.c_void_p(speed_dev.data_ptr()),
-        N.stream_handle(stream)), 'sc_rk_stage_speed')
+        N.ptr_array(uk), N.ptr_array(out), int(flags),
+        ctypes.c_void_p(speed_dev.data_ptr()) if speed_dev is not None else None,
+        N.stream_handle(stream)), 'sc_rk_stage_ex')
 
   def cfl_dt(self, speed_dev, cfl, dt_dev, stream=None):
     """dt_dev = cfl * min spacing / speed_dev, then speed_dev = 0."""
@@ -413,15 +426,15 @@ class NativeContext:
         N.stream_handle(stream)), 'sc_sweep_only')
 
   def sweep_only_timed(self, dim, form, c0, c1, dt_dev, u0, uk, out, ev_start, ev_stop,
-                       stream=None, speed_dev=None):
+                       stream=None, flags=0, speed_dev=None):
     """sweep_only with the kernel's own start / stop times recorded into two
     torch.cuda.Event objects (each recorded once before, so it exists);
-    speed_dev as in rk_stage_speed."""
+    flags and speed_dev as in rk_stage_ex."""
     N.check(N.lib().sc_sweep_only_timed(
         self._ctx, int(dim), int(form), ctypes.c_float(c0), ctypes.c_float(c1),
         ctypes.c_void_p(dt_dev.data_ptr()) if dt_dev is not None else None,
         N.ptr_array(u0) if u0 is not None else None, N.ptr_array(uk),
-        N.ptr_array(out) if out is not None else None,
+        N.ptr_array(out) if out is not None else None, int(flags),
         ctypes.c_void_p(speed_dev.data_ptr()) if speed_dev is not None else None,
         ctypes.c_void_p(ev_start.cuda_event), ctypes.c_void_p(ev_stop.cuda_event),
         N.stream_handle(stream)), 'sc_sweep_only_timed')

@@ -1,11 +1,17 @@
-"""CFL wave speed accumulated by the last sweep of an RK stage.
+"""Chained RK stages (sc_rk_stage_ex): halo cells and the CFL wave speed
+produced by the last sweep of a stage.
 
-sc_rk_stage_speed max-accumulates the wave speed of the cells it writes; it
-must equal sc_max_wave_speed on the stage's output bit for bit (both take the
-same per-cell function and a maximum, which is exact in any order), and a
-Solver stepping with it must reproduce the stepping that runs the separate
-wave-speed pass every step, state and dt, bit for bit.  The oracle pins the
-wave speed itself (test_gpu_rhs_parity.py::test_max_wave_speed_matches_oracle).
+With SC_STAGE_FILL_OUT a stage leaves its output with the halo cells filled
+(on all-periodic grids the last sweep writes every output cell's periodic
+images); they must equal what sc_fill_halos writes, bit for bit, corners
+included.  With speed_dev the stage max-accumulates the wave speed of the
+cells it writes; it must equal sc_max_wave_speed on the output bit for bit
+(both take the same per-cell function and a maximum, which is exact in any
+order).  A Solver stepping through chained stages must reproduce the plain
+stepping (halo fill at every stage, a separate wave-speed pass every step),
+state and dt, bit for bit.  The oracle pins the wave speed itself
+(test_gpu_rhs_parity.py::test_max_wave_speed_matches_oracle) and the halo fill
+(test_gpu_rhs_parity.py, tests/test_oracle_reference_cases.py).
 """
 
 import copy
@@ -59,8 +65,8 @@ def test_stage_speed_equals_separate_pass(gravity, bcs, scheme, flux, nscalar):
     acc = torch.zeros(4, device='cuda')
     # Two calls into one accumulator: the second's maximum is the same, so
     # the accumulation must leave it unchanged.
-    ctx.rk_stage_speed(form, c0, c1, dt, u0, q, out, acc)
-    ctx.rk_stage_speed(form, c0, c1, dt, u0, q, out, acc)
+    ctx.rk_stage_ex(form, c0, c1, dt, u0, q, out, speed_dev=acc)
+    ctx.rk_stage_ex(form, c0, c1, dt, u0, q, out, speed_dev=acc)
     ref_out = [torch.zeros_like(t) for t in q]
     ctx.rk_stage(form, c0, c1, dt, u0, q, ref_out)
     sep = torch.zeros(4, device='cuda')
@@ -91,20 +97,73 @@ def test_cfl_dt_resets_the_maximum():
   assert speed[0].item() == 0.0
 
 
+@pytest.mark.parametrize('n,bcs', [
+    ((70, 13, 11), 'periodic'),   # images written by the last sweep
+    ((9, 12, 6), 'periodic'),     # an extent below 2h: fill after the sweeps
+    ((14, 11, 9), 'mixed'),       # not periodic: fill after the sweeps
+])
+def test_stage_output_halos_equal_the_fill(n, bcs):
+  h = 3
+  cfg = _ctx_cfg(n, h, False, bcs)
+  st = random_state(n, h, seed=23)
+  ctx = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg))).native_context
+  q = ctx.fields({k: torch.from_numpy(v.copy()).cuda() for k, v in st.items()})
+  u0 = [t.clone() for t in q]
+  dt = torch.full((1,), 1e-3, device='cuda')
+  fused = ctx.out_images_fused()
+  assert fused == (bcs == 'periodic' and min(n) >= 2 * h)
+  for form, c0, c1 in ((N.SC_RK_EULER, 0.0, 0.0), (N.SC_RK_BLEND, 1.0 / 3.0, 2.0 / 3.0)):
+    out = [torch.full_like(t, float('nan')) for t in q]
+    ctx.rk_stage_ex(form, c0, c1, dt, u0, q, out, N.SC_STAGE_FILL_OUT)
+    ref = [t.clone() for t in out]
+    ctx.fill_halos(ref)
+    plain = [torch.zeros_like(t) for t in q]
+    ctx.rk_stage(form, c0, c1, dt, u0, q, plain)
+    torch.cuda.synchronize()
+    for a, b, c in zip(out, ref, plain):
+      assert torch.equal(a, b)  # every cell, halos and corners included
+      sl = (slice(h, -h),) * 3
+      assert torch.equal(a[sl], c[sl])
+    # The next stage with SC_STAGE_HALOS_READY on that output equals a plain
+    # stage that fills the halos itself.
+    nxt = [torch.zeros_like(t) for t in q]
+    ctx.rk_stage_ex(N.SC_RK_EULER, 0.0, 0.0, dt, None, out, nxt, N.SC_STAGE_HALOS_READY)
+    nxt_ref = [torch.zeros_like(t) for t in q]
+    ctx.rk_stage(N.SC_RK_EULER, 0.0, 0.0, dt, None, ref, nxt_ref)
+    torch.cuda.synchronize()
+    sl = (slice(h, -h),) * 3
+    for a, b in zip(nxt, nxt_ref):
+      assert torch.equal(a[sl], b[sl])
+  print(f'[halo] {n} {bcs}: fused images {fused}', flush=True)
+
+
+def test_stage_ex_rejects_unknown_flags():
+  n, h = (8, 8, 8), 3
+  cfg = _ctx_cfg(n, h, False, 'periodic')
+  ctx = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg))).native_context
+  q = [torch.zeros(ctx.shape, device='cuda') for _ in range(5)]
+  dt = torch.zeros(1, device='cuda')
+  with pytest.raises(Exception):
+    ctx.rk_stage_ex(N.SC_RK_EULER, 0.0, 0.0, dt, None, q, [t.clone() for t in q], 4)
+
+
 @pytest.mark.parametrize('use_graph', [True, False])
-def test_solver_with_fused_speed_matches_separate_pass(use_graph):
+@pytest.mark.parametrize('cfl', [0.5, None])
+def test_chained_solver_matches_plain_stepping(use_graph, cfl):
   cfg_d, st = configs.kh(n=32)
   cfg = SwirlCParameters(copy.deepcopy(cfg_d))
   gpu = {k: torch.from_numpy(v).cuda() for k, v in st.items()}
-  fused = Solver(cfg, gpu, cfl=0.5, use_graph=use_graph)
-  plain = Solver(cfg, gpu, cfl=0.5, use_graph=use_graph)
-  assert fused._fused_cfl
-  plain._fused_cfl = False  # the separate wave-speed pass every step
+  chained = Solver(cfg, gpu, cfl=cfl, use_graph=use_graph)
+  plain = Solver(cfg, gpu, cfl=cfl, use_graph=use_graph)
+  assert chained._chained and chained.ctx.out_images_fused()
+  plain._chained = False  # halo fill at every stage, separate wave-speed pass
   for _ in range(3):
-    fused.step(2)
+    chained.step(2)
     plain.step(2)
     torch.cuda.synchronize()
-    assert fused.dt.item() == plain.dt.item()
-    for a, b in zip(fused.u, plain.u):
-      assert torch.equal(a, b)
-  print(f'[cfl] solver dt after 6 steps {fused.dt.item()!r}', flush=True)
+    assert chained.dt.item() == plain.dt.item()
+    h = chained.ctx.hw
+    sl = (slice(h, -h),) * 3
+    for a, b in zip(chained.u, plain.u):
+      assert torch.equal(a[sl], b[sl])
+  print(f'[chain] cfl {cfl}: dt after 6 steps {chained.dt.item()!r}', flush=True)
