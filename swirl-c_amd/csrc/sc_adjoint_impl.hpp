@@ -43,7 +43,7 @@ struct AdjArgs {
 // adjoint of the cell's dynamic pressure p - p_ref (a constant 0 outside
 // the interior along the pass's direction).  The gravity source takes
 // rho - rho_ref with a reference state: the same derivative.
-template <int NS = 0, bool PD = false>
+template <int NS = 0, bool PD = false, bool LEAN = false>
 __device__ __forceinline__ void retire_cell(const AdjArgs& A, long long idx, const float* W,
                                             float pe, const float* lw, bool src,
                                             bool interior) {
@@ -61,7 +61,7 @@ __device__ __forceinline__ void retire_cell(const AdjArgs& A, long long idx, con
     cons_to_prim_adj<NS>(W, pe, lw, lq);
   }
   const SweepArgs& a = A.s;
-  if (src && interior && a.grav_src) {
+  if (!LEAN && src && interior && a.grav_src) {
     // R[1+d] += (rho * G) * g_d on the selected momenta.
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -152,7 +152,7 @@ __device__ __forceinline__ void face_riemann_adj(const BoundaryBc* bb, long long
 // ---------------------------------------------------------------------------
 // BB: the build with boundary face / flux values; PD: with a reference state
 // (the dynamic pressure is an eighth reconstructed variable).
-template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD, bool LEAN = false>
 __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
@@ -198,7 +198,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
   for (int w = 0; w < WN; ++w) {
     const int c = c_start - 1 - RAD + w;
-    load_prim<NS, PD>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at(a, kk(c), ii(c), j),
+    load_prim<NS, PD, LEAN>(a, base + (long long)c * sm, in_core(c, a.h, n), pe_at<LEAN>(a, kk(c), ii(c), j),
                  W[w]);
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc(w, v) = 0.0f;
@@ -220,13 +220,13 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
   load_g(c_start - 1, gPrev);
   float tPrev[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
   float ltPrev[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
-  if constexpr (VISC) transverse_grads<DIR>(a, kk(c_start - 1), ii(c_start - 1), j, tPrev);
+  if constexpr (VISC) transverse_grads<DIR, LEAN>(a, kk(c_start - 1), ii(c_start - 1), j, tPrev);
 
   for (int c = c_start; c <= c_last; ++c) {
     {
       const int cn = min(c + RAD, NM - 1);
-      load_prim<NS, PD>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
-                   pe_at(a, kk(cn), ii(cn), j), W[NW - 1]);
+      load_prim<NS, PD, LEAN>(a, base + (long long)cn * sm, in_core(cn, a.h, n),
+                   pe_at<LEAN>(a, kk(cn), ii(cn), j), W[NW - 1]);
     }
     const bool face = c <= f_hi;
     // Reconstruction of cell c: right state of face c, left state of c+1.
@@ -237,6 +237,11 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
       for (int w = 0; w < WN; ++w) s[w] = W[w + 1][v];
       recon_replay<SCHEME>(s, fpC[v], fmC[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      pin(fpC[v]);
+      pin(fmC[v]);
     }
     float gC[NF];
     load_g(c, gC);
@@ -258,9 +263,14 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
       if (bb != nullptr) flux_bc_adj<NF>(*bb, lF, lFc, lFd);
       // Boundary plane cell of this face: x faces [z][y], z faces [x][y].
       face_riemann_adj<FLUX, DIR, NS, PD>(bb, (long long)o * a.NY + j, fpPrev, fmC,
-                                  pe_at(a, kk(c), ii(c), j), lFc, lL, lR);
+                                  pe_at<LEAN>(a, kk(c), ii(c), j), lFc, lL, lR);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        pin(lL[v]);
+        pin(lR[v]);
+      }
       if constexpr (VISC) {
-        transverse_grads<DIR>(a, kk(c), ii(c), j, tCur);
+        transverse_grads<DIR, LEAN>(a, kk(c), ii(c), j, tCur);
         // Cells c-1 and c sit at window positions RAD and RAD+1.
         float lA[NV], lB[NV];
 #pragma unroll
@@ -302,7 +312,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
         acc(s0, v) = 0.0f;
       }
       if (owned(x))
-        retire_cell<NS, PD>(A, base + (long long)x * sm, W[0], pe_at(a, kk(x), ii(x), j), l0,
+        retire_cell<NS, PD, LEAN>(A, base + (long long)x * sm, W[0], pe_at<LEAN>(a, kk(x), ii(x), j), l0,
                         DIR == 0,
                     interior(x));
       s0 = s0 + 1 < WN ? s0 + 1 : 0;
@@ -335,7 +345,7 @@ __device__ __forceinline__ void adj_march_body(const AdjArgs& A, const TileIdx& 
 #pragma unroll
     for (int v = 0; v < NV; ++v) l0[v] = acc(slot(w), v);
     if (owned(x))
-      retire_cell<NS, PD>(A, base + (long long)x * sm, W[w], pe_at(a, kk(x), ii(x), j), l0,
+      retire_cell<NS, PD, LEAN>(A, base + (long long)x * sm, W[w], pe_at<LEAN>(a, kk(x), ii(x), j), l0,
                       DIR == 0,
                   interior(x));
   }
@@ -346,6 +356,16 @@ __global__ void __launch_bounds__(256) adj_march_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
   adj_march_body<DIR, SCHEME, FLUX, VISC, NS, BB, PD>(A, ti);
+}
+
+// The LEAN build (no gravity, primitive-variable BCs, boundary values or
+// reference state) fits three waves per SIMD (168 VGPRs).
+template <int DIR, int SCHEME, int FLUX, bool VISC, int NS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+adj_march_lean_kernel(const AdjArgs A) {
+  const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
+                   (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
+  adj_march_body<DIR, SCHEME, FLUX, VISC, NS, false, false, true>(A, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -374,7 +394,7 @@ __host__ __device__ inline int y_adj_windows(int nx, int ny, int h, int rad, int
   return (int)((span + outa - 1) / outa);
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD, bool LEAN = false>
 __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) {
   constexpr int RAD = YAdjGeom<SCHEME>::RAD;
   constexpr int KA = YAdjGeom<SCHEME>::KA;
@@ -411,10 +431,10 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   const bool face = valid && yc >= a.h && yc <= a.h + a.ny;
   const bool rcell = valid && yc >= a.h - 1 && yc <= a.h + a.ny;
   const bool inner = valid && yc >= a.h && yc < a.h + a.ny;
-  const float pe = pe_at(a, k, i, yc);
+  const float pe = pe_at<LEAN>(a, k, i, yc);
 
   float W0[NV];
-  load_prim<NS, PD>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
+  load_prim<NS, PD, LEAN>(a, idx, in_core(yc, a.h, a.ny), pe, W0);
   float S[NV][WN], fp[NV], fm[NV], Am[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -439,7 +459,7 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
   for (int v = 0; v < NV; ++v) lL[v] = lR[v] = lA[v] = lB[v] = 0.0f;
   float lta[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, ltb[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
   float tCur[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}}, tPrev[2][2];
-  if constexpr (VISC) transverse_grads<1>(a, k, i, yc, tCur);
+  if constexpr (VISC) transverse_grads<1, LEAN>(a, k, i, yc, tCur);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     tPrev[t][0] = wave_shr1(tCur[t][0]);
@@ -490,25 +510,26 @@ __device__ __forceinline__ void adj_y_body(const AdjArgs& A, const TileIdx& ti) 
     lt[t][1] = ltb[t][1] + wave_shl1(lta[t][1]);
   }
   if (out_lane) {
-    retire_cell<NS, PD>(A, idx, W0, pe, tot, false, inner);
+    retire_cell<NS, PD, LEAN>(A, idx, W0, pe, tot, false, inner);
     if constexpr (VISC) {
       if (rcell) add_lG<1>(A, idx, lt);
     }
   }
 }
 
-template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD, bool LEAN = false>
 __global__ void __launch_bounds__(256) adj_y_kernel(const AdjArgs A) {
   const TileIdx ti{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z,
                    (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
-  adj_y_body<SCHEME, FLUX, VISC, NS, BB, PD>(A, ti);
+  adj_y_body<SCHEME, FLUX, VISC, NS, BB, PD, LEAN>(A, ti);
 }
 
 
 // Launches the three adjoint passes (x, y, z) for one scheme.
-template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool BB, bool PD, bool LEAN = false>
 hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& tune,
                                hipStream_t stream) {
+  static_assert(!LEAN || (!BB && !PD), "LEAN builds carry no boundary values or reference state");
   constexpr int RAD = Radius<SCHEME>::value;
   const SweepArgs& a = A.s;
   const int seg = tune.adj_seg, by = tune.adj_by, by_y = tune.adj_by_y;
@@ -522,23 +543,32 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
     ax.s.inv_hdir = 1.0f / h[0];
     const int ext = a.nx + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nz + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_m, lds, stream,
-                       ax);
+    if constexpr (LEAN)
+      hipLaunchKernelGGL((adj_march_lean_kernel<0, SCHEME, FLUX, VISC, NS>), grid, blk_m, lds,
+                         stream, ax);
+    else
+      hipLaunchKernelGGL((adj_march_kernel<0, SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_m, lds,
+                         stream, ax);
   }
   {
     AdjArgs ay = A;
     ay.s.inv_hdir = 1.0f / h[1];
     const int nwin = y_adj_windows(a.nx, a.ny, a.h, RAD, YAdjGeom<SCHEME>::OUTA);
     const dim3 grid((nwin + by_y - 1) / by_y, 1, a.nz);
-    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_y, 0, stream, ay);
+    hipLaunchKernelGGL((adj_y_kernel<SCHEME, FLUX, VISC, NS, BB, PD, LEAN>), grid, blk_y, 0,
+                       stream, ay);
   }
   {
     AdjArgs az = A;
     az.s.inv_hdir = 1.0f / h[2];
     const int ext = a.nz + 2 * RAD + 2;
     const dim3 grid((a.ny + 63) / 64, (a.nx + by - 1) / by, (ext + seg - 1) / seg);
-    hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_m, lds, stream,
-                       az);
+    if constexpr (LEAN)
+      hipLaunchKernelGGL((adj_march_lean_kernel<2, SCHEME, FLUX, VISC, NS>), grid, blk_m, lds,
+                         stream, az);
+    else
+      hipLaunchKernelGGL((adj_march_kernel<2, SCHEME, FLUX, VISC, NS, BB, PD>), grid, blk_m, lds,
+                         stream, az);
   }
   return hipGetLastError();
 }
@@ -548,9 +578,16 @@ hipError_t launch_adj_sweeps_t(AdjArgs A, const float h[3], const SweepTuning& t
 template <int SCHEME, int FLUX, bool VISC>
 hipError_t launch_adj_variant(const AdjArgs& A, const float h[3], int ns,
                               const SweepTuning& tune, hipStream_t stream) {
-  if (ns == 1) return launch_adj_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(A, h, tune, stream);
-  if (ns == 2) return launch_adj_sweeps_t<SCHEME, FLUX, VISC, 2, false, false>(A, h, tune, stream);
   const bool bb = A.s.bbc_dims != 0, pd = A.s.p_ref != nullptr;
+  const SweepArgs& a = A.s;
+  const bool lean = !bb && !pd && a.prim_bc_mask == 0 && !a.has_pe && !a.grav_src;
+  if (ns == 1)
+    return lean ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, 1, false, false, true>(A, h, tune, stream)
+                : launch_adj_sweeps_t<SCHEME, FLUX, VISC, 1, false, false>(A, h, tune, stream);
+  if (ns == 2)
+    return lean ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, 2, false, false, true>(A, h, tune, stream)
+                : launch_adj_sweeps_t<SCHEME, FLUX, VISC, 2, false, false>(A, h, tune, stream);
+  if (lean) return launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, false, false, true>(A, h, tune, stream);
   if (pd)
     return bb ? launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, true, true>(A, h, tune, stream)
               : launch_adj_sweeps_t<SCHEME, FLUX, VISC, 0, false, true>(A, h, tune, stream);
