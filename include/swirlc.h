@@ -428,6 +428,18 @@ sc_status sc_batch_fill_halos(sc_batch* batch, float* const* q, void* stream);
 sc_status sc_batch_rk_stage(sc_batch* batch, int form, float c0, float c1,
                             const float* dt_dev, const float* const* u0,
                             float* const* uk, float* const* out, void* stream);
+/* sc_rk_stage_ex's flags for every instance (no reference counterpart):
+ * SC_STAGE_HALOS_READY skips the fill of uk; SC_STAGE_FILL_OUT leaves out's
+ * halos filled.  Where the fill is a pure copy along every dim - each dim
+ * PERIODIC on both faces (every instance's extent >= 2h) or NEUMANN with
+ * value 0 on both faces, the same for every field, no BC on a primitive -
+ * the z sweep writes each output cell's images, the halo cells the fill
+ * would copy from it (the wrapped or the nearest interior cell along each
+ * dim); otherwise sc_batch_fill_halos runs on out. */
+sc_status sc_batch_rk_stage_ex(sc_batch* batch, int form, float c0, float c1,
+                               const float* dt_dev, const float* const* u0,
+                               float* const* uk, float* const* out, int flags,
+                               void* stream);
 /* Tuning of the batched sweeps (sc_tuning above).  Setting it rebuilds the
  * tile lists (allocates; not allowed during graph capture). */
 sc_status sc_batch_get_tuning(const sc_batch* batch, sc_tuning* out);

@@ -41,6 +41,10 @@ struct sc_ctx {
   float* adj_lG[9];
   sc::HaloArgs halo;  // BC tables; field pointers filled per call
   int halo_all_periodic;  // one-launch periodic fill applies
+  // A stage's last sweep writes out's halos as images (all-periodic grids:
+  // copy_fill_dims with no clamping dim); img_clamp as in SweepArgs.
+  int images_ok;
+  int img_clamp;
   sc::SweepArgs sweep;
   sc::SweepTuning tune;
 };
@@ -70,6 +74,39 @@ sc_status fail(sc_status code, const std::string& msg) {
 sc_status hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return SC_OK;
   return fail(SC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Whether the halo fill of every field is a pure copy along every dim, so
+// that a halo cell equals the interior cell at its wrapped (periodic: both
+// faces PERIODIC, extent >= 2h) or clamped (both faces NEUMANN with value
+// +0: ghost = neighbour - 0, cumulatively, copies the edge cell bit for bit)
+// index in each dim; every field must take the same kind along a dim, and no
+// BC may be given on a primitive (those fill in primitive space).  Sets bit d
+// of *clamp for the clamping dims (d = 0 x, 1 y, 2 z).
+bool copy_fill_dims(const sc_params& p, int nf, const int n[3], int h, int* clamp) {
+  if (p.prim_bc_mask != 0) return false;
+  int cl = 0;
+  for (int d = 0; d < 3; ++d) {
+    const int kind = p.bc[0][d][0].type;
+    for (int f = 0; f < nf; ++f)
+      for (int s = 0; s < 2; ++s) {
+        const sc_bc_face& b = p.bc[f][d][s];
+        if (b.type != kind) return false;
+        if (kind == SC_BC_NEUMANN &&
+            (b.value != 0.0f || signbit(b.value) || b.n_layer_values != 0 ||
+             b.layer_planes != nullptr))
+          return false;
+      }
+    if (kind == SC_BC_PERIODIC) {
+      if (n[d] < 2 * h) return false;
+    } else if (kind == SC_BC_NEUMANN) {
+      cl |= 1 << d;
+    } else {
+      return false;
+    }
+  }
+  *clamp = cl;
+  return true;
 }
 
 // common/utils.py:114-142.
@@ -345,6 +382,17 @@ sc_status sc_ctx_create(const sc_grid* grid, const sc_params* params, sc_ctx** o
     for (int d = 0; d < 3; ++d)
       for (int s = 0; s < 2; ++s)
         if (p.bc[f][d][s].type != SC_BC_PERIODIC) c->halo_all_periodic = 0;
+  {
+    // Single grids take the images on all-periodic grids only: a march whose
+    // edge cells write h clamped images each (NEUMANN 0) was measured slower
+    // than the fill after the sweeps on grids of 32^3 to 160x64x64 (z sweep
+    // 57 -> 123 us: the edge waves' extra stores lengthen a launch with few
+    // waves), while the batched launches, with many waves, gain.
+    const int n3[3] = {c->grid.nx, c->grid.ny, c->grid.nz};
+    c->img_clamp = 0;
+    c->images_ok = copy_fill_dims(p, c->nf, n3, c->grid.halo, &c->img_clamp) && c->img_clamp == 0;
+    c->img_clamp = 0;  // the single-grid sweeps are built without clamped images
+  }
 
   // Sweep constants.
   sc::SweepArgs& a = c->sweep;
@@ -549,11 +597,11 @@ static sc_status rk_stage_with(sc_ctx* c, int form, float c0, float c1,
   a.c1 = c1;
   a.dt = dt_dev;
   a.speed_bits = speed_bits;
-  // out's halos: written by the last sweep as periodic images where that is
-  // exactly the fill (all-periodic, every extent >= 2h), else filled after.
+  // out's halos: written by the last sweep as images where that is exactly
+  // the fill (copy_fill_dims), else filled after.
   const bool fill_out = (flags & SC_STAGE_FILL_OUT) != 0;
-  a.out_images = fill_out && c->halo_all_periodic && a.nx >= 2 * a.h && a.ny >= 2 * a.h &&
-                 a.nz >= 2 * a.h;
+  a.out_images = fill_out && c->images_ok;
+  a.img_clamp = c->img_clamp;
   sc_status st = run_sweeps(c, a, (hipStream_t)stream);
   if (st != SC_OK || !fill_out || a.out_images) return st;
   return sc_fill_halos(c, out, stream);
@@ -603,8 +651,8 @@ static sc_status sweep_only_with(sc_ctx* c, int dim, int form, float c0, float c
   a.c1 = c1;
   a.dt = dt_dev;
   a.speed_bits = form == 0 ? nullptr : speed_bits;
-  a.out_images = form != 0 && (flags & SC_STAGE_FILL_OUT) && c->halo_all_periodic &&
-                 a.nx >= 2 * a.h && a.ny >= 2 * a.h && a.nz >= 2 * a.h;
+  a.out_images = form != 0 && (flags & SC_STAGE_FILL_OUT) && c->images_ok;
+  a.img_clamp = c->img_clamp;
   sc::SweepTuning t = c->tune;
   t.ev_start = (hipEvent_t)ev_start;
   t.ev_stop = (hipEvent_t)ev_stop;
@@ -1078,6 +1126,10 @@ struct sc_batch {
   sc::BatchHaloTile* ptiles_dev = nullptr; // primitive-BC finish tiles
   int nptiles = 0;
   bool full = false;                       // gravity, primitive BCs or boundary values
+  // Every instance's halo fill is a pure copy along every dim
+  // (copy_fill_dims): a stage's last sweep can write out's halos as images.
+  bool images_ok = false;
+  int img_clamp = 0;
   sc::HaloArgs halo_cfg;
   sc::SweepArgs proto;
   sc::SweepTuning tune;
@@ -1103,10 +1155,29 @@ sc_status upload(const std::vector<T>& v, T** dst) {
 // dominated the halo fills' many small tiles).
 int grid_for(int ntiles) { return ntiles > 0 ? ntiles : 1; }
 
+// The dispatcher hands consecutive blocks to the 8 XCDs in turn, and each
+// XCD has its own L2.  Reorders a tile list (one block per tile) so that
+// XCD k runs the k-th contiguous range of the original order: neighbouring
+// tiles of an instance, which share halo rows and transverse-neighbour
+// planes, then hit the same L2.  The same bijection as the single-grid
+// sweeps' block order, applied to the list on the host.
+template <typename T>
+void xcd_order(std::vector<T>& v) {
+  const int n = (int)v.size();
+  const int q = n / sc::kNumXcd, r = n % sc::kNumXcd;
+  std::vector<T> o(v.size());
+  for (int lin = 0; lin < n; ++lin) {
+    const int xcd = lin % sc::kNumXcd, slot = lin / sc::kNumXcd;
+    o[lin] = v[(xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot];
+  }
+  v.swap(o);
+}
+
 // Sweep tile lists of a batch for its current tuning: x sweep (y chunks of
 // 64, z groups, x segments); y sweep (groups of by_y windows of the z
-// plane's run of rows, z); z sweep (y chunks, x groups, z segments).
-// Replaces any previous lists.
+// plane's run of rows, z); z sweep (y chunks, x groups, z segments); in the
+// XCD order above when the tuning's xcd_remap is set.  Replaces any
+// previous lists.
 sc_status build_sweep_tiles(sc_batch* b) {
   const int rad = radius_of(b->params.interpolation);
   const int outw = 64 - 2 * rad - 2;
@@ -1125,6 +1196,7 @@ sc_status build_sweep_tiles(sc_batch* b) {
         for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[2].push_back({i, bx, by, bz});
   }
   for (int d = 0; d < 3; ++d) {
+    if (t.xcd_remap) xcd_order(tl[d]);
     if (b->tiles_dev[d]) hipFree(b->tiles_dev[d]);
     b->tiles_dev[d] = nullptr;
     const sc_status st = upload(tl[d], &b->tiles_dev[d]);
@@ -1374,6 +1446,15 @@ sc_status sc_batch_create(const sc_params* params, int halo, int n, const sc_ins
   a.bbc_dims = bbc_dims;
   a.vc.nu = (float)p.nu;
   a.vc.kfac = (float)(1.0 / p.pr) * sc::kCp;
+  // Halo images from the last sweep: the copy-fill condition must hold for
+  // every instance's extents (its periodic dims at least 2h long).
+  b->images_ok = true;
+  for (int i = 0; i < n && b->images_ok; ++i) {
+    const int n3[3] = {inst[i].nx, inst[i].ny, inst[i].nz};
+    int clamp = 0;
+    b->images_ok = copy_fill_dims(p, nf, n3, halo, &clamp);
+    b->img_clamp = clamp;
+  }
   *out = b;
   return SC_OK;
 }
@@ -1397,28 +1478,37 @@ sc_status sc_batch_fill_halos(sc_batch* b, float* const* q, void* stream) {
   return SC_OK;
 }
 
-sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const float* dt_dev,
-                            const float* const* u0, float* const* uk, float* const* out,
-                            void* stream) {
+sc_status sc_batch_rk_stage_ex(sc_batch* b, int form, float c0, float c1,
+                               const float* dt_dev, const float* const* u0, float* const* uk,
+                               float* const* out, int flags, void* stream) {
   if (b == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
-    return fail(SC_ERR_INVALID, "sc_batch_rk_stage: null argument");
+    return fail(SC_ERR_INVALID, "sc_batch_rk_stage_ex: null argument");
   if (form != SC_RK_EULER && form != SC_RK_BLEND)
-    return fail(SC_ERR_UNSUPPORTED, "sc_batch_rk_stage: form must be SC_RK_EULER or SC_RK_BLEND");
+    return fail(SC_ERR_UNSUPPORTED, "sc_batch_rk_stage_ex: form must be SC_RK_EULER or SC_RK_BLEND");
   if (form == SC_RK_BLEND && u0 == nullptr)
-    return fail(SC_ERR_INVALID, "sc_batch_rk_stage: u0 required");
+    return fail(SC_ERR_INVALID, "sc_batch_rk_stage_ex: u0 required");
   for (int f = 0; f < b->nf; ++f) {
     if (uk[f] == nullptr || out[f] == nullptr || (form == SC_RK_BLEND && u0[f] == nullptr))
-      return fail(SC_ERR_INVALID, "sc_batch_rk_stage: null arena");
-    if (out[f] == uk[f]) return fail(SC_ERR_INVALID, "sc_batch_rk_stage: out must not alias uk");
+      return fail(SC_ERR_INVALID, "sc_batch_rk_stage_ex: null arena");
+    if (out[f] == uk[f]) return fail(SC_ERR_INVALID, "sc_batch_rk_stage_ex: out must not alias uk");
   }
-  sc_status st = sc_batch_fill_halos(b, uk, stream);
-  if (st != SC_OK) return st;
+  if (flags & ~(SC_STAGE_HALOS_READY | SC_STAGE_FILL_OUT))
+    return fail(SC_ERR_INVALID, "sc_batch_rk_stage_ex: unknown flags");
+  if (!(flags & SC_STAGE_HALOS_READY)) {
+    const sc_status st = sc_batch_fill_halos(b, uk, stream);
+    if (st != SC_OK) return st;
+  }
+  // out's halos: written by the last sweep as images where that is exactly
+  // the fill (copy_fill_dims for every instance), else filled after.
+  const bool fill_out = (flags & SC_STAGE_FILL_OUT) != 0;
   sc::BatchArgs ba;
   memset(&ba, 0, sizeof(ba));
   ba.proto = b->proto;
   ba.proto.rk_form = form;
   ba.proto.c0 = c0;
   ba.proto.c1 = c1;
+  ba.proto.out_images = fill_out && b->images_ok;
+  ba.proto.img_clamp = b->img_clamp;
   ba.nf = b->nf;
   ba.pe = b->pe_dev;
   for (int f = 0; f < b->nf; ++f) {
@@ -1441,7 +1531,15 @@ sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const flo
   } else {
     e = sc::launch_batch_sweeps<sc::kMuscl>(ba, b->tiles, flux, visc, ns, b->full, b->tune, s);
   }
-  return hip_check(e, "batch sweep kernels");
+  const sc_status st = hip_check(e, "batch sweep kernels");
+  if (st != SC_OK || !fill_out || ba.proto.out_images) return st;
+  return sc_batch_fill_halos(b, out, stream);
+}
+
+sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const float* dt_dev,
+                            const float* const* u0, float* const* uk, float* const* out,
+                            void* stream) {
+  return sc_batch_rk_stage_ex(b, form, c0, c1, dt_dev, u0, uk, out, 0, stream);
 }
 
 sc_status sc_batch_max_wave_speed(sc_batch* b, const float* const* q, float cfl,

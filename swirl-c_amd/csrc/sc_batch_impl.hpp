@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) batch_march_kernel(const BatchArgs b) {
     const SweepArgs a = tile_args(b, tl.inst, DIR);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, false, FULL, !FULL>(a, ti);
+    sweep_march_body<DIR, SCHEME, FLUX, VISC, NS, PHASE, false, FULL, !FULL, true>(a, ti);
   }
 }
 

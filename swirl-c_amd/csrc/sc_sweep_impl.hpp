@@ -110,10 +110,14 @@ struct SweepArgs {
   // Last sweep of an RK stage: when set, the largest wave speed of the
   // stage's output cells is max-accumulated here (float bits).
   unsigned* speed_bits;
-  // Last sweep of an RK stage on an all-periodic grid with every extent >= h:
-  // also write each output cell's periodic images (the halo cells the
-  // periodic fill would copy from it), so out leaves with its halos filled.
+  // Last sweep of an RK stage on a grid whose halo fill is a pure copy along
+  // every dim (periodic: the wrapped interior cell, extent >= 2h; NEUMANN
+  // with value 0: the nearest interior cell): also write each output cell's
+  // images (the halo cells the fill would copy from it), so out leaves with
+  // its halos filled.  Bit d of img_clamp set: dim d clamps (NEUMANN 0),
+  // else it wraps (periodic).
   int out_images;
+  int img_clamp;
 };
 
 // The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
@@ -510,22 +514,39 @@ __device__ __forceinline__ void load_out_ops(const SweepArgs& a, long long u, un
 // Stores R - div (rk_form 0) into R, else the RK stage update (rk3.py:69-83,
 // rk2.py:67-86): 1 out = uk + L*dt; 2 out = u0*c0 + (uk + L*dt)*c1;
 // 3 out = 0.5*(u0 + (uk + dt*L)); one branch on the form for all fields.
-// Periodic images of the output cell (k, i, j) whose lane-0 cell sits at
-// uniform element offset u (k, i wave-uniform, j per lane), for extents >= 2h:
-// along each dim the cell is copied n further when it lies within h of the
-// low face (c < 2h) or n back when within h of the high face (c >= n), and to
-// every combination of those shifts, which are exactly the halo cells the
-// all-periodic fill copies from it.
+// Halo images of the output cell (k, i, j) whose lane-0 cell sits at uniform
+// element offset u (k, i wave-uniform, j per lane): the halo cells that the
+// sequential fill (x, then y, then z, each over full planes) leaves equal to
+// it when every dim's fill is a pure copy.  Along a wrapping dim (periodic,
+// extent n >= 2h) the cell at index c has one image, c + n when c < 2h and
+// c - n when c >= n; along a clamping dim (NEUMANN with value 0) the first
+// interior cell has the h images below it and the last the h above it.  A
+// halo cell's value is then the interior cell at its wrapped / clamped index
+// in every dim, so the images are every combination of the per-dim images
+// (the cell itself included, itself excluded): each halo cell is written
+// once, by the lane of the cell it copies.
+__device__ __forceinline__ int image_count(int c, int n, int h, bool clamp) {
+  if (!clamp) return (c < 2 * h || c >= n) ? 1 : 0;
+  return (c == h ? h : 0) + (c == h + n - 1 ? h : 0);
+}
+// Index shift of the t-th image (0 <= t < image_count).
+__device__ __forceinline__ int image_shift(int c, int n, int h, bool clamp, int t) {
+  if (!clamp) return c < 2 * h ? n : -n;
+  if (c == h && t < h) return -(t + 1);
+  return t - (c == h ? h : 0) + 1;
+}
+
+// All-periodic grids (extents >= 2h): along each dim a cell within h of the
+// low face (c < 2h) has its image n further, one within h of the high face
+// (c >= n) n back, and the images are every combination of those shifts.
 template <int NF>
-__device__ __forceinline__ void store_images(const SweepArgs& a, long long u, unsigned lb,
-                                             int k, int i, int j, const float* res) {
+__device__ __forceinline__ void store_images_wrap(const SweepArgs& a, long long u,
+                                                  unsigned lb, int k, int i, int j,
+                                                  const float* res) {
   const int h2 = 2 * a.h;
   const long long zs = k < h2 ? (long long)a.nz * a.sz : (k >= a.nz ? -(long long)a.nz * a.sz : 0);
   const long long xs = i < h2 ? (long long)a.nx * a.sx : (i >= a.nx ? -(long long)a.nx * a.sx : 0);
   const bool ylo = j < h2, yhi = j >= a.ny;
-  // Rare stores: their addresses are formed per lane in vector registers
-  // (the byte offset of cell u + lane plus the image shift), which holds no
-  // scalar base per image.
   const long long lane_byte = (long long)lane_off(lb);
 #pragma unroll
   for (int zm = 0; zm < 2; ++zm) {
@@ -545,10 +566,59 @@ __device__ __forceinline__ void store_images(const SweepArgs& a, long long u, un
   }
 }
 
+// CLAMP: the build that also takes clamping dims (a.img_clamp, the batched
+// sweeps); the single-grid sweeps, whose images are periodic only, leave
+// that code out.
+template <int NF, bool CLAMP>
+__device__ __forceinline__ void store_images(const SweepArgs& a, long long u, unsigned lb,
+                                             int k, int i, int j, const float* res) {
+  if (!CLAMP || a.img_clamp == 0) {
+    store_images_wrap<NF>(a, u, lb, k, i, j, res);
+    return;
+  }
+  const int h = a.h;
+  const bool cz = (a.img_clamp >> 2) & 1, cx = a.img_clamp & 1, cy = (a.img_clamp >> 1) & 1;
+  // Along z and x (wave-uniform) the images are counted; along y (per lane)
+  // the low / high side of the lane's cell has 1 (wrap) or h (clamp) images.
+  const int nzi = image_count(k, a.nz, h, cz), nxi = image_count(i, a.nx, h, cx);
+  const bool ylo = cy ? j == h : j < 2 * h;
+  const bool yhi = cy ? j == h + a.ny - 1 : j >= a.ny;
+  const int nyi = cy ? h : 1;
+  // Rare stores: their addresses are formed per lane in vector registers
+  // (the byte offset of cell u + lane plus the image shift), which holds no
+  // scalar base per image.
+  const long long lane_byte = (long long)lane_off(lb);
+  for (int tz = -1; tz < nzi; ++tz) {
+    const long long zs = tz < 0 ? 0 : (long long)image_shift(k, a.nz, h, cz, tz) * a.sz;
+    for (int tx = -1; tx < nxi; ++tx) {
+      const long long xs = tx < 0 ? 0 : (long long)image_shift(i, a.nx, h, cx, tx) * a.sx;
+      const long long vb = (u + zs + xs) * 4 + lane_byte;
+      if (tz >= 0 || tx >= 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) *(float*)((char*)a.out.p[f] + vb) = res[f];
+      }
+      if (ylo || yhi) {
+        for (int t = 0; t < nyi; ++t) {
+          if (ylo) {
+            const long long b = vb + (long long)(cy ? -(t + 1) : a.ny) * 4;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) *(float*)((char*)a.out.p[f] + b) = res[f];
+          }
+          if (yhi) {
+            const long long b = vb + (long long)(cy ? t + 1 : -a.ny) * 4;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) *(float*)((char*)a.out.p[f] + b) = res[f];
+          }
+        }
+      }
+    }
+  }
+}
+
 // With a.speed_bits set, the output cell's wave speed is max-folded into
 // smax (pe: the cell's gravitational potential); with a.out_images set, the
 // cell's periodic images are written too.
-template <int NF>
+template <int NF, bool CLAMP>
 __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsigned lb,
                                           const OutOps<NF>& o, const float* uk,
                                           const float* div, float dt, float pe,
@@ -578,14 +648,14 @@ __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsig
 #pragma unroll
     for (int f = 0; f < NF; ++f) st_plane(a.out.p[f] + u, l, res[f]);
   }
-  if (a.out_images) store_images<NF>(a, u, lb, k, i, j, res);
+  if (a.out_images) store_images<NF, CLAMP>(a, u, lb, k, i, j, res);
   if (a.speed_bits != nullptr) smax = fmaxf(smax, wave_speed(res, pe));
 }
 
 // BB: build with the boundary face / flux value handling (only launched when
 // the configuration has such values, so other runs carry none of its code).
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
-          bool BB = false, bool LEAN = false>
+          bool BB = false, bool LEAN = false, bool CLAMP = false>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
@@ -740,7 +810,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
           float div[NF];
 #pragma unroll
           for (int f = 0; f < NF; ++f) div[f] = F[f] - Fprev[f];
-          store_out<NF>(a, u, lb, oo, hq[0], div, dt,
+          store_out<NF, CLAMP>(a, u, lb, oo, hq[0], div, dt,
                         pe_at_u<LEAN>(a, kk(c - 1), ii(c - 1), jb, lb), smax, kk(c - 1),
                         ii(c - 1), j);
         }

@@ -133,6 +133,8 @@ def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=None):
   base = batch.get_tuning()
   if y_space is None:
     y_space = tuple((by, base['seg_y']) for by in (1, 2, 4))
+  if batch._chained and not batch._seeded:
+    batch.refresh()  # the timed stages take the state's halos as filled
   timings = []
   for d, t in _candidates(base, march_space, y_space):
     batch.set_tuning(t)

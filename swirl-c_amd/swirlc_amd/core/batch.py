@@ -3,9 +3,12 @@
 `BatchSolver` advances N simulations that share one configuration (schemes,
 BC types and values, transport, gravity, transported scalars) but have their
 own grid sizes, spacings, initial states and CFL time steps.  Each field of all instances lives in one packed arena;
-every phase of an RK stage (halo fill per dim, x / y / z sweep, wave speed)
-is one launch over a tile list covering all instances, and the whole RK3 step
-is captured once in a HIP graph and replayed.  No reference counterpart:
+every phase of an RK stage (x / y / z sweep, halo fill per dim where one is
+needed, wave speed) is one launch over a tile list covering all instances,
+and the whole RK3 step is captured once in a HIP graph and replayed.  The
+stages are chained: each leaves its output's halos filled (written by the z
+sweep as images where the fill is a pure copy), so after writing to the
+state between steps call refresh().  No reference counterpart:
 the reference runs one simulation per TPU job.
 """
 
@@ -17,7 +20,7 @@ import torch
 from swirlc_amd import _native as N
 from swirlc_amd.common import utils
 from swirlc_amd.core.parameter import SwirlCParameters, _get_cell_center_coordinates
-from swirlc_amd.equations.native_context import NativeContext
+from swirlc_amd.equations.native_context import NativeContext, copy_fill_dims
 
 _C11, _C21 = 0.75, 1.0 / 3.0
 _ALIGN = 64  # elements between instances (256 B alignment of every field)
@@ -103,9 +106,15 @@ class BatchSolver:
       N.check(N.lib().sc_batch_create(ctypes.byref(probe._params), h, self.n, descs,
                                       ctypes.byref(handle)), 'sc_batch_create')
     self._b = handle
+    self._params = probe._params
     self.use_graph = use_graph
     self._graph = None
     self.steps_taken = 0
+    # Chained stages: each stage leaves its output's halos filled (written by
+    # the z sweep as images where the fill is a pure copy), so the next stage,
+    # and the next step, start without a fill.  The first step fills once.
+    self._chained = True
+    self._seeded = False
 
   def view(self, arena, k):
     """Instance k's [nz+2h, nx+2h, ny+2h] view of one field arena."""
@@ -140,12 +149,32 @@ class BatchSolver:
     N.check(N.lib().sc_batch_set_tuning(self._b, ctypes.byref(t)), 'sc_batch_set_tuning')
     self._graph = None
 
+  def out_images_fused(self):
+    """Whether a chained stage writes its output's halos in the z sweep (the
+    halo fill a pure copy along every dim for every instance's extents)."""
+    return all(copy_fill_dims(self._params, self.nf, (nx, ny, nz), self.h) is not None
+               for nz, nx, ny in ((s[0] - 2 * self.h, s[1] - 2 * self.h, s[2] - 2 * self.h)
+                                  for s in self.shapes))
+
+  def refresh(self):
+    """Fills the halos of every instance's state (the chained stepping keeps
+    them current from step to step; call this after writing to the state)."""
+    N.check(N.lib().sc_batch_fill_halos(self._b, N.ptr_array(self.u), N.stream_handle()),
+            'sc_batch_fill_halos')
+    self._seeded = True
+
   def _stage(self, form, c0, c1, u0, uk, out):
-    N.check(N.lib().sc_batch_rk_stage(
-        self._b, form, ctypes.c_float(c0), ctypes.c_float(c1),
-        ctypes.c_void_p(self.dt.data_ptr()),
-        N.ptr_array(u0) if u0 is not None else None, N.ptr_array(uk), N.ptr_array(out),
-        N.stream_handle()), 'sc_batch_rk_stage')
+    u0p = N.ptr_array(u0) if u0 is not None else None
+    dt = ctypes.c_void_p(self.dt.data_ptr())
+    if self._chained:
+      flags = N.SC_STAGE_HALOS_READY | N.SC_STAGE_FILL_OUT
+      N.check(N.lib().sc_batch_rk_stage_ex(
+          self._b, form, ctypes.c_float(c0), ctypes.c_float(c1), dt, u0p, N.ptr_array(uk),
+          N.ptr_array(out), flags, N.stream_handle()), 'sc_batch_rk_stage_ex')
+    else:
+      N.check(N.lib().sc_batch_rk_stage(
+          self._b, form, ctypes.c_float(c0), ctypes.c_float(c1), dt, u0p, N.ptr_array(uk),
+          N.ptr_array(out), N.stream_handle()), 'sc_batch_rk_stage')
 
   def _step_body(self):
     if self.cfl is not None:
@@ -168,6 +197,8 @@ class BatchSolver:
     self._graph = g
 
   def step(self, n=1):
+    if self._chained and not self._seeded:
+      self.refresh()
     for _ in range(n):
       if self.use_graph:
         if self._graph is None:

@@ -10,6 +10,7 @@ import collections
 import copy
 import ctypes
 import hashlib
+import math
 import numbers
 
 import numpy as np
@@ -49,6 +50,37 @@ def validate_models(cfg):
       raise NotImplementedError(
           f'Unsupported source term model: "{name}". Supported models are:'
           " ['gravity']")
+
+
+def copy_fill_dims(params, nf, n, h):
+  """The dims along which the halo fill of every field is a pure copy, so
+  that the last sweep of an RK stage can write its output's halos as images:
+  each dim PERIODIC on both faces with extent >= 2h (wraps), or NEUMANN with
+  value +0 on both faces (ghost = neighbour - 0 copies the edge cell:
+  clamps), the same kind for every field, and no BC on a primitive.
+  n = (nx, ny, nz).  Returns None if some dim is neither, else the set of
+  clamping dims (0 x, 1 y, 2 z).  Mirrors the library's rule."""
+  if params.prim_bc_mask != 0:
+    return None
+  clamp = set()
+  for d in range(3):
+    kind = params.bc[0][d][0].type
+    for f in range(nf):
+      for s in range(2):
+        b = params.bc[f][d][s]
+        if b.type != kind:
+          return None
+        if kind == N.SC_BC_NEUMANN and (b.value != 0.0 or math.copysign(1.0, b.value) < 0
+                                        or b.n_layer_values != 0 or b.layer_planes):
+          return None
+    if kind == N.SC_BC_PERIODIC:
+      if n[d] < 2 * h:
+        return None
+    elif kind == N.SC_BC_NEUMANN:
+      clamp.add(d)
+    else:
+      return None
+  return clamp
 
 
 class NativeContext:
@@ -378,12 +410,9 @@ class NativeContext:
 
   def out_images_fused(self):
     """Whether rk_stage_ex with SC_STAGE_FILL_OUT fills out's halos inside its
-    last sweep (every face periodic, no primitive-variable BC, every extent
-    >= 2 h), rather than with a halo fill after it."""
-    p = self._params
-    periodic = all(p.bc[f][d][s].type == N.SC_BC_PERIODIC
-                   for f in range(self.nf) for d in range(3) for s in range(2))
-    return periodic and p.prim_bc_mask == 0 and min(self.n) >= 2 * self.hw
+    last sweep (all-periodic grids, every extent >= 2h: copy_fill_dims with no
+    clamping dim) rather than with a halo fill after it."""
+    return copy_fill_dims(self._params, self.nf, self.n, self.hw) == set()
 
   def rhs(self, q, r, stream=None):
     N.check(N.lib().sc_rhs(self._ctx, N.ptr_array(q), N.ptr_array(r),

@@ -2,9 +2,11 @@
 produced by the last sweep of a stage.
 
 With SC_STAGE_FILL_OUT a stage leaves its output with the halo cells filled
-(on all-periodic grids the last sweep writes every output cell's periodic
-images); they must equal what sc_fill_halos writes, bit for bit, corners
-included.  With speed_dev the stage max-accumulates the wave speed of the
+(where the fill is a pure copy along every dim - periodic, or NEUMANN with
+value 0 - the last sweep writes every output cell's images); they must equal
+what sc_fill_halos writes, bit for bit, corners included.  The batched
+stages (sc_batch_rk_stage_ex) are held to the same, and to per-instance
+single-grid stepping.  With speed_dev the stage max-accumulates the wave speed of the
 cells it writes; it must equal sc_max_wave_speed on the output bit for bit
 (both take the same per-cell function and a maximum, which is exact in any
 order).  A Solver stepping through chained stages must reproduce the plain
@@ -15,6 +17,7 @@ state and dt, bit for bit.  The oracle pins the wave speed itself
 """
 
 import copy
+import ctypes
 
 import numpy as np
 import pytest
@@ -23,6 +26,7 @@ import torch
 from oracle import swirlc_oracle as O
 from swirlc_amd import _native as N
 from swirlc_amd import configs
+from swirlc_amd.core.batch import BatchSolver
 from swirlc_amd.core.parameter import SwirlCParameters
 from swirlc_amd.core.solver import Solver
 from swirlc_amd.equations import compressible_navier_stokes as cns
@@ -34,12 +38,32 @@ from test_gpu_scalars import _cfg as scalar_cfg
 pytestmark = pytest.mark.gpu
 
 
+NAMES = ('rho', 'rho_u', 'rho_v', 'rho_w', 'rho_e')
+
+
+def _copy_bcs(name):
+  """BC sets whose halo fill is a pure copy along every dim (or, for
+  'neumann', one that is not: nonzero NEUMANN increments)."""
+  BC = configs.BC
+  per, neu0 = {0: (BC.PERIODIC,), 1: (BC.PERIODIC,)}, {0: (BC.NEUMANN, 0.0), 1: (BC.NEUMANN, 0.0)}
+  if name == 'neumann0':
+    dims = {'x': neu0, 'y': neu0, 'z': neu0}
+  elif name == 'copy-mixed':
+    dims = {'x': per, 'y': neu0, 'z': neu0}
+  elif name == 'copy-mixed-2':
+    dims = {'x': neu0, 'y': per, 'z': per}
+  else:
+    return None
+  return {'cell_averages': {v: dims for v in NAMES}}
+
+
 def _ctx_cfg(n, h, gravity, bcs, scheme='WENO_5', flux='HLLC', visc=True, nscalar=0):
   if nscalar:
     return scalar_cfg(n, scheme, flux, visc, bcs, nscalar=nscalar)
   cfg = grid_cfg(n, h, (1.0, 1.0, 0.05) if gravity else (1.0, 1.0, 1.0))
+  bc = _copy_bcs(bcs) or _bcs(bcs, n, h)
   cfg.update({'interpolation_scheme': scheme, 'numeric_flux_scheme': flux,
-              'include_diffusion': visc, 'bc': _bcs(bcs, n, h),
+              'include_diffusion': visc, 'bc': bc,
               'transport_parameters': {'nu': 2e-3, 'pr': 0.7}, 'dt': 2e-3})
   if gravity:
     cfg.update({'g': {'x': 0.0, 'y': 0.0, 'z': -1.0}, 'source_functions': ['gravity']})
@@ -97,12 +121,17 @@ def test_cfl_dt_resets_the_maximum():
   assert speed[0].item() == 0.0
 
 
-@pytest.mark.parametrize('n,bcs', [
-    ((70, 13, 11), 'periodic'),   # images written by the last sweep
-    ((9, 12, 6), 'periodic'),     # an extent below 2h: fill after the sweeps
-    ((14, 11, 9), 'mixed'),       # not periodic: fill after the sweeps
+@pytest.mark.parametrize('n,bcs,want_fused', [
+    ((70, 13, 11), 'periodic', True),    # images written by the last sweep
+    ((9, 12, 5), 'periodic', False),     # an extent below 2h: fill after the sweeps
+    ((14, 11, 9), 'mixed', False),       # Dirichlet faces: fill after the sweeps
+    ((14, 11, 9), 'neumann', False),     # nonzero NEUMANN increments: fill after
+    # NEUMANN 0 (clamped images, taken by the batched stages only): the fill
+    # after the sweeps.
+    ((14, 11, 9), 'neumann0', False),
+    ((70, 13, 11), 'copy-mixed', False),
 ])
-def test_stage_output_halos_equal_the_fill(n, bcs):
+def test_stage_output_halos_equal_the_fill(n, bcs, want_fused):
   h = 3
   cfg = _ctx_cfg(n, h, False, bcs)
   st = random_state(n, h, seed=23)
@@ -111,7 +140,7 @@ def test_stage_output_halos_equal_the_fill(n, bcs):
   u0 = [t.clone() for t in q]
   dt = torch.full((1,), 1e-3, device='cuda')
   fused = ctx.out_images_fused()
-  assert fused == (bcs == 'periodic' and min(n) >= 2 * h)
+  assert fused == want_fused
   for form, c0, c1 in ((N.SC_RK_EULER, 0.0, 0.0), (N.SC_RK_BLEND, 1.0 / 3.0, 2.0 / 3.0)):
     out = [torch.full_like(t, float('nan')) for t in q]
     ctx.rk_stage_ex(form, c0, c1, dt, u0, q, out, N.SC_STAGE_FILL_OUT)
@@ -149,13 +178,16 @@ def test_stage_ex_rejects_unknown_flags():
 
 @pytest.mark.parametrize('use_graph', [True, False])
 @pytest.mark.parametrize('cfl', [0.5, None])
-def test_chained_solver_matches_plain_stepping(use_graph, cfl):
-  cfg_d, st = configs.kh(n=32)
+@pytest.mark.parametrize('case', ['kh', 'sod'])
+def test_chained_solver_matches_plain_stepping(use_graph, cfl, case):
+  # KH: all-periodic (wrapped images); Sod: NEUMANN 0 (the fill after the
+  # sweeps of each stage).
+  cfg_d, st = configs.kh(n=32) if case == 'kh' else configs.sod(64)
   cfg = SwirlCParameters(copy.deepcopy(cfg_d))
   gpu = {k: torch.from_numpy(v).cuda() for k, v in st.items()}
   chained = Solver(cfg, gpu, cfl=cfl, use_graph=use_graph)
   plain = Solver(cfg, gpu, cfl=cfl, use_graph=use_graph)
-  assert chained._chained and chained.ctx.out_images_fused()
+  assert chained._chained and chained.ctx.out_images_fused() == (case == 'kh')
   plain._chained = False  # halo fill at every stage, separate wave-speed pass
   for _ in range(3):
     chained.step(2)
@@ -166,4 +198,77 @@ def test_chained_solver_matches_plain_stepping(use_graph, cfl):
     sl = (slice(h, -h),) * 3
     for a, b in zip(chained.u, plain.u):
       assert torch.equal(a[sl], b[sl])
-  print(f'[chain] cfl {cfl}: dt after 6 steps {chained.dt.item()!r}', flush=True)
+  print(f'[chain] {case} cfl {cfl}: dt after 6 steps {chained.dt.item()!r}', flush=True)
+
+
+# The first grids' rows span one, two and three 64-lane waves; (12, 1, 7) has
+# a one-cell extent (both clamped sides from one cell), (9, 2, 6) fewer y
+# cells than h.
+BATCH_SHAPES = {
+    'neumann0': [(16, 12, 10), (9, 70, 14), (20, 150, 33), (12, 1, 7), (9, 2, 6),
+                 (40, 6, 7)],
+    'copy-mixed': [(16, 12, 10), (9, 70, 14), (20, 150, 33), (12, 12, 12), (40, 6, 7)],
+    'copy-mixed-2': [(16, 12, 10), (9, 70, 14), (20, 8, 33), (12, 12, 12), (40, 6, 7)],
+}
+
+
+@pytest.mark.parametrize('bcs', ['neumann0', 'copy-mixed', 'copy-mixed-2'])
+@pytest.mark.parametrize('visc', [False, True])
+def test_batch_chained_stages(bcs, visc):
+  """Batched chained stages (sc_batch_rk_stage_ex, halos written by the z
+  sweep as images) against the batched stages that fill the halos at every
+  stage, against per-instance single-grid stepping, and the state's halos
+  against the batched fill."""
+  h = 3
+  shapes = BATCH_SHAPES[bcs]
+  members = []
+  for k, n in enumerate(shapes):
+    cfg = _ctx_cfg(n, h, False, bcs, visc=visc)
+    members.append(((*n, cfg['dx'], cfg['dy'], cfg['dz']), random_state(n, h, seed=40 + k), cfg))
+  inst = [(g, st) for g, st, _ in members]
+  chained = BatchSolver(SwirlCParameters(copy.deepcopy(members[0][2])), inst, cfl=0.5)
+  plain = BatchSolver(SwirlCParameters(copy.deepcopy(members[0][2])), inst, cfl=0.5)
+  assert chained._chained and chained.out_images_fused()
+  plain._chained = False
+  sl = (slice(h, -h),) * 3
+  for _ in range(2):
+    chained.step(2)
+    plain.step(2)
+    torch.cuda.synchronize()
+    assert torch.equal(chained.dt, plain.dt)
+    for k in range(len(shapes)):
+      a, b = chained.state(k), plain.state(k)
+      for name in a:
+        assert torch.equal(a[name][sl], b[name][sl]), (k, name)
+  # The chained state's halos hold the fill of its interior, corners included.
+  ref = [t.clone() for t in chained.u]
+  N.check(N.lib().sc_batch_fill_halos(chained._b, N.ptr_array(ref), N.stream_handle()),
+          'sc_batch_fill_halos')
+  torch.cuda.synchronize()
+  for a, b in zip(chained.u, ref):
+    assert torch.equal(a, b)
+  dts = chained.dt.cpu().numpy()
+  for k, (g, st, cfg) in enumerate(members):
+    solo = Solver(SwirlCParameters(copy.deepcopy(cfg)),
+                  {n: torch.from_numpy(v).cuda() for n, v in st.items()}, cfl=0.5,
+                  use_graph=False)
+    solo.step(4)
+    torch.cuda.synchronize()
+    assert float(solo.dt[0]) == float(dts[k]), k
+    got = chained.state(k)
+    for name, t in solo.state.items():
+      assert torch.equal(got[name][sl], t[sl]), (k, name)
+  print(f'[batch chain] {bcs} visc {visc}: {len(shapes)} instances, dt {dts}', flush=True)
+
+
+def test_batch_stage_ex_rejects_unknown_flags():
+  n, h = (8, 8, 8), 3
+  cfg = _ctx_cfg(n, h, False, 'neumann0')
+  batch = BatchSolver(SwirlCParameters(copy.deepcopy(cfg)),
+                      [((*n, cfg['dx'], cfg['dy'], cfg['dz']), random_state(n, h, seed=3))],
+                      cfl=None)
+  rc = N.lib().sc_batch_rk_stage_ex(batch._b, N.SC_RK_EULER, 0.0, 0.0,
+                                    ctypes.c_void_p(batch.dt.data_ptr()), None,
+                                    N.ptr_array(batch.u), N.ptr_array(batch.s1), 4,
+                                    N.stream_handle())
+  assert rc != 0
