@@ -435,11 +435,18 @@ sc_status sc_batch_rk_stage(sc_batch* batch, int form, float c0, float c1,
  * value 0 on both faces, the same for every field, no BC on a primitive -
  * the z sweep writes each output cell's images, the halo cells the fill
  * would copy from it (the wrapped or the nearest interior cell along each
- * dim); otherwise sc_batch_fill_halos runs on out. */
+ * dim); otherwise sc_batch_fill_halos runs on out.  speed_dev (or NULL): one
+ * float per instance in device memory into which the z sweep max-accumulates
+ * the wave speed of the instance's output cells (as sc_batch_max_wave_speed
+ * computes it, bit for bit; clear it with sc_batch_cfl_dt or to 0 first). */
 sc_status sc_batch_rk_stage_ex(sc_batch* batch, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0,
                                float* const* uk, float* const* out, int flags,
-                               void* stream);
+                               float* speed_dev, void* stream);
+/* sc_cfl_dt per instance: dt_dev[i] = cfl * min(dx_i, dy_i, dz_i) /
+ * speed_dev[i], then speed_dev[i] cleared to 0.  No reference counterpart. */
+sc_status sc_batch_cfl_dt(sc_batch* batch, float* speed_dev, float cfl, float* dt_dev,
+                          void* stream);
 /* Tuning of the batched sweeps (sc_tuning above).  Setting it rebuilds the
  * tile lists (allocates; not allowed during graph capture). */
 sc_status sc_batch_get_tuning(const sc_batch* batch, sc_tuning* out);

@@ -1480,7 +1480,7 @@ sc_status sc_batch_fill_halos(sc_batch* b, float* const* q, void* stream) {
 
 sc_status sc_batch_rk_stage_ex(sc_batch* b, int form, float c0, float c1,
                                const float* dt_dev, const float* const* u0, float* const* uk,
-                               float* const* out, int flags, void* stream) {
+                               float* const* out, int flags, float* speed_dev, void* stream) {
   if (b == nullptr || uk == nullptr || out == nullptr || dt_dev == nullptr)
     return fail(SC_ERR_INVALID, "sc_batch_rk_stage_ex: null argument");
   if (form != SC_RK_EULER && form != SC_RK_BLEND)
@@ -1519,6 +1519,7 @@ sc_status sc_batch_rk_stage_ex(sc_batch* b, int form, float c0, float c1,
   }
   ba.inst = b->inst_dev;
   ba.dt = dt_dev;
+  ba.speed = (unsigned*)speed_dev;
   hipError_t e;
   const int flux = b->params.numeric_flux;
   const bool visc = b->params.include_diffusion != 0;
@@ -1539,7 +1540,16 @@ sc_status sc_batch_rk_stage_ex(sc_batch* b, int form, float c0, float c1,
 sc_status sc_batch_rk_stage(sc_batch* b, int form, float c0, float c1, const float* dt_dev,
                             const float* const* u0, float* const* uk, float* const* out,
                             void* stream) {
-  return sc_batch_rk_stage_ex(b, form, c0, c1, dt_dev, u0, uk, out, 0, stream);
+  return sc_batch_rk_stage_ex(b, form, c0, c1, dt_dev, u0, uk, out, 0, nullptr, stream);
+}
+
+sc_status sc_batch_cfl_dt(sc_batch* b, float* speed_dev, float cfl, float* dt_dev,
+                          void* stream) {
+  if (b == nullptr || speed_dev == nullptr || dt_dev == nullptr)
+    return fail(SC_ERR_INVALID, "sc_batch_cfl_dt: null argument");
+  return hip_check(sc::launch_batch_cfl_dt_reset((unsigned*)speed_dev, b->n_inst, cfl,
+                                                 b->hmin_dev, dt_dev, (hipStream_t)stream),
+                   "batch cfl dt kernel");
 }
 
 sc_status sc_batch_max_wave_speed(sc_batch* b, const float* const* q, float cfl,

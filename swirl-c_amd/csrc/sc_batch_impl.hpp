@@ -40,6 +40,9 @@ struct BatchArgs {
   int ntiles;
   const float* dt;  // per-instance time steps
   const float* pe;  // potential tables of every instance (gravity), or null
+  // Per-instance wave-speed maxima (float bits) the z sweep max-accumulates
+  // over the cells it writes, or null.
+  unsigned* speed;
 };
 
 __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int dir) {
@@ -69,6 +72,7 @@ __device__ __forceinline__ SweepArgs tile_args(const BatchArgs& b, int id, int d
     a.vc.inv_two_h[d] = in.inv_two_h[d];
   }
   a.dt = b.dt != nullptr ? b.dt + id : nullptr;
+  a.speed_bits = b.speed != nullptr ? b.speed + id : nullptr;
   if (b.pe != nullptr && in.pe_off >= 0) {
     a.has_pe = 1;
     a.pe_x = b.pe + in.pe_off;

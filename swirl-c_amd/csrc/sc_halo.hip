@@ -338,7 +338,7 @@ batch_max_speed_kernel(CFieldPtrs q_base, const BatchHaloInst* inst, int h,
       const int i = h + (int)((t / ny) % nx);
       const int k = h + (int)(t / ((long long)ny * nx));
       const long long idx = in.off + (long long)k * NX * NY + (long long)i * NY + j;
-      float q[5], w[7];
+      float q[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) q[f] = q_base.p[f][idx];
       float pe = 0.0f;
@@ -346,10 +346,7 @@ batch_max_speed_kernel(CFieldPtrs q_base, const BatchHaloInst* inst, int h,
         const float* px = pe_arena + in.pe_off;
         pe = (px[i] - px[NX + j]) - px[NX + NY + k];
       }
-      cons_to_prim<0>(q, pe, w);
-      const float c = sqrtf(kGamma * w[kP] / w[kRho]);
-      const float s = fmaxf(fmaxf(fabsf(w[kU]) + c, fabsf(w[kV]) + c), fabsf(w[kW]) + c);
-      m = fmaxf(m, s);
+      m = fmaxf(m, wave_speed(q, pe));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
@@ -369,6 +366,17 @@ __global__ void batch_cfl_dt_kernel(const unsigned* bits, int n, float cfl,
                                     const float* hmin, float* dt) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i < n) dt[i] = cfl * hmin[i] / __uint_as_float(bits[i]);
+}
+
+// Per instance: dt from the accumulated maximum, then the maximum cleared for
+// the next accumulation (one thread per instance: its read precedes its write).
+__global__ void batch_cfl_dt_reset_kernel(unsigned* bits, int n, float cfl, const float* hmin,
+                                          float* dt) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) {
+    dt[i] = cfl * hmin[i] / __uint_as_float(bits[i]);
+    bits[i] = 0u;
+  }
 }
 
 }  // namespace
@@ -473,6 +481,13 @@ hipError_t launch_batch_max_speed(float* const* q_base, const BatchHaloInst* ins
     hipLaunchKernelGGL(batch_cfl_dt_kernel, dim3((n_inst + 255) / 256), 256, 0, stream, bits,
                        n_inst, cfl, hmin, dt);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_cfl_dt_reset(unsigned* bits, int n_inst, float cfl, const float* hmin,
+                                     float* dt, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_cfl_dt_reset_kernel, dim3((n_inst + 255) / 256), 256, 0, stream,
+                     bits, n_inst, cfl, hmin, dt);
   return hipGetLastError();
 }
 

@@ -91,12 +91,17 @@ struct SpeedArgs {
   const float* pe_z;
 };
 
-// *max_bits = bits of max over interior of max_d(|u_d| + c); then
-// *dt = cfl * hmin / max if dt is not null.
 // dt = cfl * hmin / (max speed held in max_bits), then max_bits = 0.
 hipError_t launch_cfl_dt_reset(unsigned* max_bits, float cfl, float hmin, float* dt,
                                hipStream_t stream);
 
+// The same per instance: dt[i] = cfl * hmin[i] / (bits[i] as a float), then
+// bits[i] = 0.
+hipError_t launch_batch_cfl_dt_reset(unsigned* bits, int n_inst, float cfl, const float* hmin,
+                                     float* dt, hipStream_t stream);
+
+// *max_bits = bits of max over interior of max_d(|u_d| + c); then
+// *dt = cfl * hmin / max if dt is not null.
 hipError_t launch_max_speed(const SpeedArgs& a, unsigned* max_bits, float cfl,
                             float hmin, float* dt, hipStream_t stream);
 

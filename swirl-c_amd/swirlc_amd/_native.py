@@ -35,7 +35,7 @@ EXPORTED = (
     'sc_shear_stress', 'sc_heat_flux', 'sc_exchange_halos', 'sc_fill_plane',
     'sc_face_bc_elementwise', 'sc_last_error',
     'sc_version', 'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
-    'sc_rk_stage_ex', 'sc_cfl_dt', 'sc_batch_rk_stage_ex',
+    'sc_rk_stage_ex', 'sc_cfl_dt', 'sc_batch_rk_stage_ex', 'sc_batch_cfl_dt',
 )
 
 
@@ -166,7 +166,8 @@ def lib():
     L.sc_batch_destroy.argtypes = [vp]
     L.sc_batch_fill_halos.argtypes = [vp, fpp, vp]
     L.sc_batch_rk_stage.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, vp]
-    L.sc_batch_rk_stage_ex.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, i, vp]
+    L.sc_batch_rk_stage_ex.argtypes = [vp, i, f, f, vp, fpp, fpp, fpp, i, vp, vp]
+    L.sc_batch_cfl_dt.argtypes = [vp, vp, f, vp, vp]
     L.sc_batch_max_wave_speed.argtypes = [vp, fpp, f, vp, vp, vp]
     L.sc_ctx_set_reference_state.argtypes = [vp, vp, vp]
     L.sc_ctx_get_tuning.argtypes = [vp, ctypes.POINTER(ScTuning)]
@@ -188,7 +189,7 @@ def lib():
                  'sc_linear_interp', 'sc_riemann_flux', 'sc_shear_stress', 'sc_heat_flux',
                  'sc_exchange_halos', 'sc_fill_plane', 'sc_face_bc_elementwise',
                  'sc_rk_stage_noalloc', 'sc_sweep_only_timed', 'sc_fill_plane_values',
-                 'sc_rk_stage_ex', 'sc_cfl_dt', 'sc_batch_rk_stage_ex'):
+                 'sc_rk_stage_ex', 'sc_cfl_dt', 'sc_batch_rk_stage_ex', 'sc_batch_cfl_dt'):
       getattr(L, name).restype = ctypes.c_int
     _lib = L
   return _lib

@@ -157,34 +157,49 @@ class BatchSolver:
                                   for s in self.shapes))
 
   def refresh(self):
-    """Fills the halos of every instance's state (the chained stepping keeps
-    them current from step to step; call this after writing to the state)."""
+    """Fills the halos of every instance's state and seeds the per-instance
+    CFL maxima from it (the chained stepping keeps both current from step to
+    step; call this after writing to the state)."""
     N.check(N.lib().sc_batch_fill_halos(self._b, N.ptr_array(self.u), N.stream_handle()),
             'sc_batch_fill_halos')
+    if self.cfl is not None:
+      self._max_wave_speed()
     self._seeded = True
 
-  def _stage(self, form, c0, c1, u0, uk, out):
+  def _max_wave_speed(self):
+    N.check(N.lib().sc_batch_max_wave_speed(
+        self._b, N.ptr_array(self.u), ctypes.c_float(self.cfl),
+        ctypes.c_void_p(self.speed.data_ptr()), ctypes.c_void_p(self.dt.data_ptr()),
+        N.stream_handle()), 'sc_batch_max_wave_speed')
+
+  def _stage(self, form, c0, c1, u0, uk, out, speed=False):
     u0p = N.ptr_array(u0) if u0 is not None else None
     dt = ctypes.c_void_p(self.dt.data_ptr())
     if self._chained:
       flags = N.SC_STAGE_HALOS_READY | N.SC_STAGE_FILL_OUT
+      sp = ctypes.c_void_p(self.speed.data_ptr()) if speed else None
       N.check(N.lib().sc_batch_rk_stage_ex(
           self._b, form, ctypes.c_float(c0), ctypes.c_float(c1), dt, u0p, N.ptr_array(uk),
-          N.ptr_array(out), flags, N.stream_handle()), 'sc_batch_rk_stage_ex')
+          N.ptr_array(out), flags, sp, N.stream_handle()), 'sc_batch_rk_stage_ex')
     else:
       N.check(N.lib().sc_batch_rk_stage(
           self._b, form, ctypes.c_float(c0), ctypes.c_float(c1), dt, u0p, N.ptr_array(uk),
           N.ptr_array(out), N.stream_handle()), 'sc_batch_rk_stage')
 
   def _step_body(self):
-    if self.cfl is not None:
-      N.check(N.lib().sc_batch_max_wave_speed(
-          self._b, N.ptr_array(self.u), ctypes.c_float(self.cfl),
-          ctypes.c_void_p(self.speed.data_ptr()), ctypes.c_void_p(self.dt.data_ptr()),
-          N.stream_handle()), 'sc_batch_max_wave_speed')
+    cfl = self.cfl is not None
+    if cfl:
+      if self._chained:
+        # dt from the maxima the previous step's last stage accumulated (or
+        # refresh seeded), which are then cleared for this step's.
+        N.check(N.lib().sc_batch_cfl_dt(
+            self._b, ctypes.c_void_p(self.speed.data_ptr()), ctypes.c_float(self.cfl),
+            ctypes.c_void_p(self.dt.data_ptr()), N.stream_handle()), 'sc_batch_cfl_dt')
+      else:
+        self._max_wave_speed()
     self._stage(N.SC_RK_EULER, 0.0, 0.0, None, self.u, self.s1)
     self._stage(N.SC_RK_BLEND, _C11, 1.0 - _C11, self.u, self.s1, self.s2)
-    self._stage(N.SC_RK_BLEND, _C21, 1.0 - _C21, self.u, self.s2, self.u)
+    self._stage(N.SC_RK_BLEND, _C21, 1.0 - _C21, self.u, self.s2, self.u, speed=cfl)
 
   def capture(self):
     side = torch.cuda.Stream()

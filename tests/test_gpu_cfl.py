@@ -269,6 +269,6 @@ def test_batch_stage_ex_rejects_unknown_flags():
                       cfl=None)
   rc = N.lib().sc_batch_rk_stage_ex(batch._b, N.SC_RK_EULER, 0.0, 0.0,
                                     ctypes.c_void_p(batch.dt.data_ptr()), None,
-                                    N.ptr_array(batch.u), N.ptr_array(batch.s1), 4,
+                                    N.ptr_array(batch.u), N.ptr_array(batch.s1), 4, None,
                                     N.stream_handle())
   assert rc != 0
