@@ -1175,7 +1175,8 @@ void xcd_order(std::vector<T>& v) {
 
 // Sweep tile lists of a batch for its current tuning: x sweep (y chunks of
 // 64, z groups, x segments); y sweep (groups of by_y windows of the z
-// plane's run of rows, z); z sweep (y chunks, x groups, z segments); in the
+// plane's run of rows, runs of seg_y z planes); z sweep (y chunks, x groups,
+// z segments); in the
 // XCD order above when the tuning's xcd_remap is set.  Replaces any
 // previous lists.
 sc_status build_sweep_tiles(sc_batch* b) {
@@ -1189,7 +1190,7 @@ sc_status build_sweep_tiles(sc_batch* b) {
       for (int by = 0; by < (g.nz + t.by_x - 1) / t.by_x; ++by)
         for (int bx = 0; bx < (g.ny + 63) / 64; ++bx) tl[0].push_back({i, bx, by, bz});
     const int nwin = sc::y_windows(g.nx, g.ny, b->h, outw);
-    for (int bz = 0; bz < g.nz; ++bz)
+    for (int bz = 0; bz < (g.nz + t.seg_y - 1) / t.seg_y; ++bz)
       for (int bx = 0; bx < (nwin + t.by_y - 1) / t.by_y; ++bx) tl[1].push_back({i, bx, 0, bz});
     for (int bz = 0; bz < (g.nz + t.seg_z - 1) / t.seg_z; ++bz)
       for (int by = 0; by < (g.nx + t.by_z - 1) / t.by_z; ++by)

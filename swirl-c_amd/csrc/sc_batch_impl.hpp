@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) batch_y_kernel(const BatchArgs b) {
     const SweepArgs a = tile_args(b, tl.inst, 1);
     const TileIdx ti{tl.bx, tl.by, tl.bz, (int)threadIdx.x, (int)threadIdx.y,
                      (int)blockDim.y};
-    sweep_y_body<SCHEME, FLUX, VISC, NS, false, FULL, !FULL>(a, ti, 1);
+    sweep_y_body<SCHEME, FLUX, VISC, NS, false, FULL, !FULL>(a, ti, a.seg_y);
   }
 }
 
@@ -129,6 +129,7 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
   }
   {
     BatchArgs by = b;
+    by.proto.seg_y = tune.seg_y;
     by.tiles = tiles.t[1];
     by.ntiles = tiles.n[1];
     hipLaunchKernelGGL((batch_y_kernel<SCHEME, FLUX, VISC, NS, FULL>), dim3(tiles.grid[1]),

@@ -125,14 +125,11 @@ def tune_context(nctx, q, out, dt_dev, u0=None, reps=5, march_space=MARCH_SPACE,
   return best, timings
 
 
-def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=None):
+def tune_batch(batch, reps=3, march_space=MARCH_SPACE, y_space=Y_SPACE):
   """Times one RK3 stage-2 update of the whole ensemble per candidate (the
   other directions held at the base geometry) and keeps the fastest per
-  direction (the batched y sweep runs one plane per wave).  Returns
-  (tuning, [(direction, tuning, ms)])."""
+  direction.  Returns (tuning, [(direction, tuning, ms)])."""
   base = batch.get_tuning()
-  if y_space is None:
-    y_space = tuple((by, base['seg_y']) for by in (1, 2, 4))
   if batch._chained and not batch._seeded:
     batch.refresh()  # the timed stages take the state's halos as filled
   timings = []
