@@ -121,6 +121,7 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
                                  const SweepTuning& tune, hipStream_t stream) {
   {
     BatchArgs bx = b;
+    bx.speed = nullptr;  // the CFL maxima come from the last (z) sweep only
     bx.proto.seg_len = tune.seg_x;
     bx.tiles = tiles.t[0];
     bx.ntiles = tiles.n[0];
@@ -129,6 +130,7 @@ hipError_t launch_batch_sweeps_t(BatchArgs b, const BatchTiles& tiles,
   }
   {
     BatchArgs by = b;
+    by.speed = nullptr;
     by.proto.seg_y = tune.seg_y;
     by.tiles = tiles.t[1];
     by.ntiles = tiles.n[1];
