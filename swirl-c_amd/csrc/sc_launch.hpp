@@ -37,7 +37,7 @@ hipError_t launch_branch_codes(const SweepArgs& a, int dim, int ns,
 // PD: reference-state (p_ref / rho_ref) build; BB: boundary face / flux
 // value build.
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
-          bool LEAN = false>
+          int LEAN = 0>
 hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
                                const SweepTuning& tune, hipStream_t stream,
                                int only_dir) {
@@ -77,14 +77,21 @@ hipError_t launch_rhs_sweeps_t(SweepArgs a, const float h[3],
 }
 
 // Picks the build for the configuration: boundary values (BB) and / or a
-// reference state (PD); without either and without gravity or
-// primitive-variable BCs, the LEAN build.
+// reference state (PD); without either and without primitive-variable BCs,
+// a LEAN build: kLeanPlain without gravity, kLeanGrav with both the
+// potential and the gravity source.
 template <int SCHEME, int FLUX, bool VISC, int NS>
 hipError_t launch_rhs_ns(const SweepArgs& a, const float h[3], const SweepTuning& tune,
                          hipStream_t stream, int od) {
   const bool bb = a.bbc_dims != 0;
-  if (!bb && a.p_ref == nullptr && a.prim_bc_mask == 0 && !a.has_pe && !a.grav_src)
-    return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, false, false, true>(a, h, tune, stream, od);
+  if (!bb && a.p_ref == nullptr && a.prim_bc_mask == 0) {
+    if (!a.has_pe && !a.grav_src)
+      return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, false, false, kLeanPlain>(a, h, tune,
+                                                                                  stream, od);
+    if (a.has_pe && a.grav_src)
+      return launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, false, false, kLeanGrav>(a, h, tune,
+                                                                                 stream, od);
+  }
   if (a.p_ref != nullptr)
     return bb ? launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, true, true>(a, h, tune, stream, od)
               : launch_rhs_sweeps_t<SCHEME, FLUX, VISC, NS, true, false>(a, h, tune, stream, od);

@@ -145,11 +145,16 @@ __device__ __forceinline__ TileIdx tile_of_block(int remap) {
   return TileIdx{bx, by, bz, (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.y};
 }
 
-// LEAN builds (no gravitational potential, no gravity source, no BC given on
-// a primitive) drop those run-time checks from the kernels.
-template <bool LEAN = false>
+// LEAN builds drop run-time checks from the kernels: kLeanPlain (no
+// gravitational potential, no gravity source, no BC given on a primitive) and
+// kLeanGrav (potential and gravity source both on, no BC on a primitive);
+// LEAN = 0 decides all three at run time.
+constexpr int kLeanPlain = 1;
+constexpr int kLeanGrav = 2;
+template <int LEAN = 0>
 __device__ __forceinline__ float pe_at(const SweepArgs& a, int k, int i, int j) {
-  if constexpr (LEAN) return 0.0f;
+  if constexpr (LEAN == kLeanPlain) return 0.0f;
+  if constexpr (LEAN == kLeanGrav) return (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k];
   return a.has_pe ? (a.pe_x[i] - a.pe_y[j]) - a.pe_z[k] : 0.0f;
 }
 
@@ -162,7 +167,7 @@ __device__ __forceinline__ void load_raw(const SweepArgs& a, long long idx, floa
   for (int f = 0; f < NF; ++f) q[f] = a.q.p[f][idx];
 }
 
-template <int NF, bool LEAN = false>
+template <int NF, int LEAN = 0>
 __device__ __forceinline__ void prim_bc_round(const SweepArgs& a, bool interior, float* q) {
   if constexpr (LEAN) return;
   if (a.prim_bc_mask && interior) {
@@ -172,7 +177,7 @@ __device__ __forceinline__ void prim_bc_round(const SweepArgs& a, bool interior,
   }
 }
 
-template <int NF, bool LEAN = false>
+template <int NF, int LEAN = 0>
 __device__ __forceinline__ void load_cons(const SweepArgs& a, long long idx,
                                           bool interior, float* q) {
   load_raw<NF>(a, idx, q);
@@ -187,7 +192,7 @@ __device__ __forceinline__ void set_pdyn(const SweepArgs& a, long long idx, bool
   if constexpr (PD) w[7 + NS] = interior ? w[kP] - a.p_ref[idx] : 0.0f;
 }
 
-template <int NS, bool PD = false, bool LEAN = false>
+template <int NS, bool PD = false, int LEAN = 0>
 __device__ __forceinline__ void load_prim(const SweepArgs& a, long long idx,
                                           bool interior, float pe, float* w) {
   constexpr int NF = 5 + NS;
@@ -198,7 +203,7 @@ __device__ __forceinline__ void load_prim(const SweepArgs& a, long long idx,
 }
 
 // Velocity components u_c0 and u_c1 at a cell (for transverse derivatives).
-template <bool LEAN = false>
+template <int LEAN = 0>
 __device__ __forceinline__ void load_vel2(const SweepArgs& a, long long idx,
                                           bool interior, int c0, int c1,
                                           float& v0, float& v1) {
@@ -221,7 +226,7 @@ __device__ __forceinline__ bool in_core(int c, int h, int n) {
 // Central transverse derivatives at cell (k, i, j) for a face normal to D:
 // out[t][0] = d(u_t)/dx_t, out[t][1] = d(u_D)/dx_t for the transverse dims t
 // in x, y, z order (physics/fluid.py:52-63 with numerics/gradient.py:71-94).
-template <int D, bool LEAN = false>
+template <int D, int LEAN = 0>
 __device__ __forceinline__ void transverse_grads(const SweepArgs& a, int k, int i,
                                                  int j, float (*out)[2]) {
   int t = 0;
@@ -336,10 +341,12 @@ __device__ __forceinline__ void load_raw_u(const SweepArgs& a, long long u, unsi
 }
 
 // Potential at (k, i, jb + lane) with k and i uniform.
-template <bool LEAN = false>
+template <int LEAN = 0>
 __device__ __forceinline__ float pe_at_u(const SweepArgs& a, int k, int i, int jb,
                                          unsigned lb0) {
-  if constexpr (LEAN) return 0.0f;
+  if constexpr (LEAN == kLeanPlain) return 0.0f;
+  if constexpr (LEAN == kLeanGrav)
+    return (a.pe_x[i] - ld_plane(a.pe_y + jb, lane_off(lb0))) - a.pe_z[k];
   return a.has_pe ? (a.pe_x[i] - ld_plane(a.pe_y + jb, lane_off(lb0))) - a.pe_z[k] : 0.0f;
 }
 
@@ -378,7 +385,7 @@ __device__ __forceinline__ void transverse_load_u(const SweepArgs& a, long long 
   }
 }
 
-template <int D, bool LEAN = false>
+template <int D, int LEAN = 0>
 __device__ __forceinline__ void transverse_grads_from(const SweepArgs& a, int k, int i, int j,
                                                       const TransRaw& r, float (*out)[2]) {
   int t = 0;
@@ -412,10 +419,11 @@ __device__ __forceinline__ void transverse_grads_from(const SweepArgs& a, int k,
 
 // Gravity source of field f at a cell (physics_models.py:94-143,
 // gravity.py:64-102): (rho * G) * g_d on the selected momenta, else 0.
-template <bool LEAN = false>
+template <int LEAN = 0>
 __device__ __forceinline__ float source_of(const SweepArgs& a, int f, float rho) {
-  if constexpr (LEAN) return 0.0f;
-  if (a.grav_src && f >= 1 && f <= 3 && (a.grav_mask & (1 << (f - 1)))) {
+  if constexpr (LEAN == kLeanPlain) return 0.0f;
+  if ((LEAN == kLeanGrav || a.grav_src) && f >= 1 && f <= 3 &&
+      (a.grav_mask & (1 << (f - 1)))) {
     return (rho * kG) * a.gdir[f - 1];
   }
   return 0.0f;
@@ -655,7 +663,7 @@ __device__ __forceinline__ void store_out(const SweepArgs& a, long long u, unsig
 // BB: build with the boundary face / flux value handling (only launched when
 // the configuration has such values, so other runs carry none of its code).
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
-          bool BB = false, bool LEAN = false, bool CLAMP = false>
+          bool BB = false, int LEAN = 0, bool CLAMP = false>
 __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileIdx& ti) {
   constexpr int RAD = Radius<SCHEME>::value;
   constexpr int WN = 2 * RAD + 1;
@@ -853,7 +861,7 @@ __device__ __forceinline__ void sweep_march_body(const SweepArgs& a, const TileI
 }
 
 template <int DIR, int SCHEME, int FLUX, bool VISC, int NS, int PHASE, bool PD = false,
-          bool BB = false, bool LEAN = false>
+          bool BB = false, int LEAN = 0>
 __global__ void __launch_bounds__(256)
 sweep_march_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
@@ -961,7 +969,7 @@ __device__ __forceinline__ void y_load(const SweepArgs& a, int k, const YPos& p,
 
 // Velocity components (u_c0, u_c1) from a loaded (rho, m_c0, m_c1); cells of
 // fields with a primitive BC get the reference's rho * (q / rho) rounding.
-template <bool LEAN>
+template <int LEAN>
 __device__ __forceinline__ void vel2_of(const SweepArgs& a, float rho, float m0, float m1,
                                         bool interior, int c0, int c1, float& v0, float& v1) {
   if (!LEAN && a.prim_bc_mask && interior) {
@@ -975,7 +983,7 @@ __device__ __forceinline__ void vel2_of(const SweepArgs& a, float rho, float m0,
 
 // One window of the y sweep at plane k from its loaded operands (zp, zm:
 // rho, rho_w, rho_v of planes k + 1 and k - 1): R - div_y into R.
-template <int SCHEME, int FLUX, bool VISC, int NS, bool PD, bool BB, bool LEAN>
+template <int SCHEME, int FLUX, bool VISC, int NS, bool PD, bool BB, int LEAN>
 __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& p,
                                          const YLoad<5 + NS, VISC, PD>& L, const float* zp,
                                          const float* zm) {
@@ -1070,7 +1078,7 @@ __device__ __forceinline__ void y_window(const SweepArgs& a, int k, const YPos& 
 // z neighbours of the viscous transverse derivatives are the planes before
 // and after, already in registers.
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
-          bool LEAN = false>
+          int LEAN = 0>
 __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& ti, int zseg) {
   constexpr int OUTW = YGeom<SCHEME>::OUTW;
   constexpr int NF = 5 + NS;
@@ -1114,7 +1122,7 @@ __device__ __forceinline__ void sweep_y_body(const SweepArgs& a, const TileIdx& 
 }
 
 template <int SCHEME, int FLUX, bool VISC, int NS, bool PD = false, bool BB = false,
-          bool LEAN = false>
+          int LEAN = 0>
 __global__ void __launch_bounds__(256)
 sweep_y_kernel(const SweepArgs a) {
   const TileIdx ti = tile_of_block(a.xcd_remap);
