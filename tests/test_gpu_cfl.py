@@ -130,10 +130,14 @@ def test_cfl_dt_resets_the_maximum():
     # after the sweeps.
     ((14, 11, 9), 'neumann0', False),
     ((70, 13, 11), 'copy-mixed', False),
+    ((33, 9, 4), 'periodic-h2', True),    # halo width 2 (MUSCL): extents >= 4
 ])
 def test_stage_output_halos_equal_the_fill(n, bcs, want_fused):
-  h = 3
-  cfg = _ctx_cfg(n, h, False, bcs)
+  h = 2 if bcs.endswith('-h2') else 3
+  if h == 2:
+    cfg = _ctx_cfg(n, h, False, 'periodic', scheme='MUSCL', flux='HLL')
+  else:
+    cfg = _ctx_cfg(n, h, False, bcs)
   st = random_state(n, h, seed=23)
   ctx = cns.get_rhs_fn(None, SwirlCParameters(copy.deepcopy(cfg))).native_context
   q = ctx.fields({k: torch.from_numpy(v.copy()).cuda() for k, v in st.items()})
@@ -212,18 +216,22 @@ BATCH_SHAPES = {
 }
 
 
-@pytest.mark.parametrize('bcs', ['neumann0', 'copy-mixed', 'copy-mixed-2'])
-@pytest.mark.parametrize('visc', [False, True])
-def test_batch_chained_stages(bcs, visc):
+@pytest.mark.parametrize('bcs,visc,scheme,h', [
+    ('neumann0', False, 'WENO_5', 3), ('neumann0', True, 'WENO_5', 3),
+    ('copy-mixed', False, 'WENO_5', 3), ('copy-mixed', True, 'WENO_5', 3),
+    ('copy-mixed-2', False, 'WENO_5', 3), ('copy-mixed-2', True, 'WENO_5', 3),
+    ('neumann0', True, 'MUSCL', 2), ('copy-mixed', False, 'WENO_3', 2),
+])
+def test_batch_chained_stages(bcs, visc, scheme, h):
   """Batched chained stages (sc_batch_rk_stage_ex, halos written by the z
   sweep as images) against the batched stages that fill the halos at every
   stage, against per-instance single-grid stepping, and the state's halos
   against the batched fill."""
-  h = 3
   shapes = BATCH_SHAPES[bcs]
+  flux = 'HLL' if scheme == 'MUSCL' else 'HLLC'
   members = []
   for k, n in enumerate(shapes):
-    cfg = _ctx_cfg(n, h, False, bcs, visc=visc)
+    cfg = _ctx_cfg(n, h, False, bcs, scheme=scheme, flux=flux, visc=visc)
     members.append(((*n, cfg['dx'], cfg['dy'], cfg['dz']), random_state(n, h, seed=40 + k), cfg))
   inst = [(g, st) for g, st, _ in members]
   chained = BatchSolver(SwirlCParameters(copy.deepcopy(members[0][2])), inst, cfl=0.5)
@@ -258,7 +266,8 @@ def test_batch_chained_stages(bcs, visc):
     got = chained.state(k)
     for name, t in solo.state.items():
       assert torch.equal(got[name][sl], t[sl]), (k, name)
-  print(f'[batch chain] {bcs} visc {visc}: {len(shapes)} instances, dt {dts}', flush=True)
+  print(f'[batch chain] {bcs} visc {visc} {scheme} h={h}: {len(shapes)} instances, dt {dts}',
+        flush=True)
 
 
 def test_batch_stage_ex_rejects_unknown_flags():
